@@ -1,0 +1,367 @@
+"""ctypes bindings of libgpart.so (include/gpart.h) and libgpart_hip.so (include/gpart_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+WIDTHS = {"standard": 0, "big": 1}
+SPECTRAL_MODES = {"plain_fp64": 0, "batched_fp64": 1, "batched_fp32": 2}
+PROFILE_FAMILIES = {
+    "sclp_affinity": 0,
+    "contract": 1,
+    "refine_cache": 2,
+    "refine_proposals": 3,
+    "refine_filter": 4,
+    "spectral": 5,
+}
+
+
+class GpartError(RuntimeError):
+    """A failing status from the C ABI. kind mirrors the reference's exception class."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+        self.kind = {1: "invalid_argument", 2: "overflow_error", 3: "runtime_error"}.get(code, "error")
+
+
+class PartitionReport(C.Structure):
+    _fields_ = [("final_cut", C.c_uint64), ("max_part_weight", C.c_uint64), ("capacity", C.c_uint64),
+                ("initial_cut", C.c_uint64), ("coarsest_vertices", C.c_int64), ("levels", C.c_int32),
+                ("refined_levels", C.c_int32), ("coarsen_seconds", C.c_double),
+                ("initial_seconds", C.c_double), ("uncoarsen_seconds", C.c_double),
+                ("free_seconds", C.c_double), ("algorithm_seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class RefineReport(C.Structure):
+    _fields_ = [("rounds", C.c_int32), ("admission_iterations", C.c_int32),
+                ("admission_fallbacks", C.c_int32), ("pad", C.c_int32), ("proposals", C.c_uint64),
+                ("accepted", C.c_uint64), ("initial_cut", C.c_uint64), ("final_cut", C.c_uint64),
+                ("final_max_part_weight", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
+_hip = None
+_host = None
+
+
+def lib_dir() -> Path:
+    return _LIB_DIR
+
+
+def hip_lib():
+    """libgpart_hip.so: the kernel library."""
+    global _hip
+    if _hip is None:
+        path = _LIB_DIR / "libgpart_hip.so"
+        if not path.exists():
+            raise ImportError(f"{path} is not built; run __graft_entry__.build()")
+        H = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+        H.gpart_last_error.restype = C.c_char_p
+        H.gpart_device_count.argtypes = [C.POINTER(C.c_int)]
+        H.gpart_record_layout.argtypes = [C.c_int64, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_uint64)]
+        H.gpart_record_layout.restype = None
+        H.gpart_profile_enable.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        H.gpart_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_double)]
+        H.gpart_profile_reset.argtypes = [C.c_void_p]
+        H.gpart_device_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
+        H.gpart_device_free.argtypes = [C.c_void_p, C.c_void_p]
+        H.gpart_memcpy_htod.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        H.gpart_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        H.gpart_ctx_synchronize.argtypes = [C.c_void_p]
+        _hip = H
+    return _hip
+
+
+def host_lib():
+    """libgpart.so: C++ scheduling and the high-level C ABI."""
+    global _host
+    if _host is None:
+        hip_lib()
+        path = _LIB_DIR / "libgpart.so"
+        if not path.exists():
+            raise ImportError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(path))
+        L.gpart_partition_capacity_c.argtypes = [C.c_uint64, C.c_int, C.c_double, C.POINTER(C.c_uint64)]
+        L.gpart_thread_context.restype = C.c_void_p
+        L.gpart_graph_upload.argtypes = [C.c_int, C.c_int64, _i64p, C.c_int64, _i64p, C.c_void_p, C.c_void_p,
+                                         C.c_int, C.POINTER(C.c_void_p)]
+        L.gpart_graph_load.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.gpart_graph_free.argtypes = [C.c_void_p]
+        L.gpart_graph_vertices.restype = C.c_int64
+        L.gpart_graph_vertices.argtypes = [C.c_void_p]
+        L.gpart_graph_edges.restype = C.c_int64
+        L.gpart_graph_edges.argtypes = [C.c_void_p]
+        L.gpart_graph_cut.argtypes = [C.c_void_p, _i32p, C.POINTER(C.c_uint64)]
+        L.gpart_graph_partition.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_int, C.c_int,
+                                            C.c_void_p, C.POINTER(PartitionReport)]
+        L.gpart_graph_coarsen.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_double, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, C.POINTER(C.c_void_p)]
+        L.gpart_hierarchy_read.argtypes = [C.c_int, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.gpart_hierarchy_levels.argtypes = [C.c_void_p]
+        L.gpart_hierarchy_level_size.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.gpart_hierarchy_level_copy.argtypes = [C.c_void_p, C.c_int, _i64p, _i32p, _u64p, _u64p]
+        L.gpart_hierarchy_map_copy.argtypes = [C.c_void_p, C.c_int, _i32p]
+        L.gpart_hierarchy_stop_reason.restype = C.c_char_p
+        L.gpart_hierarchy_stop_reason.argtypes = [C.c_void_p]
+        L.gpart_hierarchy_write.argtypes = [C.c_void_p, C.c_char_p]
+        L.gpart_hierarchy_free.argtypes = [C.c_void_p]
+        L.gpart_hierarchy_uncoarsen.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_double, C.c_int, C.c_double,
+                                                C.c_int, _i32p, C.POINTER(C.c_uint64)]
+        L.gpart_graph_initial_partition.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_int, _i32p,
+                                                    C.c_void_p]
+        L.gpart_graph_refine.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(RefineReport)]
+        for name in ("gpart_generate_grid2d", "gpart_generate_mesh3d", "gpart_generate_rmat"):
+            getattr(L, name).restype = C.c_void_p
+        L.gpart_generate_grid2d.argtypes = [C.c_int64, C.c_int64]
+        L.gpart_generate_mesh3d.argtypes = [C.c_int64, C.c_int64, C.c_int64]
+        L.gpart_generate_rmat.argtypes = [C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64]
+        L.gpart_host_csr_vertices.restype = C.c_int64
+        L.gpart_host_csr_vertices.argtypes = [C.c_void_p]
+        L.gpart_host_csr_entries.restype = C.c_int64
+        L.gpart_host_csr_entries.argtypes = [C.c_void_p]
+        L.gpart_host_csr_indptr.restype = C.POINTER(C.c_int64)
+        L.gpart_host_csr_indptr.argtypes = [C.c_void_p]
+        L.gpart_host_csr_indices.restype = C.POINTER(C.c_int64)
+        L.gpart_host_csr_indices.argtypes = [C.c_void_p]
+        L.gpart_host_csr_write.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
+        L.gpart_host_csr_free.argtypes = [C.c_void_p]
+        L.gpart_trim.argtypes = []
+        _host = L
+    return _host
+
+
+def _check(rc: int) -> None:
+    if rc:
+        raise GpartError(rc, hip_lib().gpart_last_error().decode())
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = hip_lib().gpart_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def partition_capacity(total: int, parts: int, ratio: float) -> int:
+    out = C.c_uint64()
+    _check(host_lib().gpart_partition_capacity_c(total, parts, ratio, C.byref(out)))
+    return out.value
+
+
+def record_layout(vertices: int, bound: int):
+    vb, wb, mask = C.c_int(), C.c_int(), C.c_uint64()
+    hip_lib().gpart_record_layout(vertices, bound, C.byref(vb), C.byref(wb), C.byref(mask))
+    return vb.value, wb.value, mask.value
+
+
+def trim() -> None:
+    _check(host_lib().gpart_trim())
+
+
+def _ctx():
+    ctx = host_lib().gpart_thread_context()
+    if not ctx:
+        raise GpartError(3, hip_lib().gpart_last_error().decode())
+    return ctx
+
+
+def profile_enable(family: str, enable: bool = True) -> None:
+    _check(hip_lib().gpart_profile_enable(_ctx(), PROFILE_FAMILIES[family], int(enable)))
+
+
+def profile_read(family: str) -> dict:
+    ms, launches, nbytes = C.c_double(), C.c_int64(), C.c_double()
+    _check(hip_lib().gpart_profile_read(_ctx(), PROFILE_FAMILIES[family], C.byref(ms), C.byref(launches),
+                                        C.byref(nbytes)))
+    return {"total_ms": ms.value, "launches": launches.value, "bytes": nbytes.value}
+
+
+def profile_reset() -> None:
+    _check(hip_lib().gpart_profile_reset(_ctx()))
+
+
+def _take_csr(handle):
+    L = host_lib()
+    if not handle:
+        raise GpartError(1, "graph generation rejected its arguments")
+    try:
+        n = L.gpart_host_csr_vertices(handle)
+        m = L.gpart_host_csr_entries(handle)
+        indptr = np.ctypeslib.as_array(L.gpart_host_csr_indptr(handle), shape=(n + 1,)).copy()
+        indices = (np.ctypeslib.as_array(L.gpart_host_csr_indices(handle), shape=(m,)).copy()
+                   if m else np.zeros(0, np.int64))
+    finally:
+        L.gpart_host_csr_free(handle)
+    return indptr, indices
+
+
+def generate_grid2d(rows: int, cols: int):
+    return _take_csr(host_lib().gpart_generate_grid2d(rows, cols))
+
+
+def generate_mesh3d(nx: int, ny: int, nz: int):
+    return _take_csr(host_lib().gpart_generate_mesh3d(nx, ny, nz))
+
+
+def generate_rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+                  seed: int = 0):
+    return _take_csr(host_lib().gpart_generate_rmat(scale, edge_factor, a, b, c, seed))
+
+
+class Hierarchy:
+    """Host copy of a coarsening hierarchy (levels, fine-to-coarse maps)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _host is not None:
+            _host.gpart_hierarchy_free(self._h)
+            self._h = None
+
+    @classmethod
+    def read(cls, path, width: str = "standard") -> "Hierarchy":
+        h = C.c_void_p()
+        _check(host_lib().gpart_hierarchy_read(WIDTHS[width], str(path).encode(), C.byref(h)))
+        return cls(h.value)
+
+    def __len__(self) -> int:
+        return int(host_lib().gpart_hierarchy_levels(self._h))
+
+    def level(self, i: int):
+        n, m = C.c_int64(), C.c_int64()
+        _check(host_lib().gpart_hierarchy_level_size(self._h, i, C.byref(n), C.byref(m)))
+        off = np.empty(n.value + 1, np.int64)
+        adj = np.empty(m.value, np.int32)
+        ew = np.empty(m.value, np.uint64)
+        vw = np.empty(n.value, np.uint64)
+        _check(host_lib().gpart_hierarchy_level_copy(self._h, i, off, adj, ew, vw))
+        return off, adj, ew, vw
+
+    def map(self, i: int):
+        n = C.c_int64()
+        m = C.c_int64()
+        _check(host_lib().gpart_hierarchy_level_size(self._h, i, C.byref(n), C.byref(m)))
+        out = np.empty(n.value, np.int32)
+        _check(host_lib().gpart_hierarchy_map_copy(self._h, i, out))
+        return out
+
+    @property
+    def stop_reason(self) -> str:
+        return host_lib().gpart_hierarchy_stop_reason(self._h).decode()
+
+    def write(self, path) -> None:
+        _check(host_lib().gpart_hierarchy_write(self._h, str(path).encode()))
+
+    def uncoarsen(self, coarsest_labels, parts: int, ratio: float = 1.03, max_rounds: int = 32,
+                  growth: float = 2.0, verify: bool = False):
+        lab = np.ascontiguousarray(coarsest_labels, dtype=np.int32)
+        n0 = self.level(0)[0].shape[0] - 1
+        out = np.empty(n0, np.int32)
+        cut = C.c_uint64()
+        _check(host_lib().gpart_hierarchy_uncoarsen(self._h, lab, parts, ratio, max_rounds, growth, int(verify),
+                                                    out, C.byref(cut)))
+        return out, cut.value
+
+
+class Graph:
+    """A validated input graph resident on the GPU (plus its host copy)."""
+
+    def __init__(self, indptr, indices, width: str = "standard", edge_weights=None, vertex_weights=None,
+                 validate: bool = False):
+        L = host_lib()
+        off = np.ascontiguousarray(indptr, dtype=np.int64)
+        adj = np.ascontiguousarray(indices, dtype=np.int64)
+        self._keep = []
+        ew = vw = None
+        if edge_weights is not None:
+            ew = np.ascontiguousarray(edge_weights, dtype=np.uint64)
+            self._keep.append(ew)
+        if vertex_weights is not None:
+            vw = np.ascontiguousarray(vertex_weights, dtype=np.uint64)
+            self._keep.append(vw)
+        h = C.c_void_p()
+        _check(L.gpart_graph_upload(WIDTHS[width], len(off) - 1, off, len(adj), adj,
+                                    ew.ctypes.data if ew is not None else None,
+                                    vw.ctypes.data if vw is not None else None, int(validate), C.byref(h)))
+        self._h = h.value
+        self.width = width
+
+    @classmethod
+    def load(cls, indptr_path, indices_path, width: str = "standard", validate: bool = False) -> "Graph":
+        g = cls.__new__(cls)
+        h = C.c_void_p()
+        _check(host_lib().gpart_graph_load(WIDTHS[width], str(indptr_path).encode(), str(indices_path).encode(),
+                                           int(validate), C.byref(h)))
+        g._h = h.value
+        g._keep = []
+        g.width = width
+        return g
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _host is not None:
+            _host.gpart_graph_free(self._h)
+            self._h = None
+
+    @property
+    def n(self) -> int:
+        return int(host_lib().gpart_graph_vertices(self._h))
+
+    @property
+    def m(self) -> int:
+        return int(host_lib().gpart_graph_edges(self._h))
+
+    def cut(self, labels) -> int:
+        out = C.c_uint64()
+        _check(host_lib().gpart_graph_cut(self._h, np.ascontiguousarray(labels, dtype=np.int32), C.byref(out)))
+        return out.value
+
+    def partition(self, parts: int, ratio: float = 1.03, seed: int = 0, verify: bool = False,
+                  spectral_mode: str = "batched_fp64", want_labels: bool = True):
+        labels = np.empty(self.n, np.int32) if want_labels else None
+        rep = PartitionReport()
+        _check(host_lib().gpart_graph_partition(self._h, parts, ratio, seed, int(verify),
+                                                SPECTRAL_MODES[spectral_mode],
+                                                labels.ctypes.data if labels is not None else None,
+                                                C.byref(rep)))
+        return labels, rep.as_dict()
+
+    def coarsen(self, parts: int, seed: int = 0, stop_ratio: float = 0.90, max_levels: int = 24,
+                cluster_size_factor: int = 256, pack_isolated: bool = False, verify: bool = False) -> Hierarchy:
+        h = C.c_void_p()
+        _check(host_lib().gpart_graph_coarsen(self._h, parts, seed, stop_ratio, max_levels, cluster_size_factor,
+                                              int(pack_isolated), int(verify), C.byref(h)))
+        return Hierarchy(h.value)
+
+    def initial_partition(self, parts: int, ratio: float = 1.03, seed: int = 0,
+                          spectral_mode: str = "batched_fp64", with_coords: bool = False):
+        labels = np.empty(self.n, np.int32)
+        coords = np.empty(self.n, np.float64) if with_coords else None
+        _check(host_lib().gpart_graph_initial_partition(self._h, parts, ratio, seed, SPECTRAL_MODES[spectral_mode],
+                                                        labels, coords.ctypes.data if coords is not None else None))
+        return (labels, coords) if with_coords else labels
+
+    def refine(self, labels, parts: int, ratio: float = 1.03, max_rounds: int = 32, stale_rounds: int = 3,
+               verify: bool = False):
+        lab = np.array(labels, dtype=np.int32, copy=True)
+        rep = RefineReport()
+        _check(host_lib().gpart_graph_refine(self._h, lab, parts, ratio, max_rounds, stale_rounds, int(verify),
+                                             C.byref(rep)))
+        return lab, rep.as_dict()

@@ -1,0 +1,1007 @@
+// coarsen.hip -- SCLP coarsening kernels for gfx950 (src/coarsen.cu of the reference).
+//
+// Per LP round every vertex needs, over its neighbour clusters t != cluster(v):
+//   conn[t] = sum of w(v,u), u != v, cluster(u) == t
+//   the lexicographic maximum of (conn[t], tie(v,t)) among role-admissible t
+// plus current = conn[cluster(v)]. These are order-independent, so the kernels
+// below choose their own thread mapping per degree class:
+//   low    (deg <= 8)    8 lanes per vertex, duplicates merged with 8 shuffles
+//   medium (deg <= 256)  one wave per vertex, 512-slot LDS open-addressing table
+//   high   (deg > 256)   one workgroup per vertex, 4096-slot LDS table, or a
+//                        per-hub table in global memory above 2048 entries
+// and fuse the reference's decode step (sclp_decode_gain_kernel) into the
+// reduction, writing proposal/gain/tie directly.
+#include "internal.hpp"
+#include "scan.hpp"
+
+#include <algorithm>
+
+namespace gpart_hip {
+namespace {
+
+constexpr int kLowDegree = 8;
+constexpr int kMediumDegree = 256;
+constexpr int kMediumSlots = 512;
+constexpr int kHighLdsDegree = 2048;
+constexpr int kHighSlots = 4096;
+constexpr int kBlock = 256;
+
+struct RoundParams {
+    std::uint32_t role_salt, tie_salt, threshold;
+    bool filter;
+};
+
+__device__ __forceinline__ bool is_mover(std::int32_t cluster, const RoundParams& r) {
+    return mix32(static_cast<std::uint32_t>(cluster) ^ r.role_salt) < r.threshold;
+}
+
+__device__ __forceinline__ unsigned long long tie_key(std::int32_t v, std::int32_t t, std::uint32_t salt) {
+    const auto hash = mix32(static_cast<std::uint32_t>(v) ^ mix32(static_cast<std::uint32_t>(t)) ^ salt);
+    return (static_cast<unsigned long long>(hash) << 32) | (0xffffffffULL - static_cast<std::uint32_t>(t));
+}
+
+// sclp_better_candidate: affinity 0 never wins (src/coarsen.cu:224-229)
+__device__ __forceinline__ bool better(unsigned long long a, unsigned long long at, unsigned long long b,
+                                      unsigned long long bt) {
+    return a > b || (a == b && a != 0 && at > bt);
+}
+
+template <int G>
+__device__ __forceinline__ void group_best(unsigned long long& conn, unsigned long long& tie) {
+#pragma unroll
+    for (int m = G / 2; m > 0; m /= 2) {
+        const auto oc = shfl_xor_u64(conn, m, G);
+        const auto ot = shfl_xor_u64(tie, m, G);
+        if (better(oc, ot, conn, tie)) {
+            conn = oc;
+            tie = ot;
+        }
+    }
+}
+
+// Candidate of one distinct neighbour cluster t with connection conn.
+__device__ __forceinline__ void offer(std::int32_t v, std::int32_t t, std::int32_t source, unsigned long long conn,
+                                      const RoundParams& r, unsigned long long& current,
+                                      unsigned long long& best, unsigned long long& best_tie) {
+    if (t == source) {
+        current = conn > current ? conn : current;
+        return;
+    }
+    if (conn == 0 || (r.filter && is_mover(t, r))) return;
+    const auto tie = tie_key(v, t, r.tie_salt);
+    if (better(conn, tie, best, best_tie)) {
+        best = conn;
+        best_tie = tie;
+    }
+}
+
+// sclp_decode_gain_kernel (src/coarsen.cu:562-598)
+__device__ __forceinline__ void decode(std::int32_t v, std::int32_t source, unsigned long long current,
+                                       unsigned long long best, unsigned long long best_tie, const RoundParams& r,
+                                       std::int32_t* proposals, unsigned long long* gains, std::uint32_t* ties) {
+    const bool can_move = !r.filter || is_mover(source, r);
+    std::int32_t p = kSclpInvalid;
+    unsigned long long g = 0;
+    std::uint32_t t32 = 0;
+    if (can_move && best > current) {
+        const auto target = static_cast<std::int32_t>(0xffffffffULL - (best_tie & 0xffffffffULL));
+        if (target >= 0 && target != source) {
+            p = target;
+            g = best - current;
+            t32 = static_cast<std::uint32_t>(best_tie >> 32);
+        }
+    }
+    proposals[v] = p;
+    gains[v] = g;
+    ties[v] = t32;
+}
+
+// ---------------------------------------------------------------- classes
+
+template <typename O>
+__global__ void degree_class_kernel(std::int64_t n, const O* offsets, std::int32_t* low, std::int32_t* medium,
+                                    std::int32_t* high, unsigned long long* counters) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = v < n;
+    const std::int64_t deg = valid ? static_cast<std::int64_t>(offsets[v + 1] - offsets[v]) : 0;
+    const int cls = !valid ? -1 : deg <= kLowDegree ? 0 : deg <= kMediumDegree ? 1 : 2;
+    std::int32_t* lists[3] = {low, medium, high};
+    const std::uint64_t below = lanemask_lt();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const std::uint64_t mask = __ballot(cls == c);
+        if (!mask) continue;
+        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        unsigned long long base = 0;
+        if (lane_id() == leader) base = atomicAdd(counters + c, static_cast<unsigned long long>(__popcll(mask)));
+        base = shfl_u64(base, leader, kWave);
+        if (cls == c) lists[c][base + __popcll(mask & below)] = static_cast<std::int32_t>(v);
+    }
+    unsigned long long hub_edges = cls == 2 ? static_cast<unsigned long long>(deg) : 0;
+    unsigned long long max_deg = static_cast<unsigned long long>(deg);
+    hub_edges = group_sum_u64<kWave>(hub_edges);
+    max_deg = group_max_u64<kWave>(max_deg);
+    if (lane_id() == 0) {
+        if (hub_edges) atomicAdd(counters + 3, hub_edges);
+        atomicMax(counters + 4, max_deg);
+    }
+}
+
+// ---------------------------------------------------------------- affinity
+
+template <typename O, typename W, bool Singleton>
+__global__ void __launch_bounds__(kBlock) sclp_low_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
+                                                          const std::int32_t* neighbors, const W* weights,
+                                                          const std::int32_t* clusters, RoundParams r,
+                                                          std::int32_t* proposals, unsigned long long* gains,
+                                                          std::uint32_t* ties) {
+    constexpr int G = kLowDegree;
+    const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+    const int sub = threadIdx.x & (G - 1);
+    if (group >= count) return;  // uniform per group
+    const std::int32_t v = list[group];
+    const std::int64_t begin = offsets[v];
+    const int deg = static_cast<int>(offsets[v + 1] - begin);
+    bool valid = sub < deg;
+    std::int32_t t = -1;
+    unsigned long long w = 0;
+    if (valid) {
+        const std::int32_t u = neighbors[begin + sub];
+        valid = u != v;
+        if (valid) {
+            t = Singleton ? u : clusters[u];
+            w = static_cast<unsigned long long>(weights[begin + sub]);
+        }
+    }
+    unsigned long long conn = w;
+    if constexpr (!Singleton) {
+        conn = 0;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const std::int32_t tj = __shfl(t, j, G);
+            const auto wj = shfl_u64(w, j, G);
+            if (valid && tj == t) conn += wj;
+        }
+    }
+    const std::int32_t source = clusters[v];
+    unsigned long long current = 0, best = 0, best_tie = 0;
+    if (valid) offer(v, t, source, conn, r, current, best, best_tie);
+    current = group_max_u64<G>(current);
+    group_best<G>(best, best_tie);
+    if (sub == 0) decode(v, source, current, best, best_tie, r, proposals, gains, ties);
+}
+
+template <typename O, typename W, bool Singleton>
+__global__ void __launch_bounds__(kBlock) sclp_medium_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
+                                                             const std::int32_t* neighbors, const W* weights,
+                                                             const std::int32_t* clusters, RoundParams r,
+                                                             std::int32_t* proposals, unsigned long long* gains,
+                                                             std::uint32_t* ties) {
+    constexpr int kWavesPerBlock = kBlock / kWave;
+    __shared__ std::int32_t table_keys[Singleton ? 1 : kWavesPerBlock * kMediumSlots];
+    __shared__ unsigned long long table_values[Singleton ? 1 : kWavesPerBlock * kMediumSlots];
+    const int wave = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const std::int64_t index = static_cast<std::int64_t>(blockIdx.x) * kWavesPerBlock + wave;
+    const bool active = index < count;
+    const std::int32_t v = active ? list[index] : 0;
+    const std::int64_t begin = active ? static_cast<std::int64_t>(offsets[v]) : 0;
+    const std::int64_t end = active ? static_cast<std::int64_t>(offsets[v + 1]) : 0;
+    const std::int32_t source = active ? clusters[v] : 0;
+    unsigned long long current = 0, best = 0, best_tie = 0;
+    if constexpr (Singleton) {
+        for (std::int64_t e = begin + lane; e < end; e += kWave) {
+            const std::int32_t u = neighbors[e];
+            if (u == v) continue;
+            offer(v, u, source, static_cast<unsigned long long>(weights[e]), r, current, best, best_tie);
+        }
+    } else {
+        std::int32_t* keys = table_keys + wave * kMediumSlots;
+        unsigned long long* values = table_values + wave * kMediumSlots;
+        for (int s = lane; s < kMediumSlots; s += kWave) {
+            keys[s] = -1;
+            values[s] = 0;
+        }
+        __syncthreads();
+        for (std::int64_t e = begin + lane; e < end; e += kWave) {
+            const std::int32_t u = neighbors[e];
+            if (u == v) continue;
+            const std::int32_t t = clusters[u];
+            int slot = static_cast<int>(mix32(static_cast<std::uint32_t>(t))) & (kMediumSlots - 1);
+            while (true) {
+                const std::int32_t found = atomicCAS(keys + slot, -1, t);
+                if (found == -1 || found == t) {
+                    atomicAdd(values + slot, static_cast<unsigned long long>(weights[e]));
+                    break;
+                }
+                slot = (slot + 1) & (kMediumSlots - 1);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            for (int s = lane; s < kMediumSlots; s += kWave) {
+                const std::int32_t t = keys[s];
+                if (t < 0) continue;
+                offer(v, t, source, values[s], r, current, best, best_tie);
+            }
+        }
+    }
+    current = group_max_u64<kWave>(current);
+    group_best<kWave>(best, best_tie);
+    if (active && lane == 0) decode(v, source, current, best, best_tie, r, proposals, gains, ties);
+}
+
+// Workgroup per hub. Tables: LDS when deg <= kHighLdsDegree, else the hub's
+// region of a global table (table_offset[i] slots, power of two).
+template <typename O, typename W, bool Singleton>
+__global__ void __launch_bounds__(kBlock) sclp_high_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
+                                                           const std::int32_t* neighbors, const W* weights,
+                                                           const std::int32_t* clusters, RoundParams r,
+                                                           const std::int64_t* table_offset,
+                                                           std::int32_t* global_keys,
+                                                           unsigned long long* global_values,
+                                                           std::int32_t* proposals, unsigned long long* gains,
+                                                           std::uint32_t* ties) {
+    __shared__ std::int32_t lds_keys[Singleton ? 1 : kHighSlots];
+    __shared__ unsigned long long lds_values[Singleton ? 1 : kHighSlots];
+    __shared__ unsigned long long red_current[kBlock / kWave], red_best[kBlock / kWave], red_tie[kBlock / kWave];
+    const std::int64_t index = blockIdx.x;
+    if (index >= count) return;  // uniform per block
+    const std::int32_t v = list[index];
+    const std::int64_t begin = offsets[v];
+    const std::int64_t end = offsets[v + 1];
+    const std::int32_t source = clusters[v];
+    unsigned long long current = 0, best = 0, best_tie = 0;
+    if constexpr (Singleton) {
+        for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
+            const std::int32_t u = neighbors[e];
+            if (u == v) continue;
+            offer(v, u, source, static_cast<unsigned long long>(weights[e]), r, current, best, best_tie);
+        }
+    } else {
+        const bool in_lds = end - begin <= kHighLdsDegree;
+        std::int64_t slots = kHighSlots;
+        std::int32_t* gkeys = nullptr;
+        unsigned long long* gvalues = nullptr;
+        if (!in_lds) {
+            const std::int64_t off = table_offset[index];
+            slots = table_offset[index + 1] - off;
+            gkeys = global_keys + off;
+            gvalues = global_values + off;
+        }
+        const std::int64_t slot_mask = slots - 1;
+        if (in_lds) {
+            for (int s = threadIdx.x; s < kHighSlots; s += kBlock) {
+                lds_keys[s] = -1;
+                lds_values[s] = 0;
+            }
+        }
+        __syncthreads();
+        for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
+            const std::int32_t u = neighbors[e];
+            if (u == v) continue;
+            const std::int32_t t = clusters[u];
+            std::int64_t slot = static_cast<std::int64_t>(mix32(static_cast<std::uint32_t>(t))) & slot_mask;
+            const auto w = static_cast<unsigned long long>(weights[e]);
+            if (in_lds) {
+                while (true) {
+                    const std::int32_t found = atomicCAS(lds_keys + slot, -1, t);
+                    if (found == -1 || found == t) {
+                        atomicAdd(lds_values + slot, w);
+                        break;
+                    }
+                    slot = (slot + 1) & slot_mask;
+                }
+            } else {
+                while (true) {
+                    const std::int32_t found = atomicCAS(gkeys + slot, -1, t);
+                    if (found == -1 || found == t) {
+                        atomicAdd(gvalues + slot, w);
+                        break;
+                    }
+                    slot = (slot + 1) & slot_mask;
+                }
+            }
+        }
+        if (!in_lds) __threadfence();
+        __syncthreads();
+        for (std::int64_t s = threadIdx.x; s < slots; s += kBlock) {
+            const std::int32_t t = in_lds ? lds_keys[s] : __hip_atomic_load(gkeys + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t < 0) continue;
+            const unsigned long long c = in_lds ? lds_values[s]
+                                                : __hip_atomic_load(gvalues + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            offer(v, t, source, c, r, current, best, best_tie);
+        }
+    }
+    current = group_max_u64<kWave>(current);
+    group_best<kWave>(best, best_tie);
+    const int wave = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        red_current[wave] = current;
+        red_best[wave] = best;
+        red_tie[wave] = best_tie;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / kWave; ++w) {
+            current = red_current[w] > current ? red_current[w] : current;
+            if (better(red_best[w], red_tie[w], best, best_tie)) {
+                best = red_best[w];
+                best_tie = red_tie[w];
+            }
+        }
+        decode(v, source, current, best, best_tie, r, proposals, gains, ties);
+    }
+}
+
+template <typename O>
+__global__ void hub_table_size_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
+                                      std::int64_t* sizes) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const std::int32_t v = list[i];
+    const std::int64_t deg = offsets[v + 1] - offsets[v];
+    std::int64_t size = 0;
+    if (deg > kHighLdsDegree) {
+        size = 1;
+        while (size < 2 * deg) size <<= 1;
+    }
+    sizes[i] = size;
+}
+
+// ---------------------------------------------------------------- admission
+
+__global__ void valid_flags_kernel(std::int64_t n, const std::int32_t* targets, std::int32_t* flags) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v < n) flags[v] = targets[v] != kSclpInvalid;
+}
+
+// Key components of the admission order (target asc, gain desc, tie desc, id asc).
+enum KeyStage { kTieOnly = 0, kGainTie = 1, kGainOnly = 2, kTargetOnly = 3, kAll = 4 };
+
+__global__ void admission_key_kernel(std::int64_t p, const std::int32_t* order, const std::int32_t* targets,
+                                     const unsigned long long* gains, const std::uint32_t* ties, int stage,
+                                     int gain_bits, std::uint64_t* keys) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= p) return;
+    const std::int32_t v = order[i];
+    const std::uint64_t gmask = gain_bits >= 64 ? ~0ULL : ((1ULL << gain_bits) - 1);
+    const std::uint64_t rgain = gmask - (gains[v] & gmask);  // descending gain
+    const std::uint64_t rtie = static_cast<std::uint32_t>(~ties[v]);
+    const std::uint64_t target = static_cast<std::uint32_t>(targets[v]);
+    std::uint64_t key = 0;
+    switch (stage) {
+        case kTieOnly: key = rtie; break;
+        case kGainTie: key = (rgain << 32) | rtie; break;
+        case kGainOnly: key = rgain; break;
+        case kTargetOnly: key = target; break;
+        default: key = (((target << gain_bits) | rgain) << 32) | rtie; break;
+    }
+    keys[i] = key;
+}
+
+__global__ void gather_order_kernel(std::int64_t p, const std::int32_t* order, const std::int32_t* targets,
+                                    const std::uint64_t* vertex_weights, std::int32_t* ordered_targets,
+                                    std::uint64_t* ordered_weights) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= p) return;
+    const std::int32_t v = order[i];
+    ordered_targets[i] = targets[v];
+    ordered_weights[i] = vertex_weights[v];
+}
+
+struct SegWeightLoad {
+    const std::int32_t* targets;
+    const std::uint64_t* weights;
+    __device__ __forceinline__ SegValue operator()(std::int64_t i) const {
+        return SegValue{weights[i], static_cast<unsigned>(i == 0 || targets[i] != targets[i - 1])};
+    }
+};
+struct SegOneLoad {
+    const std::int32_t* targets;
+    __device__ __forceinline__ SegValue operator()(std::int64_t i) const {
+        return SegValue{1, static_cast<unsigned>(i == 0 || targets[i] != targets[i - 1])};
+    }
+};
+struct SegInclusiveStore {
+    std::uint64_t* out;
+    __device__ __forceinline__ void operator()(std::int64_t i, const SegValue& prefix, const SegValue& item) const {
+        out[i] = SegOp{}(prefix, item).value;
+    }
+};
+
+// sclp_commit_kernel (src/coarsen.cu:652-680), split into a decision pass that
+// reads the pre-commit cluster weights and an apply pass with atomics.
+__global__ void admit_decide_kernel(std::int64_t p, const std::int32_t* ordered_targets,
+                                    const std::uint64_t* prefix, const unsigned long long* cluster_weights,
+                                    std::uint64_t capacity, std::int32_t* accept) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= p) return;
+    const std::uint64_t base = cluster_weights[ordered_targets[i]];
+    accept[i] = base <= capacity && prefix[i] <= capacity - base;
+}
+
+__global__ void admit_apply_kernel(std::int64_t p, const std::int32_t* order, const std::int32_t* ordered_targets,
+                                   const std::uint64_t* ordered_weights, const std::int32_t* accept,
+                                   std::int32_t* clusters, unsigned long long* cluster_weights,
+                                   unsigned long long* accepted) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool ok = false;
+    if (i < p && accept[i]) {
+        ok = true;
+        const std::int32_t v = order[i];
+        const std::int32_t t = ordered_targets[i];
+        const unsigned long long w = ordered_weights[i];
+        atomicAdd(cluster_weights + clusters[v], 0ULL - w);
+        atomicAdd(cluster_weights + t, w);
+        clusters[v] = t;
+    }
+    const unsigned long long c = static_cast<unsigned long long>(__popcll(__ballot(ok)));
+    if (lane_id() == 0 && c) atomicAdd(accepted, c);
+}
+
+__global__ void count_nonzero_kernel(std::int64_t n, const unsigned long long* values, unsigned long long* count) {
+    __shared__ unsigned long long waves[kBlock / kWave];
+    unsigned long long local = 0;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
+        local += values[i] != 0;
+    local = group_sum_u64<kWave>(local);
+    if (lane_id() == 0) waves[threadIdx.x / kWave] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long total = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) total += waves[w];
+        if (total) atomicAdd(count, total);
+    }
+}
+
+// ---------------------------------------------------------------- isolated / two-hop
+
+template <typename O>
+__global__ void zero_degree_flags_kernel(std::int64_t n, const O* offsets, std::int32_t* flags) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v < n) flags[v] = offsets[v] == offsets[v + 1];
+}
+
+struct GatherWeightLoad {
+    const std::int32_t* list;
+    const std::uint64_t* weights;
+    __device__ __forceinline__ std::uint64_t operator()(std::int64_t i) const { return weights[list[i]]; }
+};
+
+// isolated_representative_kernel / isolated_commit_kernel (src/coarsen.cu:741-773)
+__global__ void isolated_rep_kernel(std::int64_t count, const std::int32_t* vertices, const std::uint64_t* prefix,
+                                    const std::uint64_t* weights, std::uint64_t capacity, std::int32_t* reps) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const std::int32_t v = vertices[i];
+    const std::uint64_t first = (prefix[i] - weights[v]) / capacity;
+    const std::uint64_t last = (prefix[i] - 1) / capacity;
+    if (first == last) atomicMin(reps + first, v);
+}
+
+__global__ void isolated_commit_kernel(std::int64_t count, const std::int32_t* vertices, const std::uint64_t* prefix,
+                                       const std::uint64_t* weights, std::uint64_t capacity,
+                                       const std::int32_t* reps, std::int32_t* clusters,
+                                       unsigned long long* cluster_weights) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const std::int32_t v = vertices[i];
+    const std::uint64_t first = (prefix[i] - weights[v]) / capacity;
+    const std::uint64_t last = (prefix[i] - 1) / capacity;
+    if (first != last) return;
+    const std::int32_t target = reps[first];
+    if (target == v) return;
+    clusters[v] = target;
+    cluster_weights[v] = 0;
+    atomicAdd(cluster_weights + target, static_cast<unsigned long long>(weights[v]));
+}
+
+__global__ void fill_i32_kernel(std::int32_t* out, std::int32_t value, std::int64_t n) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = value;
+}
+
+// sclp_singleton_favorite_kernel (src/coarsen.cu:682-692)
+__global__ void favorite_kernel(std::int64_t n, const std::uint64_t* vertex_weights,
+                                const unsigned long long* cluster_weights, const std::int32_t* clusters,
+                                const std::int32_t* proposals, std::int32_t* favorites) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    favorites[v] = cluster_weights[clusters[v]] == vertex_weights[v] ? proposals[v] : kSclpInvalid;
+}
+
+// sclp_pair_flags_kernel (:694-704)
+__global__ void pair_flags_kernel(std::int64_t count, const std::uint64_t* ordered_weights,
+                                  const std::uint64_t* ranks, std::uint64_t capacity, std::int64_t* pair_flags) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const bool second = (ranks[i] & 1ULL) == 0ULL;
+    pair_flags[i] = second && i > 0 && ordered_weights[i - 1] <= capacity - ordered_weights[i] ? 1 : 0;
+}
+
+// sclp_two_hop_pair_commit_kernel (:706-724)
+__global__ void two_hop_commit_kernel(std::int64_t count, const std::int32_t* order, const std::int64_t* pair_flags,
+                                      const std::int64_t* positions, std::uint64_t budget,
+                                      const std::uint64_t* ordered_weights, std::int32_t* clusters,
+                                      unsigned long long* cluster_weights, unsigned long long* merged) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count || !pair_flags[i] || static_cast<std::uint64_t>(positions[i]) >= budget) return;
+    const std::int32_t v = order[i];
+    const std::int32_t source = clusters[v];
+    const std::int32_t target = clusters[order[i - 1]];
+    if (source == target) return;
+    const unsigned long long w = ordered_weights[i];
+    atomicAdd(cluster_weights + source, 0ULL - w);
+    atomicAdd(cluster_weights + target, w);
+    clusters[v] = target;
+    atomicAdd(merged, 1ULL);
+}
+
+// ---------------------------------------------------------------- compaction
+
+struct NonemptyLoad {
+    const unsigned long long* weights;
+    __device__ __forceinline__ std::int64_t operator()(std::int64_t i) const { return weights[i] > 0 ? 1 : 0; }
+};
+struct CompactIdStore {
+    std::int32_t* ids;
+    __device__ __forceinline__ void operator()(std::int64_t i, std::int64_t prefix, std::int64_t) const {
+        ids[i] = static_cast<std::int32_t>(prefix);
+    }
+};
+
+template <typename W>
+__global__ void map_and_weights_kernel(std::int64_t n, const std::int32_t* clusters, const std::int32_t* compact,
+                                       const W* vertex_weights, std::int32_t* map, W* coarse_weights) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const std::int32_t c = compact[clusters[v]];
+    map[v] = c;
+    if constexpr (sizeof(W) == 4)
+        atomicAdd(reinterpret_cast<unsigned int*>(coarse_weights + c), static_cast<unsigned int>(vertex_weights[v]));
+    else
+        atomicAdd(reinterpret_cast<unsigned long long*>(coarse_weights + c),
+                  static_cast<unsigned long long>(vertex_weights[v]));
+}
+
+template <typename W>
+__global__ void max_weight_kernel(std::int64_t n, const W* weights, unsigned long long* out) {
+    unsigned long long local = 0;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
+        const auto w = static_cast<unsigned long long>(weights[i]);
+        local = w > local ? w : local;
+    }
+    local = group_max_u64<kWave>(local);
+    if (lane_id() == 0 && local) atomicMax(out, local);
+}
+
+// ---------------------------------------------------------------- record bounds
+
+template <typename O, typename W, int G>
+__global__ void record_bound_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
+                                    const std::int32_t* neighbors, const W* weights,
+                                    unsigned long long* row_bounds, unsigned long long* meta) {
+    const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+    const int sub = threadIdx.x & (G - 1);
+    if (group >= count) return;
+    const std::int32_t v = list[group];
+    const std::int64_t begin = offsets[v], end = offsets[v + 1];
+    unsigned long long degree = 0;
+    bool unsorted = false;
+    for (std::int64_t e = begin + sub; e < end; e += G) {
+        const auto w = static_cast<unsigned long long>(weights[e]);
+        degree = ~0ULL - degree < w ? ~0ULL : degree + w;
+        unsorted = unsorted || (e > begin && neighbors[e] <= neighbors[e - 1]);
+    }
+#pragma unroll
+    for (int m = G / 2; m > 0; m /= 2) {
+        const auto other = shfl_xor_u64(degree, m, G);
+        degree = ~0ULL - degree < other ? ~0ULL : degree + other;
+    }
+    unsorted = group_any<G>(unsorted);
+    if (sub == 0) {
+        row_bounds[v] = degree;
+        atomicMax(meta, degree);
+        if (unsorted) atomicOr(meta + 1, 1ULL);
+    }
+}
+
+// contract_record_bounds (src/coarsen.cu:72-85): saturating add through the map.
+__global__ void contract_bounds_kernel(std::int64_t n, const std::int32_t* map, const unsigned long long* fine,
+                                       unsigned long long* coarse) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    unsigned long long* target = coarse + map[v];
+    const unsigned long long amount = fine[v];
+    unsigned long long previous = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (true) {
+        const unsigned long long sum = ~0ULL - previous < amount ? ~0ULL : previous + amount;
+        const unsigned long long observed = atomicCAS(target, previous, sum);
+        if (observed == previous) break;
+        previous = observed;
+    }
+}
+
+__global__ void identity_list_kernel(std::int64_t n, std::int32_t* list) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v < n) list[v] = static_cast<std::int32_t>(v);
+}
+
+inline unsigned grid_for(std::int64_t threads) { return static_cast<unsigned>((threads + kBlock - 1) / kBlock); }
+
+// Sort the valid-proposal list `order` (p entries, vertex ids ascending) into
+// admission order. Returns the buffer holding the sorted order (order or alt).
+std::int32_t* admission_sort(gpart_ctx* ctx, std::int64_t n, std::int64_t p, std::int32_t* order,
+                             std::int32_t* order_alt, const std::int32_t* targets,
+                             const unsigned long long* gains, const std::uint32_t* ties,
+                             std::uint64_t gain_bound) {
+    if (p <= 1) return order;
+    const int target_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(n - 1)));
+    const int gain_bits = std::max(1, bit_width_u64(gain_bound));
+    Scratch<std::uint64_t> keys(ctx, p), keys_alt(ctx, p);
+    std::int32_t* cur = order;
+    std::int32_t* alt = order_alt;
+    auto stage = [&](int kind, int bits) {
+        hipLaunchKernelGGL(admission_key_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, cur, targets,
+                           gains, ties, kind, gain_bits, keys.get());
+        GPART_LAUNCH_CHECK();
+        const bool swapped = radix_sort_pairs(ctx, keys.get(), keys_alt.get(), reinterpret_cast<std::uint32_t*>(cur),
+                                              reinterpret_cast<std::uint32_t*>(alt), p, 0, bits);
+        if (swapped) std::swap(cur, alt);
+    };
+    if (target_bits + gain_bits + 32 <= 64) {
+        stage(kAll, target_bits + gain_bits + 32);
+    } else if (gain_bits + 32 <= 64) {
+        stage(kGainTie, gain_bits + 32);
+        stage(kTargetOnly, target_bits);
+    } else {
+        stage(kTieOnly, 32);
+        stage(kGainOnly, gain_bits);
+        stage(kTargetOnly, target_bits);
+    }
+    return cur;
+}
+
+unsigned long long count_nonempty(gpart_ctx* ctx, std::int64_t n, const unsigned long long* weights) {
+    Scratch<unsigned long long> c(ctx, 1);
+    fill_bytes(ctx, c.get(), 0, 8);
+    const unsigned blocks = std::min<unsigned>(grid_for(n), 2048);
+    hipLaunchKernelGGL(count_nonzero_kernel, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, weights, c.get());
+    GPART_LAUNCH_CHECK();
+    return read_scalar(ctx, c.get());
+}
+
+template <typename O, typename W>
+void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_t* clusters,
+                      const gpart_degree_classes* c, const gpart_sclp_round* round, std::int32_t* proposals,
+                      unsigned long long* gains, std::uint32_t* ties) {
+    const auto* off = static_cast<const O*>(g->offsets);
+    const auto* w = static_cast<const W*>(g->edge_weights);
+    RoundParams r{round->role_salt, round->tie_salt, round->mover_threshold, round->filter_roles != 0};
+    const bool single = round->singleton != 0;
+    // per-hub global tables for hubs beyond the LDS table
+    Scratch<std::int64_t> table_offset;
+    Scratch<std::int32_t> gkeys;
+    Scratch<unsigned long long> gvalues;
+    if (!single && c->high_count > 0 && c->max_degree > kHighLdsDegree) {
+        table_offset.reset(ctx, c->high_count + 1);
+        fill_bytes(ctx, table_offset.get(), 0, static_cast<std::size_t>(c->high_count + 1) * 8);
+        hipLaunchKernelGGL(hub_table_size_kernel<O>, dim3(grid_for(c->high_count)), dim3(kBlock), 0, ctx->stream,
+                           c->high_count, c->high, off, table_offset.get());
+        GPART_LAUNCH_CHECK();
+        exclusive_scan<std::int64_t>(ctx, table_offset.get(), table_offset.get(), c->high_count + 1);
+        // the last entry (size 0 appended by the scan of count+1 values) holds the total
+        const std::int64_t total = read_scalar(ctx, table_offset.get() + c->high_count);
+        gkeys.reset(ctx, total);
+        gvalues.reset(ctx, total);
+        fill_bytes(ctx, gkeys.get(), 0xff, static_cast<std::size_t>(total) * 4);
+        fill_bytes(ctx, gvalues.get(), 0, static_cast<std::size_t>(total) * 8);
+    }
+    profile_begin(ctx, GPART_PROFILE_SCLP_AFFINITY);
+    if (c->low_count > 0) {
+        const unsigned blocks = grid_for(c->low_count * kLowDegree);
+        if (single)
+            hipLaunchKernelGGL((sclp_low_kernel<O, W, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, c->low_count,
+                               c->low, off, g->neighbors, w, clusters, r, proposals, gains, ties);
+        else
+            hipLaunchKernelGGL((sclp_low_kernel<O, W, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
+                               c->low_count, c->low, off, g->neighbors, w, clusters, r, proposals, gains, ties);
+        GPART_LAUNCH_CHECK();
+    }
+    if (c->medium_count > 0) {
+        const unsigned blocks = static_cast<unsigned>((c->medium_count + 3) / 4);
+        if (single)
+            hipLaunchKernelGGL((sclp_medium_kernel<O, W, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
+                               c->medium_count, c->medium, off, g->neighbors, w, clusters, r, proposals, gains, ties);
+        else
+            hipLaunchKernelGGL((sclp_medium_kernel<O, W, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
+                               c->medium_count, c->medium, off, g->neighbors, w, clusters, r, proposals, gains, ties);
+        GPART_LAUNCH_CHECK();
+    }
+    if (c->high_count > 0) {
+        const unsigned blocks = static_cast<unsigned>(c->high_count);
+        if (single)
+            hipLaunchKernelGGL((sclp_high_kernel<O, W, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
+                               c->high_count, c->high, off, g->neighbors, w, clusters, r, table_offset.get(),
+                               gkeys.get(), gvalues.get(), proposals, gains, ties);
+        else
+            hipLaunchKernelGGL((sclp_high_kernel<O, W, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
+                               c->high_count, c->high, off, g->neighbors, w, clusters, r, table_offset.get(),
+                               gkeys.get(), gvalues.get(), proposals, gains, ties);
+        GPART_LAUNCH_CHECK();
+    }
+    const double n = static_cast<double>(g->n), m = static_cast<double>(g->m);
+    const double csr = (n + 1) * sizeof(O) + m * (4 + sizeof(W));
+    profile_end(ctx, GPART_PROFILE_SCLP_AFFINITY, csr + (single ? 0.0 : 4 * m) + 20 * n);
+}
+
+template <typename O, typename W>
+void build_classes(gpart_ctx* ctx, const gpart_csr_view* g, gpart_degree_classes* c) {
+    Scratch<unsigned long long> counters(ctx, 5);
+    fill_bytes(ctx, counters.get(), 0, 40);
+    if (g->n > 0) {
+        hipLaunchKernelGGL(degree_class_kernel<O>, dim3(grid_for(g->n)), dim3(kBlock), 0, ctx->stream, g->n,
+                           static_cast<const O*>(g->offsets), c->low, c->medium, c->high, counters.get());
+        GPART_LAUNCH_CHECK();
+    }
+    unsigned long long h[5];
+    read_back(ctx, h, counters.get(), sizeof(h));
+    c->low_count = static_cast<std::int64_t>(h[0]);
+    c->medium_count = static_cast<std::int64_t>(h[1]);
+    c->high_count = static_cast<std::int64_t>(h[2]);
+    c->high_edges = static_cast<std::int64_t>(h[3]);
+    c->max_degree = static_cast<std::int64_t>(h[4]);
+}
+
+template <typename O, typename W>
+void record_bounds_impl(gpart_ctx* ctx, const gpart_csr_view* g, std::uint64_t* row_bounds,
+                        std::uint64_t* maximum, int* unsorted) {
+    Scratch<unsigned long long> meta(ctx, 2);
+    fill_bytes(ctx, meta.get(), 0, 16);
+    if (g->n > 0) {
+        Scratch<std::int32_t> low(ctx, g->n), medium(ctx, g->n), high(ctx, g->n);
+        gpart_degree_classes c{low.get(), medium.get(), high.get(), 0, 0, 0, 0, 0};
+        build_classes<O, W>(ctx, g, &c);
+        const auto* off = static_cast<const O*>(g->offsets);
+        const auto* w = static_cast<const W*>(g->edge_weights);
+        auto* rb = reinterpret_cast<unsigned long long*>(row_bounds);
+        if (c.low_count)
+            hipLaunchKernelGGL((record_bound_kernel<O, W, 8>), dim3(grid_for(c.low_count * 8)), dim3(kBlock), 0,
+                               ctx->stream, c.low_count, c.low, off, g->neighbors, w, rb, meta.get());
+        if (c.medium_count)
+            hipLaunchKernelGGL((record_bound_kernel<O, W, 64>), dim3(grid_for(c.medium_count * 64)), dim3(kBlock), 0,
+                               ctx->stream, c.medium_count, c.medium, off, g->neighbors, w, rb, meta.get());
+        if (c.high_count)
+            hipLaunchKernelGGL((record_bound_kernel<O, W, 64>), dim3(grid_for(c.high_count * 64)), dim3(kBlock), 0,
+                               ctx->stream, c.high_count, c.high, off, g->neighbors, w, rb, meta.get());
+        GPART_LAUNCH_CHECK();
+    }
+    unsigned long long h[2];
+    read_back(ctx, h, meta.get(), sizeof(h));
+    *maximum = h[0];
+    *unsorted = h[1] != 0;
+}
+
+}  // namespace
+}  // namespace gpart_hip
+
+using namespace gpart_hip;
+
+extern "C" {
+
+int gpart_degree_classes_build(gpart_ctx* ctx, const gpart_csr_view* g, gpart_degree_classes* classes) {
+    return guarded([&] {
+        if (g->width == GPART_WIDTH_STANDARD) build_classes<std::int32_t, std::uint32_t>(ctx, g, classes);
+        else build_classes<std::int64_t, std::uint64_t>(ctx, g, classes);
+    });
+}
+
+int gpart_record_bounds(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* row_bounds, uint64_t* maximum,
+                        int* unsorted) {
+    return guarded([&] {
+        if (g->width == GPART_WIDTH_STANDARD)
+            record_bounds_impl<std::int32_t, std::uint32_t>(ctx, g, row_bounds, maximum, unsorted);
+        else
+            record_bounds_impl<std::int64_t, std::uint64_t>(ctx, g, row_bounds, maximum, unsorted);
+    });
+}
+
+int gpart_contract_record_bounds(gpart_ctx* ctx, int64_t n, const int32_t* map, const uint64_t* fine_bounds,
+                                 int64_t n_coarse, uint64_t* coarse_bounds, uint64_t* maximum) {
+    return guarded([&] {
+        fill_bytes(ctx, coarse_bounds, 0, static_cast<std::size_t>(n_coarse) * 8);
+        auto* cb = reinterpret_cast<unsigned long long*>(coarse_bounds);
+        if (n > 0) {
+            hipLaunchKernelGGL(contract_bounds_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, map,
+                               reinterpret_cast<const unsigned long long*>(fine_bounds), cb);
+            GPART_LAUNCH_CHECK();
+        }
+        Scratch<unsigned long long> mx(ctx, 1);
+        fill_bytes(ctx, mx.get(), 0, 8);
+        if (n_coarse > 0) {
+            hipLaunchKernelGGL(max_weight_kernel<unsigned long long>, dim3(std::min<unsigned>(grid_for(n_coarse), 2048)),
+                               dim3(kBlock), 0, ctx->stream, n_coarse, cb, mx.get());
+            GPART_LAUNCH_CHECK();
+        }
+        *maximum = read_scalar(ctx, mx.get());
+    });
+}
+
+int gpart_sclp_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* clusters,
+                         const gpart_degree_classes* classes, const gpart_sclp_round* round, int32_t* proposals,
+                         uint64_t* gains, uint32_t* ties) {
+    return guarded([&] {
+        auto* gn = reinterpret_cast<unsigned long long*>(gains);
+        if (g->width == GPART_WIDTH_STANDARD)
+            launch_proposals<std::int32_t, std::uint32_t>(ctx, g, clusters, classes, round, proposals, gn, ties);
+        else
+            launch_proposals<std::int64_t, std::uint64_t>(ctx, g, clusters, classes, round, proposals, gn, ties);
+    });
+}
+
+int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const uint64_t* gains,
+                     const uint32_t* ties, const uint64_t* vertex_weights, uint64_t capacity, uint64_t gain_bound,
+                     int32_t* clusters, uint64_t* cluster_weights, int64_t* valid, int64_t* accepted,
+                     int64_t* nonempty) {
+    return guarded([&] {
+        auto* cw = reinterpret_cast<unsigned long long*>(cluster_weights);
+        const auto* gn = reinterpret_cast<const unsigned long long*>(gains);
+        Scratch<std::int32_t> flags(ctx, n), order(ctx, n), order_alt(ctx, n);
+        Scratch<std::int64_t> d_count(ctx, 1);
+        hipLaunchKernelGGL(valid_flags_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, proposals,
+                           flags.get());
+        GPART_LAUNCH_CHECK();
+        compact_indices(ctx, flags.get(), n, order.get(), d_count.get());
+        const std::int64_t p = read_scalar(ctx, d_count.get());
+        *valid = p;
+        *accepted = 0;
+        if (p > 0) {
+            std::int32_t* sorted = admission_sort(ctx, n, p, order.get(), order_alt.get(), proposals, gn, ties,
+                                                  gain_bound);
+            Scratch<std::int32_t> ordered_targets(ctx, p), accept(ctx, p);
+            Scratch<std::uint64_t> ordered_weights(ctx, p), prefix(ctx, p);
+            hipLaunchKernelGGL(gather_order_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
+                               proposals, vertex_weights, ordered_targets.get(), ordered_weights.get());
+            GPART_LAUNCH_CHECK();
+            device_scan<SegValue, SegOp>(ctx, SegWeightLoad{ordered_targets.get(), ordered_weights.get()},
+                                         SegInclusiveStore{prefix.get()}, p, SegOp{});
+            hipLaunchKernelGGL(admit_decide_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
+                               ordered_targets.get(), prefix.get(), cw, capacity, accept.get());
+            GPART_LAUNCH_CHECK();
+            Scratch<unsigned long long> acc(ctx, 1);
+            fill_bytes(ctx, acc.get(), 0, 8);
+            hipLaunchKernelGGL(admit_apply_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
+                               ordered_targets.get(), ordered_weights.get(), accept.get(), clusters, cw, acc.get());
+            GPART_LAUNCH_CHECK();
+            *accepted = static_cast<std::int64_t>(read_scalar(ctx, acc.get()));
+        }
+        *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));
+    });
+}
+
+int gpart_sclp_isolated(gpart_ctx* ctx, const gpart_csr_view* g, const uint64_t* vertex_weights, uint64_t capacity,
+                        int32_t* clusters, uint64_t* cluster_weights, int64_t* isolated, int64_t* nonempty) {
+    return guarded([&] {
+        const auto n = g->n;
+        auto* cw = reinterpret_cast<unsigned long long*>(cluster_weights);
+        Scratch<std::int32_t> flags(ctx, n), list(ctx, n);
+        Scratch<std::int64_t> d_count(ctx, 1);
+        if (g->width == GPART_WIDTH_STANDARD)
+            hipLaunchKernelGGL(zero_degree_flags_kernel<std::int32_t>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+                               n, static_cast<const std::int32_t*>(g->offsets), flags.get());
+        else
+            hipLaunchKernelGGL(zero_degree_flags_kernel<std::int64_t>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+                               n, static_cast<const std::int64_t*>(g->offsets), flags.get());
+        GPART_LAUNCH_CHECK();
+        compact_indices(ctx, flags.get(), n, list.get(), d_count.get());
+        const std::int64_t count = read_scalar(ctx, d_count.get());
+        *isolated = count;
+        if (count > 1) {
+            Scratch<std::uint64_t> prefix(ctx, count);
+            device_scan<std::uint64_t, SumOp<std::uint64_t>>(ctx, GatherWeightLoad{list.get(), vertex_weights},
+                                                              InclusiveStore<std::uint64_t>{prefix.get()}, count,
+                                                              SumOp<std::uint64_t>{});
+            const std::uint64_t total = read_scalar(ctx, prefix.get() + count - 1);
+            const std::int64_t bins = static_cast<std::int64_t>(total / capacity + (total % capacity != 0));
+            Scratch<std::int32_t> reps(ctx, bins);
+            hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(bins)), dim3(kBlock), 0, ctx->stream, reps.get(),
+                               0x7fffffff, bins);
+            hipLaunchKernelGGL(isolated_rep_kernel, dim3(grid_for(count)), dim3(kBlock), 0, ctx->stream, count,
+                               list.get(), prefix.get(), vertex_weights, capacity, reps.get());
+            hipLaunchKernelGGL(isolated_commit_kernel, dim3(grid_for(count)), dim3(kBlock), 0, ctx->stream, count,
+                               list.get(), prefix.get(), vertex_weights, capacity, reps.get(), clusters, cw);
+            GPART_LAUNCH_CHECK();
+        }
+        *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));
+    });
+}
+
+int gpart_sclp_two_hop(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const uint64_t* gains,
+                       const uint32_t* ties, const uint64_t* vertex_weights, uint64_t capacity, uint64_t gain_bound,
+                       int64_t merge_budget, int32_t* clusters, uint64_t* cluster_weights, int64_t* candidates,
+                       int64_t* pairable, int64_t* merged, int64_t* nonempty) {
+    return guarded([&] {
+        auto* cw = reinterpret_cast<unsigned long long*>(cluster_weights);
+        const auto* gn = reinterpret_cast<const unsigned long long*>(gains);
+        Scratch<std::int32_t> favorites(ctx, n), flags(ctx, n), order(ctx, n), order_alt(ctx, n);
+        Scratch<std::int64_t> d_count(ctx, 1);
+        hipLaunchKernelGGL(favorite_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, vertex_weights, cw,
+                           clusters, proposals, favorites.get());
+        hipLaunchKernelGGL(valid_flags_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, favorites.get(),
+                           flags.get());
+        GPART_LAUNCH_CHECK();
+        compact_indices(ctx, flags.get(), n, order.get(), d_count.get());
+        const std::int64_t p = read_scalar(ctx, d_count.get());
+        *candidates = p;
+        *pairable = 0;
+        *merged = 0;
+        if (p > 0) {
+            std::int32_t* sorted = admission_sort(ctx, n, p, order.get(), order_alt.get(), favorites.get(), gn, ties,
+                                                  gain_bound);
+            Scratch<std::int32_t> ordered_targets(ctx, p);
+            Scratch<std::uint64_t> ordered_weights(ctx, p), ranks(ctx, p);
+            Scratch<std::int64_t> pair_flags(ctx, p), positions(ctx, p + 1);
+            hipLaunchKernelGGL(gather_order_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
+                               favorites.get(), vertex_weights, ordered_targets.get(), ordered_weights.get());
+            GPART_LAUNCH_CHECK();
+            device_scan<SegValue, SegOp>(ctx, SegOneLoad{ordered_targets.get()}, SegInclusiveStore{ranks.get()}, p,
+                                         SegOp{});
+            hipLaunchKernelGGL(pair_flags_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
+                               ordered_weights.get(), ranks.get(), capacity, pair_flags.get());
+            GPART_LAUNCH_CHECK();
+            exclusive_scan<std::int64_t>(ctx, pair_flags.get(), positions.get(), p);
+            const std::int64_t last_pos = read_scalar(ctx, positions.get() + p - 1);
+            const std::int64_t last_flag = read_scalar(ctx, pair_flags.get() + p - 1);
+            const std::int64_t pair_count = last_pos + last_flag;
+            *pairable = pair_count;
+            const auto budget = static_cast<std::uint64_t>(std::max<std::int64_t>(0, merge_budget));
+            if (pair_count > 0 && budget > 0) {
+                Scratch<unsigned long long> m(ctx, 1);
+                fill_bytes(ctx, m.get(), 0, 8);
+                hipLaunchKernelGGL(two_hop_commit_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
+                                   pair_flags.get(), positions.get(), budget, ordered_weights.get(), clusters, cw,
+                                   m.get());
+                GPART_LAUNCH_CHECK();
+                *merged = static_cast<std::int64_t>(read_scalar(ctx, m.get()));
+            }
+        }
+        *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));
+    });
+}
+
+int gpart_sclp_compact(gpart_ctx* ctx, int64_t n, const int32_t* clusters, const uint64_t* cluster_weights,
+                       const gpart_csr_view* g, int32_t* map, int64_t n_coarse, void* coarse_vertex_weights,
+                       uint64_t* maximum_weight) {
+    return guarded([&] {
+        const auto* cw = reinterpret_cast<const unsigned long long*>(cluster_weights);
+        Scratch<std::int32_t> compact(ctx, n);
+        device_scan<std::int64_t, SumOp<std::int64_t>>(ctx, NonemptyLoad{cw}, CompactIdStore{compact.get()}, n,
+                                                       SumOp<std::int64_t>{});
+        const std::size_t wbytes = g->width == GPART_WIDTH_STANDARD ? 4 : 8;
+        fill_bytes(ctx, coarse_vertex_weights, 0, static_cast<std::size_t>(n_coarse) * wbytes);
+        Scratch<unsigned long long> mx(ctx, 1);
+        fill_bytes(ctx, mx.get(), 0, 8);
+        if (g->width == GPART_WIDTH_STANDARD) {
+            hipLaunchKernelGGL(map_and_weights_kernel<std::uint32_t>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+                               n, clusters, compact.get(), static_cast<const std::uint32_t*>(g->vertex_weights), map,
+                               static_cast<std::uint32_t*>(coarse_vertex_weights));
+            hipLaunchKernelGGL(max_weight_kernel<std::uint32_t>, dim3(std::min<unsigned>(grid_for(n_coarse), 2048)),
+                               dim3(kBlock), 0, ctx->stream, n_coarse,
+                               static_cast<const std::uint32_t*>(coarse_vertex_weights), mx.get());
+        } else {
+            hipLaunchKernelGGL(map_and_weights_kernel<std::uint64_t>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
+                               n, clusters, compact.get(), static_cast<const std::uint64_t*>(g->vertex_weights), map,
+                               static_cast<std::uint64_t*>(coarse_vertex_weights));
+            hipLaunchKernelGGL(max_weight_kernel<std::uint64_t>, dim3(std::min<unsigned>(grid_for(n_coarse), 2048)),
+                               dim3(kBlock), 0, ctx->stream, n_coarse,
+                               static_cast<const std::uint64_t*>(coarse_vertex_weights), mx.get());
+        }
+        GPART_LAUNCH_CHECK();
+        *maximum_weight = read_scalar(ctx, mx.get());
+    });
+}
+
+}  // extern "C"

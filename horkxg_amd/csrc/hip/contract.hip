@@ -1,0 +1,269 @@
+// contract.hip -- CSR contraction for gfx950 (contract_typed, src/coarsen.cu:1674-1814).
+//
+// 1. encode: every cross entry (map[v] != map[u]) becomes a record, appended with
+//    one atomic per wave (record order is irrelevant: the sort defines it and
+//    duplicate keys are summed exactly);
+// 2. LSD radix sort of the records by (source, target): 8-byte keys only in the
+//    packed layout (weight in the low bits), key + value otherwise;
+// 3. one scan fuses segment counting and segmented weight sums; the store
+//    functor emits each (source, target, sum) at its segment end;
+// 4. row offsets are written once per row from row boundaries.
+#include "internal.hpp"
+#include "scan.hpp"
+
+#include <algorithm>
+
+namespace gpart_hip {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Layout {
+    int vertex_bits;
+    int weight_bits;  // 0 = unpacked
+    std::uint64_t weight_mask;
+    std::uint64_t vertex_mask;
+};
+
+// SclpRecordLayout::checked(n, bound, 1) (include/detail/sclp_record.hpp:20-35)
+Layout choose_layout(std::int64_t vertices, std::uint64_t bound) {
+    int bits = 0;
+    for (auto v = vertices; v; v >>= 1) ++bits;
+    Layout l{bits, 0, 0, (bits >= 64 ? ~0ULL : (1ULL << bits) - 1)};
+    if (vertices <= 0 || vertices > 0x7fffffff || bound == ~0ULL) return l;
+    const int wb = 64 - 2 * bits;
+    if (wb <= 0) return l;
+    const std::uint64_t mask = (std::uint64_t{1} << wb) - 1;
+    if (bound > mask) return l;
+    l.weight_bits = wb;
+    l.weight_mask = mask;
+    return l;
+}
+
+template <typename O, typename W, typename V, int G>
+__global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
+                                                        const std::int32_t* neighbors, const W* weights,
+                                                        const std::int32_t* map, Layout layout, std::uint64_t* keys,
+                                                        V* values, unsigned long long* cursor) {
+    // G lanes per vertex; G == kBlock means one workgroup per vertex.
+    const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+    const int sub = threadIdx.x & (G - 1);
+    const bool active = group < count;
+    const std::int32_t v = active ? list[group] : 0;
+    const std::int64_t begin = active ? static_cast<std::int64_t>(offsets[v]) : 0;
+    const std::int64_t end = active ? static_cast<std::int64_t>(offsets[v + 1]) : 0;
+    const std::uint64_t source = active ? static_cast<std::uint32_t>(map[v]) : 0;
+    const std::uint64_t below = lanemask_lt();
+    // iterate in lock-step over the group's longest trip count so the whole wave
+    // takes part in each ballot
+    const std::int64_t my_trips = end - begin > sub ? (end - begin - sub + G - 1) / G : 0;
+    const auto trips = static_cast<std::int64_t>(group_max_u64<kWave>(static_cast<unsigned long long>(my_trips)));
+    for (std::int64_t it = 0; it < trips; ++it) {
+        const std::int64_t e = begin + sub + it * G;
+        bool emit = false;
+        std::uint64_t key = 0;
+        W w = 0;
+        if (e < end) {
+            const std::uint64_t target = static_cast<std::uint32_t>(map[neighbors[e]]);
+            if (target != source) {
+                emit = true;
+                w = weights[e];
+                key = layout.weight_bits
+                          ? (((source << layout.vertex_bits) | target) << layout.weight_bits) |
+                                static_cast<std::uint64_t>(w)
+                          : (source << layout.vertex_bits) | target;
+            }
+        }
+        const std::uint64_t mask = __ballot(emit);
+        if (!mask) continue;
+        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        unsigned long long base = 0;
+        if (lane_id() == leader) base = atomicAdd(cursor, static_cast<unsigned long long>(__popcll(mask)));
+        base = shfl_u64(base, leader, kWave);
+        if (emit) {
+            const std::uint64_t pos = base + __popcll(mask & below);
+            keys[pos] = key;
+            if (layout.weight_bits == 0) values[pos] = static_cast<V>(w);
+        }
+    }
+}
+
+struct RunValue {
+    unsigned long long heads;  // segment count
+    unsigned long long value;  // segmented weight sum
+    unsigned int head;
+    unsigned int pad;
+};
+struct RunOp {
+    __device__ __forceinline__ RunValue operator()(const RunValue& a, const RunValue& b) const {
+        return RunValue{a.heads + b.heads, b.head ? b.value : a.value + b.value, a.head | b.head, 0};
+    }
+    __device__ __forceinline__ RunValue identity() const { return RunValue{0, 0, 0, 0}; }
+};
+
+template <typename V>
+struct RunLoad {
+    const std::uint64_t* keys;
+    const V* values;
+    int shift;
+    std::uint64_t weight_mask;
+    bool packed;
+    __device__ __forceinline__ RunValue operator()(std::int64_t i) const {
+        const std::uint64_t k = keys[i];
+        const bool head = i == 0 || (keys[i - 1] >> shift) != (k >> shift);
+        const std::uint64_t w = packed ? (k & weight_mask) : static_cast<std::uint64_t>(values[i]);
+        return RunValue{head ? 1ULL : 0ULL, w, head ? 1U : 0U, 0};
+    }
+};
+
+template <typename W>
+struct RunStore {
+    const std::uint64_t* keys;
+    std::int64_t n;
+    int shift;
+    std::uint64_t vertex_mask;
+    int vertex_bits;
+    std::uint32_t* sources;  // unique sources (for offsets)
+    std::int32_t* neighbors;
+    W* weights;
+    unsigned long long* total;
+    __device__ __forceinline__ void operator()(std::int64_t i, const RunValue& prefix, const RunValue& item) const {
+        const RunValue incl = RunOp{}(prefix, item);
+        if (i == n - 1) *total = incl.heads;
+        const std::uint64_t k = keys[i] >> shift;
+        const bool last = i == n - 1 || (keys[i + 1] >> shift) != k;
+        if (!last) return;
+        const std::int64_t j = static_cast<std::int64_t>(incl.heads) - 1;
+        sources[j] = static_cast<std::uint32_t>(k >> vertex_bits);
+        neighbors[j] = static_cast<std::int32_t>(k & vertex_mask);
+        weights[j] = static_cast<W>(incl.value);
+    }
+};
+
+template <typename O>
+__global__ void row_offsets_kernel(std::int64_t m, const std::uint32_t* sources, std::int64_t n_coarse, O* offsets) {
+    const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j > m) return;
+    // rows (previous source, this source] start at j; the sentinel j == m closes the tail
+    const std::int64_t here = j < m ? static_cast<std::int64_t>(sources[j]) : n_coarse;
+    const std::int64_t prev = j == 0 ? -1 : static_cast<std::int64_t>(sources[j - 1]);
+    for (std::int64_t r = prev + 1; r <= here; ++r) offsets[r] = static_cast<O>(j);
+}
+
+inline unsigned grid_for(std::int64_t threads) { return static_cast<unsigned>((threads + kBlock - 1) / kBlock); }
+
+template <typename O, typename W>
+void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degree_classes* classes,
+                   const std::int32_t* map, std::int64_t n_coarse, std::uint64_t bound, void** out_offsets,
+                   std::int32_t** out_neighbors, void** out_weights, std::int64_t* out_m) {
+    using V = W;  // unpacked value type follows the graph's weight type
+    const Layout layout = choose_layout(n_coarse, bound);
+    const std::int64_t m = fine->m;
+    const auto* off = static_cast<const O*>(fine->offsets);
+    const auto* ew = static_cast<const W*>(fine->edge_weights);
+    profile_begin(ctx, GPART_PROFILE_CONTRACT_SORT);
+    Scratch<std::uint64_t> keys(ctx, std::max<std::int64_t>(m, 1)), keys_alt(ctx, std::max<std::int64_t>(m, 1));
+    Scratch<V> values, values_alt;
+    if (!layout.weight_bits) {
+        values.reset(ctx, std::max<std::int64_t>(m, 1));
+        values_alt.reset(ctx, std::max<std::int64_t>(m, 1));
+    }
+    Scratch<unsigned long long> cursor(ctx, 1);
+    fill_bytes(ctx, cursor.get(), 0, 8);
+    if (classes->low_count)
+        hipLaunchKernelGGL((encode_kernel<O, W, V, 8>), dim3(grid_for(classes->low_count * 8)), dim3(kBlock), 0,
+                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, ew, map, layout,
+                           keys.get(), values.get(), cursor.get());
+    if (classes->medium_count)
+        hipLaunchKernelGGL((encode_kernel<O, W, V, 64>), dim3(grid_for(classes->medium_count * 64)), dim3(kBlock), 0,
+                           ctx->stream, classes->medium_count, classes->medium, off, fine->neighbors, ew, map, layout,
+                           keys.get(), values.get(), cursor.get());
+    if (classes->high_count)
+        hipLaunchKernelGGL((encode_kernel<O, W, V, kBlock>), dim3(static_cast<unsigned>(classes->high_count)),
+                           dim3(kBlock), 0, ctx->stream, classes->high_count, classes->high, off, fine->neighbors, ew,
+                           map, layout, keys.get(), values.get(), cursor.get());
+    GPART_LAUNCH_CHECK();
+    const auto cross = static_cast<std::int64_t>(read_scalar(ctx, cursor.get()));
+    const int shift = layout.weight_bits;
+    const int key_bits = 2 * layout.vertex_bits;
+    std::uint64_t* sorted_keys = keys.get();
+    V* sorted_values = values.get();
+    if (cross > 1) {
+        bool alt;
+        if (layout.weight_bits)
+            alt = radix_sort_keys(ctx, keys.get(), keys_alt.get(), cross, shift, shift + key_bits);
+        else
+            alt = radix_sort_pairs(ctx, keys.get(), keys_alt.get(), values.get(), values_alt.get(), cross, 0, key_bits);
+        if (alt) {
+            sorted_keys = keys_alt.get();
+            sorted_values = values_alt.get();
+        }
+    }
+    // reduce: one scan counts (source, target) segments and sums their weights
+    Scratch<std::uint32_t> sources(ctx, std::max<std::int64_t>(cross, 1));
+    Scratch<std::int32_t> tmp_nbr(ctx, std::max<std::int64_t>(cross, 1));
+    Scratch<W> tmp_w(ctx, std::max<std::int64_t>(cross, 1));
+    Scratch<unsigned long long> segments(ctx, 1);
+    fill_bytes(ctx, segments.get(), 0, 8);
+    if (cross > 0) {
+        device_scan<RunValue, RunOp>(
+            ctx, RunLoad<V>{sorted_keys, sorted_values, shift, layout.weight_mask, layout.weight_bits != 0},
+            RunStore<W>{sorted_keys, cross, shift, layout.vertex_mask, layout.vertex_bits, sources.get(),
+                        tmp_nbr.get(), tmp_w.get(), segments.get()},
+            cross, RunOp{});
+    }
+    const auto coarse_m = static_cast<std::int64_t>(read_scalar(ctx, segments.get()));
+    std::int32_t* nbr = nullptr;
+    W* wts = nullptr;
+    O* offsets = static_cast<O*>(ctx_alloc(ctx, sizeof(O) * static_cast<std::size_t>(n_coarse + 1)));
+    nbr = static_cast<std::int32_t*>(ctx_alloc(ctx, sizeof(std::int32_t) * std::max<std::int64_t>(coarse_m, 1)));
+    wts = static_cast<W*>(ctx_alloc(ctx, sizeof(W) * std::max<std::int64_t>(coarse_m, 1)));
+    if (coarse_m > 0) {
+        GPART_HIP_CHECK(hipMemcpyAsync(nbr, tmp_nbr.get(), sizeof(std::int32_t) * coarse_m, hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+        GPART_HIP_CHECK(hipMemcpyAsync(wts, tmp_w.get(), sizeof(W) * coarse_m, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    hipLaunchKernelGGL(row_offsets_kernel<O>, dim3(grid_for(coarse_m + 1)), dim3(kBlock), 0, ctx->stream, coarse_m,
+                       sources.get(), n_coarse, offsets);
+    GPART_LAUNCH_CHECK();
+    const double n = static_cast<double>(fine->n), md = static_cast<double>(m), mc = static_cast<double>(coarse_m);
+    const double bytes = (n + 1) * sizeof(O) + md * (4 + sizeof(W)) + 4 * md + 4 * n + 32 * md + 16 * mc +
+                         mc * (4 + sizeof(W)) + (n_coarse + 1.0) * sizeof(O) +
+                         (layout.weight_bits ? 0.0 : 2.0 * sizeof(W) * md);
+    profile_end(ctx, GPART_PROFILE_CONTRACT_SORT, bytes);
+    *out_offsets = offsets;
+    *out_neighbors = nbr;
+    *out_weights = wts;
+    *out_m = coarse_m;
+}
+
+}  // namespace
+}  // namespace gpart_hip
+
+using namespace gpart_hip;
+
+extern "C" void gpart_record_layout(int64_t vertices, uint64_t bound, int* vertex_bits, int* weight_bits,
+                                    uint64_t* weight_mask) {
+    const Layout l = choose_layout(vertices, bound);
+    // the reference reports 32 vertex bits for the unpacked layout
+    *vertex_bits = l.weight_bits ? l.vertex_bits : 32;
+    *weight_bits = l.weight_bits;
+    *weight_mask = l.weight_mask;
+}
+
+extern "C" int gpart_contract(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degree_classes* classes,
+                              const int32_t* map, int64_t n_coarse, uint64_t coarse_row_bound, void** coarse_offsets,
+                              int32_t** coarse_neighbors, void** coarse_edge_weights, int64_t* coarse_m) {
+    return guarded([&] {
+        if (n_coarse <= 0 || n_coarse > 0x7fffffff)
+            throw Error(GPART_ERR_INVALID_ARGUMENT, "invalid coarse vertex count");
+        if (fine->width == GPART_WIDTH_STANDARD)
+            contract_impl<std::int32_t, std::uint32_t>(ctx, fine, classes, map, n_coarse, coarse_row_bound,
+                                                       coarse_offsets, coarse_neighbors, coarse_edge_weights,
+                                                       coarse_m);
+        else
+            contract_impl<std::int64_t, std::uint64_t>(ctx, fine, classes, map, n_coarse, coarse_row_bound,
+                                                       coarse_offsets, coarse_neighbors, coarse_edge_weights,
+                                                       coarse_m);
+    });
+}

@@ -1,0 +1,600 @@
+// initial.hip -- weighted recursive spectral bisection for gfx950
+// (src/initial_partition_gpu.cu of the reference).
+//
+// Both forms keep the reference's FP64 summation order so that results are
+// reproducible and identical between forms:
+//   * a vertex is owned by 32 lanes (half a wave64); lane l accumulates entries
+//     e = begin+l, begin+l+32, ... in increasing e, then a 32-lane shuffle-down
+//     tree (16, 8, 4, 2, 1) combines them;
+//   * moments are combined per 8 consecutive vertices with the reference's
+//     block tree ((z0+z4)+(z2+z6))+((z1+z5)+(z3+z7)), z = m + 0.0;
+//   * the normalisation sums those partials with 256 strided accumulators and
+//     the same trees.
+// Multiply-adds are fused only where written with fma() (the SpMV accumulation
+// and the variance); the library is built with -ffp-contract=off.
+//
+// Plain form: the reference's launch structure, one task at a time.
+// Batched form: all tasks of one recursion depth in one launch, one 1024-thread
+// workgroup per task: start, 64 diffusion steps, the (x, id) sort (LDS bitonic),
+// the weight prefix, the feasible split and the label assignment, emitting the
+// child tasks for the next depth.
+#include "internal.hpp"
+#include "scan.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace gpart_hip {
+namespace {
+
+constexpr int kHalf = 32;
+constexpr int kMaxBatchedTask = 4096;
+constexpr int kBatchThreads = 1024;
+
+struct Moments {
+    double mass, sum, squares;
+};
+
+__device__ __forceinline__ double tree32(double v) {
+#pragma unroll
+    for (int s = 16; s; s >>= 1) v += __shfl_down(v, s, kHalf);
+    return v;
+}
+
+__device__ __forceinline__ double z(double v) { return v + 0.0; }
+
+// Lane 0 of a 32-lane warp_sum over lanes 0..7 holding b[0..7], others zero.
+__device__ __forceinline__ double tree8(const double* b) {
+    double c0 = z(z(b[0])) + z(z(b[4])), c1 = z(z(b[1])) + z(z(b[5]));
+    double c2 = z(z(b[2])) + z(z(b[6])), c3 = z(z(b[3])) + z(z(b[7]));
+    return (c0 + c2) + (c1 + c3);
+}
+
+// One 8-vertex block partial: each vertex moment sits on lane 0 of its warp, so
+// warp_sum leaves it as z(m); the eight warp totals then use tree8.
+__device__ __forceinline__ Moments block_partial(const Moments* vm, int first, int count) {
+    double a[8], b[8], c[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        const int i = first + w;
+        const Moments m = i < count ? vm[i] : Moments{0, 0, 0};
+        a[w] = z(z(z(z(z(m.mass)))));
+        b[w] = z(z(z(z(z(m.sum)))));
+        c[w] = z(z(z(z(z(m.squares)))));
+    }
+    return Moments{tree8(a), tree8(b), tree8(c)};
+}
+
+struct Norm {
+    double mean, scale;
+};
+
+__device__ __forceinline__ Norm normalization_of(const Moments& m) {
+    const double mean = m.sum / m.mass;
+    const double variance = fmax(0.0, fma(-mean, mean, m.squares / m.mass));
+    return Norm{mean, variance > 1e-30 ? 1.0 / sqrt(variance) : 1.0};
+}
+
+// finish_moments over `count` partials, executed by threads 0..255 of the block
+// (other threads only pass the barriers). Returns the normalisation to all.
+__device__ Norm finish_block(const Moments* partials, int count, double* shared8, Norm* shared_norm) {
+    const int t = threadIdx.x;
+    double ms = 0.0, ss = 0.0, qs = 0.0;
+    if (t < 256) {
+        for (int i = t; i < count; i += 256) {
+            ms += partials[i].mass;
+            ss += partials[i].sum;
+            qs += partials[i].squares;
+        }
+    }
+    // 32-lane trees inside threads 0..255 (half-waves 0..7)
+    if (t < 256) {
+        ms = tree32(ms);
+        ss = tree32(ss);
+        qs = tree32(qs);
+        if ((t & 31) == 0) {
+            shared8[(t >> 5) * 3 + 0] = ms;
+            shared8[(t >> 5) * 3 + 1] = ss;
+            shared8[(t >> 5) * 3 + 2] = qs;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        double a[8], b[8], c[8];
+        for (int w = 0; w < 8; ++w) {
+            a[w] = shared8[w * 3 + 0];
+            b[w] = shared8[w * 3 + 1];
+            c[w] = shared8[w * 3 + 2];
+        }
+        *shared_norm = normalization_of(Moments{tree8(a), tree8(b), tree8(c)});
+    }
+    __syncthreads();
+    return *shared_norm;
+}
+
+// ------------------------------------------------------------------ plain form
+
+template <typename O, typename W>
+__global__ void __launch_bounds__(256) spectral_start_kernel(int count, const int* ids, const int* labels, int group,
+                                                             const O* offsets, const int* neighbors, const W* weights,
+                                                             double* degree, double* x, unsigned seed,
+                                                             Moments* partials) {
+    __shared__ Moments vm[8];
+    const int lane = threadIdx.x & 31, hw = threadIdx.x >> 5;
+    const int i = blockIdx.x * 8 + hw;
+    double d = 0;
+    if (i < count) {
+        const int v = ids[i];
+        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32)
+            if (labels[neighbors[e]] == group) d += static_cast<double>(weights[e]);
+    }
+    d = tree32(d);
+    if (lane == 0) {
+        Moments m{0, 0, 0};
+        if (i < count) {
+            const int v = ids[i];
+            const double a = static_cast<double>(mix32(static_cast<unsigned>(v) ^ seed)) / 4294967296.0 - 0.5;
+            degree[v] = d;
+            x[v] = a;
+            const double mass = d > 0 ? d : 1.0;
+            m = Moments{mass, mass * a, mass * a * a};
+        }
+        vm[hw] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = block_partial(vm, 0, 8);
+}
+
+template <typename O, typename W, typename T>
+__global__ void __launch_bounds__(256) spectral_step_kernel(int count, const int* ids, const int* labels, int group,
+                                                            const O* offsets, const int* neighbors, const W* weights,
+                                                            const double* degree, const T* x, T* y,
+                                                            Moments* partials) {
+    __shared__ Moments vm[8];
+    const int lane = threadIdx.x & 31, hw = threadIdx.x >> 5;
+    const int i = blockIdx.x * 8 + hw;
+    double sum = 0;
+    if (i < count) {
+        const int v = ids[i];
+        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32) {
+            const int u = neighbors[e];
+            if (labels[u] == group) sum = fma(static_cast<double>(weights[e]), static_cast<double>(x[u]), sum);
+        }
+    }
+    sum = tree32(sum);
+    if (lane == 0) {
+        Moments m{0, 0, 0};
+        if (i < count) {
+            const int v = ids[i];
+            const double d = degree[v];
+            const double a = d > 0 ? 0.5 * (static_cast<double>(x[v]) + sum / d) : static_cast<double>(x[v]);
+            y[v] = static_cast<T>(a);
+            const double mass = d > 0 ? d : 1.0;
+            m = Moments{mass, mass * a, mass * a * a};
+        }
+        vm[hw] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = block_partial(vm, 0, 8);
+}
+
+__global__ void __launch_bounds__(256) finish_moments_kernel(int count, const Moments* partials, Norm* norm) {
+    __shared__ double shared8[24];
+    __shared__ Norm shared_norm;
+    const Norm r = finish_block(partials, count, shared8, &shared_norm);
+    if (threadIdx.x == 0) *norm = r;
+}
+
+__global__ void normalize_kernel(int count, const int* ids, double* x, const Norm* norm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const int v = ids[i];
+        x[v] = (x[v] - norm->mean) * norm->scale;
+    }
+}
+
+// Order-preserving integer image of a double; -0.0 and +0.0 map together so that
+// (x, id) ties match the reference comparator (x[a] == x[b] for signed zeros).
+__device__ __forceinline__ std::uint64_t ordered_bits(double v) {
+    if (v == 0.0) v = 0.0;
+    const std::uint64_t b = static_cast<std::uint64_t>(__double_as_longlong(v));
+    return (b >> 63) ? ~b : (b | (1ULL << 63));
+}
+
+__device__ __forceinline__ std::uint64_t ordered_bits(float v) {
+    if (v == 0.0f) v = 0.0f;
+    const std::uint32_t b = static_cast<std::uint32_t>(__float_as_uint(v));
+    return (b >> 31) ? static_cast<std::uint32_t>(~b) : (b | 0x80000000U);
+}
+
+__global__ void sort_keys_kernel(int count, const int* ids, const double* x, std::uint64_t* keys, std::uint64_t* vals,
+                                 bool by_id) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int v = ids[i];
+    keys[i] = by_id ? static_cast<std::uint64_t>(v) : ordered_bits(x[v]);
+    vals[i] = static_cast<std::uint64_t>(v);
+}
+
+__global__ void copy_ids_kernel(int count, const std::uint64_t* vals, int* ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) ids[i] = static_cast<int>(vals[i]);
+}
+
+struct OrderWeightLoad {
+    const int* ids;
+    const std::uint64_t* weights;
+    __device__ __forceinline__ std::uint64_t operator()(std::int64_t i) const { return weights[ids[i]]; }
+};
+
+__global__ void assign_split_kernel(int count, const int* ids, int boundary, int left, int right, int* labels) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) labels[ids[i]] = i < boundary ? left : right;
+}
+
+// ------------------------------------------------------------------ batched form
+
+template <typename T>
+struct VecTraits;
+template <>
+struct VecTraits<double> {
+    __device__ static double fmadd(double w, double x, double s) { return fma(w, x, s); }
+};
+template <>
+struct VecTraits<float> {
+    __device__ static float fmadd(float w, float x, float s) { return fmaf(w, x, s); }
+};
+
+struct BatchShared {
+    Moments partials[kMaxBatchedTask / 8];
+    std::uint64_t keys[kMaxBatchedTask];
+    int ids[kMaxBatchedTask];
+    double shared8[24];
+    Norm norm;
+    int boundary;
+    int status;
+};
+
+// floor(weight * lp / parts) exactly (the reference evaluates it in long double,
+// which is exact for weight < 2^58).
+__device__ __forceinline__ std::uint64_t split_target(std::uint64_t weight, int lp, int parts) {
+    const std::uint64_t q = weight / static_cast<std::uint64_t>(parts);
+    const std::uint64_t r = weight % static_cast<std::uint64_t>(parts);
+    return q * static_cast<std::uint64_t>(lp) + (r * static_cast<std::uint64_t>(lp)) / static_cast<std::uint64_t>(parts);
+}
+
+template <typename O, typename W, typename T>
+__global__ void __launch_bounds__(kBatchThreads) spectral_depth_kernel(
+    const gpart_bisect_task* tasks, int task_count, const O* offsets, const int* neighbors, const W* weights,
+    int* labels, int* order, const std::uint64_t* vertex_weights, unsigned seed, std::uint64_t capacity,
+    double* degree, T* xa, T* xb, Moments* vm, double* coords, gpart_bisect_task* next, int* next_count,
+    int* status) {
+    __shared__ BatchShared sh;
+    const int task_index = blockIdx.x;
+    if (task_index >= task_count) return;  // uniform per block
+    const gpart_bisect_task task = tasks[task_index];
+    const int count = task.count;
+    int* ids = order + task.begin;
+    Moments* tvm = vm + task.begin;
+    const int group = task.first_part;
+    const unsigned tseed = seed ^ static_cast<unsigned>(task.first_part);
+    const int lane = threadIdx.x & 31, hw = threadIdx.x >> 5;
+    constexpr int kHalves = kBatchThreads / kHalf;
+    const int blocks8 = (count + 7) / 8;
+    T* x = xa;
+    T* y = xb;
+
+    // spectral_start
+    for (int base = 0; base < count; base += kHalves) {
+        const int i = base + hw;
+        double d = 0;
+        if (i < count) {
+            const int v = ids[i];
+            for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32)
+                if (labels[neighbors[e]] == group) d += static_cast<double>(weights[e]);
+        }
+        d = tree32(d);
+        if (lane == 0 && i < count) {
+            const int v = ids[i];
+            const double a = static_cast<double>(mix32(static_cast<unsigned>(v) ^ tseed)) / 4294967296.0 - 0.5;
+            degree[v] = d;
+            x[v] = static_cast<T>(a);
+            const double av = static_cast<double>(static_cast<T>(a));
+            const double mass = d > 0 ? d : 1.0;
+            tvm[i] = Moments{mass, mass * av, mass * av * av};
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int b = threadIdx.x; b < blocks8; b += kBatchThreads) sh.partials[b] = block_partial(tvm, b * 8, count);
+    __syncthreads();
+    Norm norm = finish_block(sh.partials, blocks8, sh.shared8, &sh.norm);
+    for (int iteration = 0; iteration <= 64; ++iteration) {
+        for (int i = threadIdx.x; i < count; i += kBatchThreads) {
+            const int v = ids[i];
+            x[v] = static_cast<T>((static_cast<double>(x[v]) - norm.mean) * norm.scale);
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (iteration == 64) break;
+        for (int base = 0; base < count; base += kHalves) {
+            const int i = base + hw;
+            T sum = 0;
+            if (i < count) {
+                const int v = ids[i];
+                for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32) {
+                    const int u = neighbors[e];
+                    if (labels[u] == group) sum = VecTraits<T>::fmadd(static_cast<T>(weights[e]), x[u], sum);
+                }
+            }
+            double sd;
+            if constexpr (sizeof(T) == 8) {
+                sd = tree32(static_cast<double>(sum));
+            } else {
+                float s = sum;
+#pragma unroll
+                for (int s2 = 16; s2; s2 >>= 1) s += __shfl_down(s, s2, kHalf);
+                sd = static_cast<double>(s);
+            }
+            if (lane == 0 && i < count) {
+                const int v = ids[i];
+                const double d = degree[v];
+                double a;
+                if constexpr (sizeof(T) == 8) {
+                    a = d > 0 ? 0.5 * (static_cast<double>(x[v]) + sd / d) : static_cast<double>(x[v]);
+                } else {
+                    const float df = static_cast<float>(d);
+                    const float af = d > 0 ? 0.5f * (x[v] + static_cast<float>(sd) / df) : x[v];
+                    a = static_cast<double>(af);
+                }
+                y[v] = static_cast<T>(a);
+                const double mass = d > 0 ? d : 1.0;
+                tvm[i] = Moments{mass, mass * a, mass * a * a};
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (int b = threadIdx.x; b < blocks8; b += kBatchThreads) sh.partials[b] = block_partial(tvm, b * 8, count);
+        __syncthreads();
+        norm = finish_block(sh.partials, blocks8, sh.shared8, &sh.norm);
+        T* t = x;
+        x = y;
+        y = t;
+    }
+    // (x, id) sort: bitonic over the next power of two, padding keys sort last
+    int size = 1;
+    while (size < count) size <<= 1;
+    for (int i = threadIdx.x; i < size; i += kBatchThreads) {
+        if (i < count) {
+            const int v = ids[i];
+            sh.keys[i] = ordered_bits(x[v]);
+            sh.ids[i] = v;
+            if (coords) coords[v] = static_cast<double>(x[v]);
+        } else {
+            sh.keys[i] = ~0ULL;
+            sh.ids[i] = 0x7fffffff;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= size; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < size; i += kBatchThreads) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const bool up = (i & k) == 0;
+                    const std::uint64_t ki = sh.keys[i], kp = sh.keys[p];
+                    const int ii = sh.ids[i], ip = sh.ids[p];
+                    const bool greater = ki > kp || (ki == kp && ii > ip);
+                    if (greater == up) {
+                        sh.keys[i] = kp;
+                        sh.keys[p] = ki;
+                        sh.ids[i] = ip;
+                        sh.ids[p] = ii;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // inclusive weight prefix of the sorted segment, in place of the keys
+    for (int i = threadIdx.x; i < count; i += kBatchThreads) {
+        ids[i] = sh.ids[i];
+        sh.keys[i] = vertex_weights[sh.ids[i]];
+    }
+    __syncthreads();
+    for (int offset = 1; offset < count; offset <<= 1) {
+        std::uint64_t add[4] = {0, 0, 0, 0};
+        int slot = 0;
+        for (int i = threadIdx.x; i < count; i += kBatchThreads, ++slot)
+            add[slot] = i >= offset ? sh.keys[i - offset] : 0;
+        __syncthreads();
+        slot = 0;
+        for (int i = threadIdx.x; i < count; i += kBatchThreads, ++slot) sh.keys[i] += add[slot];
+        __syncthreads();
+    }
+    // split selection (src/initial_partition_gpu.cu:233-248)
+    if (threadIdx.x == 0) {
+        const int lp = task.parts / 2, rp = task.parts - lp;
+        const std::uint64_t W_ = task.weight;
+        const std::uint64_t target = split_target(W_, lp, task.parts);
+        const std::uint64_t left_limit = capacity > W_ / lp ? W_ : capacity * lp;
+        const std::uint64_t right_limit = capacity > W_ / rp ? W_ : capacity * rp;
+        const std::uint64_t min_left = W_ > right_limit ? W_ - right_limit : 0;
+        int lo = 0, hi = count;  // lower_bound(prefix, target)
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (sh.keys[mid] < target) lo = mid + 1;
+            else hi = mid;
+        }
+        int boundary = lo + 1;
+        boundary = max(lp, min(count - rp, boundary));
+        std::uint64_t left_weight = sh.keys[boundary - 1];
+        if (left_weight > left_limit && boundary > lp) {
+            --boundary;
+            left_weight = sh.keys[boundary - 1];
+        }
+        if (left_weight < min_left && boundary < count - rp) {
+            ++boundary;
+            left_weight = sh.keys[boundary - 1];
+        }
+        int st = 0;
+        if (left_weight > left_limit || left_weight < min_left) st = 1;
+        const std::uint64_t right_weight = W_ - left_weight;
+        if (!st) {
+            if (lp == 1 && left_weight > capacity) st = 2;
+            if (rp == 1 && right_weight > capacity) st = 2;
+        }
+        sh.boundary = boundary;
+        sh.status = st;
+        if (st) {
+            atomicMax(status, st);
+        } else {
+            if (rp > 1) {
+                const int slot = atomicAdd(next_count, 1);
+                next[slot] = gpart_bisect_task{task.begin + boundary, count - boundary, task.first_part + lp, rp,
+                                               right_weight};
+            }
+            if (lp > 1) {
+                const int slot = atomicAdd(next_count, 1);
+                next[slot] = gpart_bisect_task{task.begin, boundary, task.first_part, lp, left_weight};
+            }
+        }
+    }
+    __syncthreads();
+    if (sh.status) return;
+    const int boundary = sh.boundary;
+    const int lp = task.parts / 2;
+    for (int i = threadIdx.x; i < count; i += kBatchThreads)
+        labels[sh.ids[i]] = i < boundary ? task.first_part : task.first_part + lp;
+}
+
+template <typename O, typename W>
+void plain_task(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, int32_t* order,
+                const std::uint64_t* vertex_weights, const gpart_bisect_task* task, uint32_t seed, double* x,
+                double* y, double* degree, std::uint64_t* prefix) {
+    const int count = task->count;
+    if (count <= 0) return;
+    int* ids = order + task->begin;
+    const auto* off = static_cast<const O*>(g->offsets);
+    const auto* w = static_cast<const W*>(g->edge_weights);
+    const int blocks = (count + 255) / 256;
+    const int spectral_blocks = (count + 7) / 8;
+    Scratch<Moments> partials(ctx, spectral_blocks);
+    Scratch<Norm> norm(ctx, 1);
+    profile_begin(ctx, GPART_PROFILE_SPECTRAL);
+    hipLaunchKernelGGL((spectral_start_kernel<O, W>), dim3(spectral_blocks), dim3(256), 0, ctx->stream, count, ids,
+                       labels, task->first_part, off, g->neighbors, w, degree, x,
+                       seed ^ static_cast<unsigned>(task->first_part), partials.get());
+    hipLaunchKernelGGL(finish_moments_kernel, dim3(1), dim3(256), 0, ctx->stream, spectral_blocks, partials.get(),
+                       norm.get());
+    GPART_LAUNCH_CHECK();
+    for (int iteration = 0; iteration <= 64; ++iteration) {
+        hipLaunchKernelGGL(normalize_kernel, dim3(blocks), dim3(256), 0, ctx->stream, count, ids, x, norm.get());
+        if (iteration == 64) break;
+        hipLaunchKernelGGL((spectral_step_kernel<O, W, double>), dim3(spectral_blocks), dim3(256), 0, ctx->stream,
+                           count, ids, labels, task->first_part, off, g->neighbors, w, degree, x, y, partials.get());
+        hipLaunchKernelGGL(finish_moments_kernel, dim3(1), dim3(256), 0, ctx->stream, spectral_blocks, partials.get(),
+                           norm.get());
+        GPART_LAUNCH_CHECK();
+        std::swap(x, y);
+    }
+    profile_end(ctx, GPART_PROFILE_SPECTRAL, 0.0);
+    // (x, id) total order: stable sort by id, then stable sort by x
+    Scratch<std::uint64_t> keys(ctx, count), keys_alt(ctx, count), vals(ctx, count), vals_alt(ctx, count);
+    hipLaunchKernelGGL(sort_keys_kernel, dim3(blocks), dim3(256), 0, ctx->stream, count, ids, x, keys.get(), vals.get(),
+                       true);
+    GPART_LAUNCH_CHECK();
+    std::uint64_t* cur_v = vals.get();
+    if (radix_sort_pairs(ctx, keys.get(), keys_alt.get(), vals.get(), vals_alt.get(), count, 0, 31))
+        cur_v = vals_alt.get();
+    hipLaunchKernelGGL(copy_ids_kernel, dim3(blocks), dim3(256), 0, ctx->stream, count, cur_v, ids);
+    hipLaunchKernelGGL(sort_keys_kernel, dim3(blocks), dim3(256), 0, ctx->stream, count, ids, x, keys.get(), vals.get(),
+                       false);
+    GPART_LAUNCH_CHECK();
+    cur_v = vals.get();
+    if (radix_sort_pairs(ctx, keys.get(), keys_alt.get(), vals.get(), vals_alt.get(), count, 0, 64))
+        cur_v = vals_alt.get();
+    hipLaunchKernelGGL(copy_ids_kernel, dim3(blocks), dim3(256), 0, ctx->stream, count, cur_v, ids);
+    GPART_LAUNCH_CHECK();
+    device_scan<std::uint64_t, SumOp<std::uint64_t>>(ctx, OrderWeightLoad{ids, vertex_weights},
+                                                      InclusiveStore<std::uint64_t>{prefix}, count,
+                                                      SumOp<std::uint64_t>{});
+    // x now holds the final coordinates (the caller's x pointer after an even swap count)
+}
+
+}  // namespace
+}  // namespace gpart_hip
+
+using namespace gpart_hip;
+
+extern "C" {
+
+int gpart_spectral_task(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, int32_t* order,
+                        const uint64_t* vertex_weights, const gpart_bisect_task* task, uint32_t seed, double* x,
+                        double* y, double* degree, uint64_t* prefix) {
+    return guarded([&] {
+        if (g->width == GPART_WIDTH_STANDARD)
+            plain_task<std::int32_t, std::uint32_t>(ctx, g, labels, order, vertex_weights, task, seed, x, y, degree,
+                                                    prefix);
+        else
+            plain_task<std::int64_t, std::uint64_t>(ctx, g, labels, order, vertex_weights, task, seed, x, y, degree,
+                                                    prefix);
+    });
+}
+
+int gpart_spectral_assign(gpart_ctx* ctx, const int32_t* order, const gpart_bisect_task* task, int32_t boundary,
+                          int32_t left_parts, int32_t* labels) {
+    return guarded([&] {
+        const int count = task->count;
+        if (count <= 0) return;
+        hipLaunchKernelGGL(assign_split_kernel, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, count,
+                           order + task->begin, boundary, task->first_part, task->first_part + left_parts, labels);
+        GPART_LAUNCH_CHECK();
+    });
+}
+
+int64_t gpart_spectral_depth_max_task(void) { return kMaxBatchedTask; }
+
+int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* labels, int32_t* order,
+                         const uint64_t* vertex_weights, const gpart_bisect_task* tasks, int32_t task_count,
+                         uint32_t seed, uint64_t capacity, int precision, double* coords, gpart_bisect_task* next,
+                         int32_t* next_count, int32_t* status) {
+    return guarded([&] {
+        *next_count = 0;
+        *status = 0;
+        if (task_count <= 0) return;
+        const std::int64_t n = g->n;
+        Scratch<double> degree(ctx, n), xa(ctx, n), xb(ctx, n);
+        Scratch<Moments> vm(ctx, n);
+        Scratch<int> counters(ctx, 2);
+        fill_bytes(ctx, counters.get(), 0, 8);
+        profile_begin(ctx, GPART_PROFILE_SPECTRAL);
+        auto launch = [&](auto o_tag, auto w_tag) {
+            using O = decltype(o_tag);
+            using W = decltype(w_tag);
+            const auto* off = static_cast<const O*>(g->offsets);
+            const auto* w = static_cast<const W*>(g->edge_weights);
+            if (precision == GPART_SPECTRAL_FP32)
+                hipLaunchKernelGGL((spectral_depth_kernel<O, W, float>), dim3(task_count), dim3(kBatchThreads), 0,
+                                   ctx->stream, tasks, task_count, off, g->neighbors, w, labels, order,
+                                   vertex_weights, seed, capacity, degree.get(), reinterpret_cast<float*>(xa.get()),
+                                   reinterpret_cast<float*>(xb.get()), vm.get(), coords, next, counters.get(),
+                                   counters.get() + 1);
+            else
+                hipLaunchKernelGGL((spectral_depth_kernel<O, W, double>), dim3(task_count), dim3(kBatchThreads), 0,
+                                   ctx->stream, tasks, task_count, off, g->neighbors, w, labels, order,
+                                   vertex_weights, seed, capacity, degree.get(), xa.get(), xb.get(), vm.get(), coords,
+                                   next, counters.get(), counters.get() + 1);
+            GPART_LAUNCH_CHECK();
+        };
+        if (g->width == GPART_WIDTH_STANDARD) launch(std::int32_t{}, std::uint32_t{});
+        else launch(std::int64_t{}, std::uint64_t{});
+        profile_end(ctx, GPART_PROFILE_SPECTRAL, 0.0);
+        int h[2];
+        read_back(ctx, h, counters.get(), sizeof(h));
+        *next_count = h[0];
+        *status = h[1];
+    });
+}
+
+}  // extern "C"

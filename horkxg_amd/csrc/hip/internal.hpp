@@ -1,0 +1,160 @@
+// internal.hpp -- host-side internals shared by the kernel library's sources.
+#pragma once
+
+#include "../../../include/gpart_hip.h"
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+struct gpart_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // Caching allocator: blocks keyed by rounded size; never returned to HIP until
+    // gpart_release_cache or destruction.
+    std::multimap<std::size_t, void*> free_blocks;
+    std::map<void*, std::size_t> live_blocks;
+    std::size_t cached_bytes = 0;
+    std::size_t live_bytes = 0;
+    // Small pinned staging buffer for host-out scalars.
+    void* pinned = nullptr;
+    // Profiling per family.
+    struct Profile {
+        bool enabled = false;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+        double bytes = 0.0;
+        std::int64_t launches = 0;
+    } profile[GPART_PROFILE_FAMILIES];
+    std::vector<hipEvent_t> spare_events;
+};
+
+namespace gpart_hip {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& message) : std::runtime_error(message), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+void set_last_error(const std::string& message);
+
+[[noreturn]] void throw_hip(hipError_t error, const char* file, int line, const char* call);
+
+#define GPART_HIP_CHECK(call)                                                   \
+    do {                                                                        \
+        const hipError_t error__ = (call);                                      \
+        if (error__ != hipSuccess) gpart_hip::throw_hip(error__, __FILE__, __LINE__, #call); \
+    } while (0)
+
+#define GPART_LAUNCH_CHECK() GPART_HIP_CHECK(hipGetLastError())
+
+template <typename F>
+int guarded(F&& body) {
+    try {
+        body();
+        return GPART_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code();
+    } catch (const std::invalid_argument& e) {
+        set_last_error(e.what());
+        return GPART_ERR_INVALID_ARGUMENT;
+    } catch (const std::overflow_error& e) {
+        set_last_error(e.what());
+        return GPART_ERR_OVERFLOW;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return GPART_ERR_RUNTIME;
+    }
+}
+
+void* ctx_alloc(gpart_ctx* ctx, std::size_t bytes);
+void ctx_free(gpart_ctx* ctx, void* p);
+
+// RAII scratch buffer from the context allocator.
+template <typename T>
+class Scratch {
+public:
+    Scratch() = default;
+    Scratch(gpart_ctx* ctx, std::int64_t count) { reset(ctx, count); }
+    ~Scratch() { release(); }
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    void reset(gpart_ctx* ctx, std::int64_t count) {
+        release();
+        ctx_ = ctx;
+        count_ = count;
+        ptr_ = count > 0 ? static_cast<T*>(ctx_alloc(ctx, sizeof(T) * static_cast<std::size_t>(count)))
+                         : nullptr;
+    }
+    void release() {
+        if (ptr_) ctx_free(ctx_, ptr_);
+        ptr_ = nullptr;
+        count_ = 0;
+    }
+    T* get() const { return ptr_; }
+    std::int64_t size() const { return count_; }
+
+private:
+    gpart_ctx* ctx_ = nullptr;
+    T* ptr_ = nullptr;
+    std::int64_t count_ = 0;
+};
+
+// Host-out readback through the context's pinned staging buffer (synchronises).
+void read_back(gpart_ctx* ctx, void* host, const void* device, std::size_t bytes);
+
+template <typename T>
+T read_scalar(gpart_ctx* ctx, const T* device) {
+    T value{};
+    read_back(ctx, &value, device, sizeof(T));
+    return value;
+}
+
+void fill_bytes(gpart_ctx* ctx, void* dst, int value, std::size_t bytes);
+
+// Profiling hooks around a family's main kernel launches.
+void profile_begin(gpart_ctx* ctx, int family);
+void profile_end(gpart_ctx* ctx, int family, double algorithmic_bytes);
+
+// ---- primitives (primitives.hip) ----
+// Stable LSD radix sort over key bits [begin_bit, end_bit); alternate buffers
+// provided by the caller; returns true when the result lies in the alternate
+// buffers. count may be read from device memory (d_count) when non-null, with n
+// the upper bound used for grid sizing.
+template <typename V>
+bool radix_sort_pairs(gpart_ctx* ctx, std::uint64_t* keys, std::uint64_t* keys_alt, V* values,
+                      V* values_alt, std::int64_t n, int begin_bit, int end_bit);
+bool radix_sort_keys(gpart_ctx* ctx, std::uint64_t* keys, std::uint64_t* keys_alt, std::int64_t n,
+                     int begin_bit, int end_bit);
+
+// Exclusive / inclusive scans (sum) of n values; out may alias in.
+template <typename T>
+void exclusive_scan(gpart_ctx* ctx, const T* in, T* out, std::int64_t n, T* total_device = nullptr);
+template <typename T>
+void inclusive_scan(gpart_ctx* ctx, const T* in, T* out, std::int64_t n);
+
+// Stable compaction of i in [0, n) with flags[i] != 0 into out; count written to
+// d_count (device).
+void compact_indices(gpart_ctx* ctx, const std::int32_t* flags, std::int64_t n, std::int32_t* out,
+                     std::int64_t* d_count);
+
+inline int bit_width_u64(std::uint64_t v) {
+    int b = 0;
+    while (v) {
+        ++b;
+        v >>= 1;
+    }
+    return b;
+}
+
+}  // namespace gpart_hip

@@ -1,0 +1,279 @@
+// primitives.hip -- project-written sort, scan and compaction for gfx950.
+//
+// Replaces the reference's CUB/Thrust calls (cub::DeviceRadixSort SortKeys /
+// SortPairs, thrust scans, copy_if; src/coarsen.cu:898-934,1017,1069-1111,
+// 1336-1339,1550; src/refine_incremental.cuh:277).
+//
+// Radix sort: stable LSD over 8-bit digits. Per pass:
+//   1. digit histogram of each 2048-key tile (LDS counters);
+//   2. exclusive scan of the digit-major [256 x tiles] histogram (global bases);
+//   3. scatter: each wave ranks its keys with a 64-lane multisplit (eight
+//      ballots give the lanes sharing a digit), waves are combined in LDS, the
+//      tile is staged in LDS in digit order and written out so that consecutive
+//      threads store consecutive addresses of one digit run.
+#include "internal.hpp"
+#include "scan.hpp"
+
+namespace gpart_hip {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kSortItems = 8;  // per lane
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kDigits = 256;
+
+__device__ __forceinline__ std::int64_t tile_item(int wave, int it, int lane) {
+    return static_cast<std::int64_t>(wave) * (kWave * kSortItems) + it * kWave + lane;
+}
+
+__global__ void __launch_bounds__(kSortThreads) radix_histogram_kernel(const std::uint64_t* keys, std::int64_t n, int shift,
+                                                                       std::uint64_t mask, std::uint32_t* hist,
+                                                                       std::int64_t tiles) {
+    __shared__ std::uint32_t counts[kSortWaves][kDigits];
+    for (int i = threadIdx.x; i < kSortWaves * kDigits; i += kSortThreads)
+        (&counts[0][0])[i] = 0;
+    __syncthreads();
+    const int wave = threadIdx.x / kWave, lane = lane_id();
+    const std::int64_t tile_start = static_cast<std::int64_t>(blockIdx.x) * kSortTile;
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const std::int64_t idx = tile_start + tile_item(wave, it, lane);
+        if (idx < n) {
+            const int d = static_cast<int>((keys[idx] >> shift) & mask);
+            atomicAdd(&counts[wave][d], 1U);
+        }
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kDigits; d += kSortThreads) {
+        std::uint32_t total = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) total += counts[w][d];
+        hist[static_cast<std::int64_t>(d) * tiles + blockIdx.x] = total;
+    }
+}
+
+template <typename V, bool HasValues>
+__global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const std::uint64_t* keys_in, std::uint64_t* keys_out,
+                                                                     const V* values_in, V* values_out, std::int64_t n,
+                                                                     int shift, std::uint64_t mask,
+                                                                     const std::uint32_t* offsets, std::int64_t tiles) {
+    __shared__ std::uint32_t wave_counts[kSortWaves][kDigits];
+    __shared__ std::uint32_t digit_start[kDigits + 1];
+    __shared__ std::uint32_t global_base[kDigits];
+    __shared__ std::uint64_t stage_keys[kSortTile];
+    __shared__ V stage_values[HasValues ? kSortTile : 1];
+    __shared__ std::uint32_t scan_waves[kSortWaves + 1];
+
+    const int wave = threadIdx.x / kWave, lane = lane_id();
+    for (int i = threadIdx.x; i < kSortWaves * kDigits; i += kSortThreads)
+        (&wave_counts[0][0])[i] = 0;
+    __syncthreads();
+    const std::int64_t tile_start = static_cast<std::int64_t>(blockIdx.x) * kSortTile;
+    const int tile_valid = static_cast<int>(n - tile_start < kSortTile ? n - tile_start : kSortTile);
+
+    std::uint64_t key[kSortItems];
+    V value[kSortItems];
+    int digit[kSortItems];
+    std::uint32_t rank[kSortItems];
+    const std::uint64_t below = lanemask_lt();
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const std::int64_t idx = tile_start + tile_item(wave, it, lane);
+        const bool valid = idx < n;
+        key[it] = valid ? keys_in[idx] : 0;
+        if constexpr (HasValues) value[it] = valid ? values_in[idx] : V{};
+        const int d = valid ? static_cast<int>((key[it] >> shift) & mask) : 0;
+        digit[it] = valid ? d : -1;
+        std::uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const std::uint64_t bits = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? bits : ~bits;
+        }
+        std::uint32_t before = 0;
+        if (valid) before = wave_counts[wave][d];
+        const std::uint32_t mine = static_cast<std::uint32_t>(__popcll(peers & below));
+        rank[it] = before + mine;
+        // the lowest peer publishes the new running count
+        if (valid && mine == 0) wave_counts[wave][d] = before + static_cast<std::uint32_t>(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: wave offsets within the tile, digit totals
+    std::uint32_t digit_total = 0;
+    if (threadIdx.x < kDigits) {
+        const int d = threadIdx.x;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            const std::uint32_t c = wave_counts[w][d];
+            wave_counts[w][d] = digit_total;
+            digit_total += c;
+        }
+        global_base[d] = offsets[static_cast<std::int64_t>(d) * tiles + blockIdx.x];
+    }
+    std::uint32_t total;
+    const std::uint32_t start = block_exclusive_scan(digit_total, SumOp<std::uint32_t>{}, &total, scan_waves);
+    if (threadIdx.x < kDigits) digit_start[threadIdx.x] = start;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        if (digit[it] < 0) continue;
+        const int d = digit[it];
+        const std::uint32_t pos = digit_start[d] + wave_counts[wave][d] + rank[it];
+        stage_keys[pos] = key[it];
+        if constexpr (HasValues) stage_values[pos] = value[it];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < tile_valid; j += kSortThreads) {
+        const std::uint64_t k = stage_keys[j];
+        const int d = static_cast<int>((k >> shift) & mask);
+        const std::int64_t out = static_cast<std::int64_t>(global_base[d]) + (j - static_cast<int>(digit_start[d]));
+        keys_out[out] = k;
+        if constexpr (HasValues) values_out[out] = stage_values[j];
+    }
+}
+
+template <typename V, bool HasValues>
+bool radix_sort_impl(gpart_ctx* ctx, std::uint64_t* keys, std::uint64_t* keys_alt, V* values, V* values_alt,
+                     std::int64_t n, int begin_bit, int end_bit) {
+    if (n <= 1 || end_bit <= begin_bit) return false;
+    if (n > (std::int64_t{1} << 32) - kSortTile)
+        throw Error(GPART_ERR_INVALID_ARGUMENT, "radix sort supports fewer than 2^32 keys per call");
+    const std::int64_t tiles = (n + kSortTile - 1) / kSortTile;
+    Scratch<std::uint32_t> hist(ctx, tiles * kDigits);
+    std::uint64_t* src_k = keys;
+    std::uint64_t* dst_k = keys_alt;
+    V* src_v = values;
+    V* dst_v = values_alt;
+    bool in_alt = false;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
+        const std::uint64_t mask = (1ULL << bits) - 1;
+        hipLaunchKernelGGL(radix_histogram_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0,
+                           ctx->stream, src_k, n, shift, mask, hist.get(), tiles);
+        GPART_LAUNCH_CHECK();
+        device_scan<std::uint32_t, SumOp<std::uint32_t>>(ctx, ArrayLoad<std::uint32_t>{hist.get()},
+                                                         ExclusiveStore<std::uint32_t>{hist.get()},
+                                                         tiles * kDigits, SumOp<std::uint32_t>{});
+        hipLaunchKernelGGL((radix_scatter_kernel<V, HasValues>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads),
+                           0, ctx->stream, src_k, dst_k, src_v, dst_v, n, shift, mask, hist.get(), tiles);
+        GPART_LAUNCH_CHECK();
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
+        in_alt = !in_alt;
+    }
+    return in_alt;
+}
+
+struct FlagLoad {
+    const std::int32_t* flags;
+    __device__ __forceinline__ std::int64_t operator()(std::int64_t i) const { return flags[i] != 0 ? 1 : 0; }
+};
+
+struct CompactStore {
+    std::int32_t* out;
+    std::int64_t n;
+    std::int64_t* count;
+    __device__ __forceinline__ void operator()(std::int64_t i, std::int64_t prefix, std::int64_t item) const {
+        if (item) out[prefix] = static_cast<std::int32_t>(i);
+        if (i == n - 1) *count = prefix + item;
+    }
+};
+
+}  // namespace
+
+template <typename V>
+bool radix_sort_pairs(gpart_ctx* ctx, std::uint64_t* keys, std::uint64_t* keys_alt, V* values, V* values_alt,
+                      std::int64_t n, int begin_bit, int end_bit) {
+    return radix_sort_impl<V, true>(ctx, keys, keys_alt, values, values_alt, n, begin_bit, end_bit);
+}
+template bool radix_sort_pairs<std::uint32_t>(gpart_ctx*, std::uint64_t*, std::uint64_t*, std::uint32_t*,
+                                              std::uint32_t*, std::int64_t, int, int);
+template bool radix_sort_pairs<std::uint64_t>(gpart_ctx*, std::uint64_t*, std::uint64_t*, std::uint64_t*,
+                                              std::uint64_t*, std::int64_t, int, int);
+
+bool radix_sort_keys(gpart_ctx* ctx, std::uint64_t* keys, std::uint64_t* keys_alt, std::int64_t n, int begin_bit,
+                     int end_bit) {
+    return radix_sort_impl<std::uint32_t, false>(ctx, keys, keys_alt, nullptr, nullptr, n, begin_bit, end_bit);
+}
+
+template <typename T>
+void exclusive_scan(gpart_ctx* ctx, const T* in, T* out, std::int64_t n, T* total_device) {
+    device_scan<T, SumOp<T>>(ctx, ArrayLoad<T>{in}, ExclusiveStore<T>{out}, n, SumOp<T>{});
+    (void)total_device;
+}
+template <typename T>
+void inclusive_scan(gpart_ctx* ctx, const T* in, T* out, std::int64_t n) {
+    device_scan<T, SumOp<T>>(ctx, ArrayLoad<T>{in}, InclusiveStore<T>{out}, n, SumOp<T>{});
+}
+template void exclusive_scan<std::uint64_t>(gpart_ctx*, const std::uint64_t*, std::uint64_t*, std::int64_t,
+                                            std::uint64_t*);
+template void exclusive_scan<std::int64_t>(gpart_ctx*, const std::int64_t*, std::int64_t*, std::int64_t,
+                                           std::int64_t*);
+template void exclusive_scan<std::uint32_t>(gpart_ctx*, const std::uint32_t*, std::uint32_t*, std::int64_t,
+                                            std::uint32_t*);
+template void inclusive_scan<std::uint64_t>(gpart_ctx*, const std::uint64_t*, std::uint64_t*, std::int64_t);
+
+void compact_indices(gpart_ctx* ctx, const std::int32_t* flags, std::int64_t n, std::int32_t* out,
+                     std::int64_t* d_count) {
+    if (n <= 0) {
+        fill_bytes(ctx, d_count, 0, sizeof(std::int64_t));
+        return;
+    }
+    device_scan<std::int64_t, SumOp<std::int64_t>>(ctx, FlagLoad{flags}, CompactStore{out, n, d_count}, n,
+                                                   SumOp<std::int64_t>{});
+}
+
+}  // namespace gpart_hip
+
+using namespace gpart_hip;
+
+extern "C" {
+
+int gpart_sort_pairs_u64(gpart_ctx* ctx, uint64_t* keys, void* values, int value_bytes, int64_t n,
+                         int begin_bit, int end_bit) {
+    return guarded([&] {
+        if (begin_bit < 0 || end_bit > 64 || begin_bit > end_bit)
+            throw Error(GPART_ERR_INVALID_ARGUMENT, "invalid radix sort bit range");
+        if (n <= 1) return;
+        Scratch<std::uint64_t> keys_alt(ctx, n);
+        bool alt = false;
+        if (!values) {
+            alt = radix_sort_keys(ctx, keys, keys_alt.get(), n, begin_bit, end_bit);
+            if (alt) GPART_HIP_CHECK(hipMemcpyAsync(keys, keys_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        } else if (value_bytes == 4) {
+            Scratch<std::uint32_t> values_alt(ctx, n);
+            alt = radix_sort_pairs(ctx, keys, keys_alt.get(), static_cast<std::uint32_t*>(values), values_alt.get(), n,
+                                   begin_bit, end_bit);
+            if (alt) {
+                GPART_HIP_CHECK(hipMemcpyAsync(keys, keys_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                GPART_HIP_CHECK(hipMemcpyAsync(values, values_alt.get(), n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            }
+        } else if (value_bytes == 8) {
+            Scratch<std::uint64_t> values_alt(ctx, n);
+            alt = radix_sort_pairs(ctx, keys, keys_alt.get(), static_cast<std::uint64_t*>(values), values_alt.get(), n,
+                                   begin_bit, end_bit);
+            if (alt) {
+                GPART_HIP_CHECK(hipMemcpyAsync(keys, keys_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                GPART_HIP_CHECK(hipMemcpyAsync(values, values_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            }
+        } else {
+            throw Error(GPART_ERR_INVALID_ARGUMENT, "value_bytes must be 4 or 8");
+        }
+    });
+}
+
+int gpart_exclusive_scan_u64(gpart_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n) {
+    return guarded([&] { exclusive_scan<std::uint64_t>(ctx, in, out, n); });
+}
+
+int gpart_compact_flags(gpart_ctx* ctx, const int32_t* flags, int64_t n, int32_t* out, int64_t* count) {
+    return guarded([&] {
+        Scratch<std::int64_t> d_count(ctx, 1);
+        compact_indices(ctx, flags, n, out, d_count.get());
+        *count = read_scalar(ctx, d_count.get());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// capi.cpp -- the high-level C ABI (gpart.h) over the C++ API.
+//
+// gpart_graph_partition reproduces the CLI pipeline of app/partition_main.cu:42-119:
+// production coarsening options, initial partition on the coarsest level,
+// uncoarsening with refinement, hierarchy release and allocator trim, with the
+// same timed scope (coarsen start -> memory released).
+#include "gpart.h"
+
+#include "coarsen.hpp"
+#include "initial_partition.hpp"
+#include "partition_capacity.hpp"
+#include "uncoarsen.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <variant>
+
+namespace {
+
+template <typename Types>
+struct GraphOf {
+    gpart::WeightedGraph<Types> host;
+    gpart::DeviceWeightedGraph<Types> device;
+};
+
+thread_local std::string capi_error;
+
+template <typename F>
+int guard(F&& body) {
+    try {
+        body();
+        return GPART_OK;
+    } catch (const std::invalid_argument& e) {
+        capi_error = e.what();
+        gpart_hip_set_error(capi_error.c_str());
+        return GPART_ERR_INVALID_ARGUMENT;
+    } catch (const std::overflow_error& e) {
+        capi_error = e.what();
+        gpart_hip_set_error(capi_error.c_str());
+        return GPART_ERR_OVERFLOW;
+    } catch (const std::exception& e) {
+        capi_error = e.what();
+        gpart_hip_set_error(capi_error.c_str());
+        return GPART_ERR_RUNTIME;
+    }
+}
+
+template <typename Types>
+gpart::WeightedGraph<Types> host_graph(std::int64_t n, const std::int64_t* indptr, std::int64_t m,
+                                       const std::int64_t* indices, const std::uint64_t* ew,
+                                       const std::uint64_t* vw) {
+    auto g = gpart::graph_from_int64<Types>(n, indptr, m, indices);
+    using W = typename Types::WeightT;
+    if (ew)
+        for (std::int64_t e = 0; e < m; ++e) {
+            if (static_cast<std::uint64_t>(static_cast<W>(ew[e])) != ew[e])
+                throw std::overflow_error("edge weight exceeds the graph width");
+            g.edge_weights[e] = static_cast<W>(ew[e]);
+        }
+    if (vw)
+        for (std::int64_t v = 0; v < n; ++v) {
+            if (static_cast<std::uint64_t>(static_cast<W>(vw[v])) != vw[v])
+                throw std::overflow_error("vertex weight exceeds the graph width");
+            g.vertex_weights[v] = static_cast<W>(vw[v]);
+        }
+    return g;
+}
+
+template <typename Types>
+void copy_level(const gpart::WeightedGraph<Types>& g, int64_t* offsets, int32_t* neighbors, uint64_t* ew,
+                uint64_t* vw) {
+    if (offsets) std::copy(g.offsets.begin(), g.offsets.end(), offsets);
+    if (neighbors) std::copy(g.neighbors.begin(), g.neighbors.end(), neighbors);
+    if (ew) std::copy(g.edge_weights.begin(), g.edge_weights.end(), ew);
+    if (vw) std::copy(g.vertex_weights.begin(), g.vertex_weights.end(), vw);
+}
+
+template <typename Types>
+void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify, int mode, int32_t* labels,
+                    gpart_partition_report* report) {
+    using clock = std::chrono::steady_clock;
+    auto* ctx = gpart::detail::context();
+    std::uint64_t total = 0;
+    for (const auto w : g.host.vertex_weights) total += static_cast<std::uint64_t>(w);
+    gpart::partition_capacity(total, parts, ratio);
+    gpart::CoarsenOptions co;
+    co.parts = parts;
+    co.seed = seed;
+    co.strict_verify = verify != 0;
+    co.cluster_size_factor = std::max(64, (512 + parts - 1) / parts);
+    co.pack_isolated = true;
+    gpart::DeviceWeightedGraph<Types> input = g.device;  // device copy, outside the timed scope
+    GPART_CHECK(gpart_ctx_synchronize(ctx));
+    const auto t0 = clock::now();
+    auto hierarchy = gpart::coarsen<Types>(std::move(input), co);
+    GPART_CHECK(gpart_ctx_synchronize(ctx));
+    const auto t1 = clock::now();
+    auto coarse = gpart::initial_partition_gpu<Types>(hierarchy.levels.back(), parts, ratio, seed,
+                                                      static_cast<gpart::SpectralMode>(mode));
+    const auto coarsest_vertices = hierarchy.levels.back().vertices();
+    std::uint64_t initial_cut = 0;
+    {
+        const auto view = hierarchy.levels.back().view();
+        GPART_CHECK(gpart_cut(ctx, &view, coarse.data(), &initial_cut));
+    }
+    GPART_CHECK(gpart_ctx_synchronize(ctx));
+    const auto t2 = clock::now();
+    const int levels = static_cast<int>(hierarchy.levels.size());
+    gpart::RefineOptions ro;
+    ro.parts = parts;
+    ro.imbalance_ratio = ratio;
+    ro.max_rounds = 32;
+    ro.strict_verify = verify != 0;
+    ro.refinement_growth_ratio = 2.0;
+    auto result = gpart::uncoarsen<Types>(hierarchy, std::move(coarse), ro);
+    GPART_CHECK(gpart_ctx_synchronize(ctx));
+    const auto t3 = clock::now();
+    hierarchy.levels.clear();
+    hierarchy.fine_to_coarse.clear();
+    auto partition = std::move(result.device_partition);
+    GPART_CHECK(gpart_release_cache(ctx));
+    const auto t4 = clock::now();
+    if (labels) {
+        const auto host = partition.to_host();
+        std::copy(host.begin(), host.end(), labels);
+    }
+    if (report) {
+        report->final_cut = result.final_cut;
+        report->max_part_weight =
+            *std::max_element(result.final_part_weights.begin(), result.final_part_weights.end());
+        report->capacity = gpart::partition_capacity(total, parts, ratio);
+        report->initial_cut = initial_cut;
+        report->coarsest_vertices = coarsest_vertices;
+        report->levels = levels;
+        int refined = 0;
+        for (const auto& l : result.levels) refined += l.refinement.rounds > 0 ? 1 : 0;
+        report->refined_levels = refined;
+        auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+        report->coarsen_seconds = secs(t0, t1);
+        report->initial_seconds = secs(t1, t2);
+        report->uncoarsen_seconds = secs(t2, t3);
+        report->free_seconds = secs(t3, t4);
+        report->algorithm_seconds = secs(t0, t4);
+    }
+}
+
+}  // namespace
+
+struct gpart_graph {
+    int width;
+    std::variant<GraphOf<gpart::StandardTypes>, GraphOf<gpart::BigTypes>> g;
+};
+
+struct gpart_hierarchy {
+    int width;
+    std::variant<gpart::Hierarchy<gpart::StandardTypes>, gpart::Hierarchy<gpart::BigTypes>> h;
+};
+
+extern "C" {
+
+int gpart_partition_capacity_c(uint64_t total, int parts, double ratio, uint64_t* out) {
+    return guard([&] { *out = gpart::partition_capacity(total, parts, ratio); });
+}
+
+gpart_ctx* gpart_thread_context(void) {
+    gpart_ctx* ctx = nullptr;
+    guard([&] { ctx = gpart::detail::context(); });
+    return ctx;
+}
+
+int gpart_trim(void) {
+    return guard([&] { gpart::detail::release_context_cache(); });
+}
+
+int gpart_graph_upload(int width, int64_t n, const int64_t* indptr, int64_t m, const int64_t* indices,
+                       const uint64_t* edge_weights, const uint64_t* vertex_weights, int validate, gpart_graph** out) {
+    return guard([&] {
+        auto* g = new gpart_graph{width, GraphOf<gpart::StandardTypes>{}};
+        try {
+            auto build = [&](auto tag) {
+                using T = decltype(tag);
+                GraphOf<T> holder;
+                holder.host = host_graph<T>(n, indptr, m, indices, edge_weights, vertex_weights);
+                if (validate) gpart::validate_weighted_csr(holder.host, true);
+                else gpart::validate_weighted_shape(holder.host, true);
+                holder.device = gpart::make_device_weighted(holder.host);
+                g->g = std::move(holder);
+            };
+            if (width == GPART_WIDTH_STANDARD) build(gpart::StandardTypes{});
+            else if (width == GPART_WIDTH_BIG) build(gpart::BigTypes{});
+            else throw std::invalid_argument("unknown graph width");
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+int gpart_graph_load(int width, const char* indptr_path, const char* indices_path, int validate, gpart_graph** out) {
+    return guard([&] {
+        auto* g = new gpart_graph{width, GraphOf<gpart::StandardTypes>{}};
+        try {
+            auto build = [&](auto tag) {
+                using T = decltype(tag);
+                GraphOf<T> holder;
+                holder.host = gpart::load_weighted_graph<T>(indptr_path, indices_path);
+                if (validate) gpart::validate_weighted_csr(holder.host, true);
+                else gpart::validate_weighted_shape(holder.host, true);
+                holder.device = gpart::make_device_weighted(holder.host);
+                g->g = std::move(holder);
+            };
+            if (width == GPART_WIDTH_STANDARD) build(gpart::StandardTypes{});
+            else if (width == GPART_WIDTH_BIG) build(gpart::BigTypes{});
+            else throw std::invalid_argument("unknown graph width");
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
+void gpart_graph_free(gpart_graph* g) { delete g; }
+
+int64_t gpart_graph_vertices(const gpart_graph* g) {
+    return std::visit([](const auto& x) { return x.host.vertices(); }, g->g);
+}
+
+int64_t gpart_graph_edges(const gpart_graph* g) {
+    return std::visit([](const auto& x) { return x.host.edges(); }, g->g);
+}
+
+int gpart_graph_cut(gpart_graph* g, const int32_t* labels, uint64_t* cut) {
+    return guard([&] {
+        std::visit(
+            [&](auto& x) {
+                gpart::DeviceVector<std::int32_t> d(std::vector<std::int32_t>(labels, labels + x.host.vertices()));
+                const auto view = x.device.view();
+                GPART_CHECK(gpart_cut(gpart::detail::context(), &view, d.data(), cut));
+            },
+            g->g);
+    });
+}
+
+int gpart_graph_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int verify, int spectral_mode,
+                          int32_t* labels, gpart_partition_report* report) {
+    return guard([&] {
+        if (spectral_mode < 0 || spectral_mode > 2) throw std::invalid_argument("unknown spectral mode");
+        std::visit([&](auto& x) { partition_impl(x, parts, ratio, seed, verify, spectral_mode, labels, report); }, g->g);
+    });
+}
+
+int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ratio, int max_levels,
+                        int cluster_size_factor, int pack_isolated, int verify, gpart_hierarchy** out) {
+    return guard([&] {
+        gpart::CoarsenOptions o;
+        o.parts = parts;
+        o.seed = seed;
+        o.stop_contraction_ratio = stop_ratio;
+        o.max_levels = max_levels;
+        o.cluster_size_factor = cluster_size_factor;
+        o.pack_isolated = pack_isolated != 0;
+        o.strict_verify = verify != 0;
+        std::visit(
+            [&](auto& x) {
+                auto h = gpart::coarsen(x.host, o);
+                *out = new gpart_hierarchy{g->width, std::move(h)};
+            },
+            g->g);
+    });
+}
+
+int gpart_hierarchy_read(int width, const char* path, gpart_hierarchy** out) {
+    return guard([&] {
+        if (width == GPART_WIDTH_STANDARD)
+            *out = new gpart_hierarchy{width, gpart::read_hierarchy<gpart::StandardTypes>(path)};
+        else
+            *out = new gpart_hierarchy{width, gpart::read_hierarchy<gpart::BigTypes>(path)};
+    });
+}
+
+int gpart_hierarchy_levels(const gpart_hierarchy* h) {
+    return std::visit([](const auto& x) { return static_cast<int>(x.levels.size()); }, h->h);
+}
+
+int gpart_hierarchy_level_size(const gpart_hierarchy* h, int level, int64_t* n, int64_t* m) {
+    return guard([&] {
+        std::visit(
+            [&](const auto& x) {
+                if (level < 0 || level >= static_cast<int>(x.levels.size()))
+                    throw std::invalid_argument("hierarchy level out of range");
+                *n = x.levels[level].vertices();
+                *m = x.levels[level].edges();
+            },
+            h->h);
+    });
+}
+
+int gpart_hierarchy_level_copy(const gpart_hierarchy* h, int level, int64_t* offsets, int32_t* neighbors,
+                               uint64_t* edge_weights, uint64_t* vertex_weights) {
+    return guard([&] {
+        std::visit(
+            [&](const auto& x) {
+                if (level < 0 || level >= static_cast<int>(x.levels.size()))
+                    throw std::invalid_argument("hierarchy level out of range");
+                copy_level(x.levels[level], offsets, neighbors, edge_weights, vertex_weights);
+            },
+            h->h);
+    });
+}
+
+int gpart_hierarchy_map_copy(const gpart_hierarchy* h, int level, int32_t* map) {
+    return guard([&] {
+        std::visit(
+            [&](const auto& x) {
+                if (level < 0 || level >= static_cast<int>(x.fine_to_coarse.size()))
+                    throw std::invalid_argument("hierarchy map out of range");
+                std::copy(x.fine_to_coarse[level].begin(), x.fine_to_coarse[level].end(), map);
+            },
+            h->h);
+    });
+}
+
+const char* gpart_hierarchy_stop_reason(const gpart_hierarchy* h) {
+    return std::visit([](const auto& x) { return x.stop_reason.c_str(); }, h->h);
+}
+
+int gpart_hierarchy_write(const gpart_hierarchy* h, const char* path) {
+    return guard([&] { std::visit([&](const auto& x) { gpart::write_hierarchy(x, path); }, h->h); });
+}
+
+void gpart_hierarchy_free(gpart_hierarchy* h) { delete h; }
+
+int gpart_hierarchy_uncoarsen(const gpart_hierarchy* h, const int32_t* coarsest_labels, int parts, double ratio,
+                              int max_rounds, double growth, int verify, int32_t* labels, uint64_t* final_cut) {
+    return guard([&] {
+        std::visit(
+            [&](const auto& x) {
+                gpart::RefineOptions o;
+                o.parts = parts;
+                o.imbalance_ratio = ratio;
+                o.max_rounds = max_rounds;
+                o.refinement_growth_ratio = growth;
+                o.strict_verify = verify != 0;
+                const auto& coarsest = x.levels.back();
+                std::vector<std::int32_t> c(coarsest_labels, coarsest_labels + coarsest.vertices());
+                auto r = gpart::refine_hierarchy_device(x, c, o);
+                std::copy(r.partition.begin(), r.partition.end(), labels);
+                if (final_cut) *final_cut = r.final_cut;
+            },
+            h->h);
+    });
+}
+
+int gpart_graph_initial_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int spectral_mode,
+                                  int32_t* labels, double* coords) {
+    return guard([&] {
+        if (spectral_mode < 0 || spectral_mode > 2) throw std::invalid_argument("unknown spectral mode");
+        std::visit(
+            [&](auto& x) {
+                using Types = std::conditional_t<std::is_same_v<std::decay_t<decltype(x)>, GraphOf<gpart::StandardTypes>>,
+                                                 gpart::StandardTypes, gpart::BigTypes>;
+                gpart::DeviceVector<double> c;
+                auto l = gpart::initial_partition_gpu<Types>(x.device, parts, ratio, seed,
+                                                             static_cast<gpart::SpectralMode>(spectral_mode),
+                                                             coords ? &c : nullptr);
+                const auto host = l.to_host();
+                std::copy(host.begin(), host.end(), labels);
+                if (coords) {
+                    const auto hc = c.to_host();
+                    std::copy(hc.begin(), hc.end(), coords);
+                }
+            },
+            g->g);
+    });
+}
+
+int gpart_graph_refine(gpart_graph* g, int32_t* labels, int parts, double ratio, int max_rounds, int stale_rounds,
+                       int verify, gpart_refine_report* report) {
+    return guard([&] {
+        std::visit(
+            [&](auto& x) {
+                gpart::RefineOptions o;
+                o.parts = parts;
+                o.imbalance_ratio = ratio;
+                o.max_rounds = max_rounds;
+                o.stale_rounds = stale_rounds;
+                o.strict_verify = verify != 0;
+                std::vector<std::int32_t> l(labels, labels + x.host.vertices());
+                gpart::RefineStats s;
+                gpart::refine_partition(x.host, l, o, &s);
+                std::copy(l.begin(), l.end(), labels);
+                if (report) {
+                    report->rounds = s.rounds;
+                    report->admission_iterations = s.admission_iterations;
+                    report->admission_fallbacks = s.admission_fallbacks;
+                    report->pad = 0;
+                    report->proposals = s.proposals;
+                    report->accepted = s.accepted;
+                    report->initial_cut = s.initial_cut;
+                    report->final_cut = s.final_cut;
+                    report->final_max_part_weight = s.final_max_part_weight;
+                }
+            },
+            g->g);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+// initial.cpp -- host driver of the recursive spectral bisection
+// (reference: src/initial_partition_gpu.cu:170-276).
+//
+// PlainFp64 keeps the reference's host task stack and per-task launch sequence
+// (split chosen on the host from the device prefix, as the reference does).
+// The batched modes run every task of one recursion depth in one launch; tasks
+// at one depth touch disjoint vertices and label values, so the processing
+// order does not change the result (src/initial_partition_gpu.cu:198-255).
+#include "initial_partition.hpp"
+#include "partition_capacity.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+#include <vector>
+
+namespace gpart {
+namespace {
+
+using U64 = std::uint64_t;
+
+// Split selection of src/initial_partition_gpu.cu:233-248 from a host copy of
+// the sorted segment's inclusive weight prefix. Returns the boundary.
+int choose_split(const std::vector<U64>& prefix, const gpart_bisect_task& task, U64 capacity, U64& left_weight) {
+    const int count = task.count;
+    const int left_parts = task.parts / 2, right_parts = task.parts - left_parts;
+    const U64 target = static_cast<U64>(static_cast<long double>(task.weight) * left_parts / task.parts);
+    const U64 left_limit = capacity > task.weight / left_parts ? task.weight : capacity * left_parts;
+    const U64 right_limit = capacity > task.weight / right_parts ? task.weight : capacity * right_parts;
+    const U64 min_left = task.weight > right_limit ? task.weight - right_limit : 0;
+    int boundary = static_cast<int>(std::lower_bound(prefix.begin(), prefix.begin() + count, target) - prefix.begin()) + 1;
+    boundary = std::max(left_parts, std::min(count - right_parts, boundary));
+    left_weight = prefix[boundary - 1];
+    if (left_weight > left_limit && boundary > left_parts) {
+        --boundary;
+        left_weight = prefix[boundary - 1];
+    }
+    if (left_weight < min_left && boundary < count - right_parts) {
+        ++boundary;
+        left_weight = prefix[boundary - 1];
+    }
+    if (left_weight > left_limit || left_weight < min_left)
+        throw std::runtime_error("GPU spectral split cannot satisfy capacity");
+    return boundary;
+}
+
+template <typename Types>
+void plain_task(const DeviceWeightedGraph<Types>& graph, DeviceVector<std::int32_t>& labels,
+                DeviceVector<std::int32_t>& order, const DeviceVector<U64>& vertex_weights, const gpart_bisect_task& task,
+                std::uint32_t seed, U64 capacity, DeviceVector<double>& x, DeviceVector<double>& y,
+                DeviceVector<double>& degree, DeviceVector<U64>& prefix, std::vector<gpart_bisect_task>& children) {
+    auto* ctx = detail::context();
+    const auto view = graph.view();
+    GPART_CHECK(gpart_spectral_task(ctx, &view, labels.data(), order.data(), vertex_weights.data(), &task, seed,
+                                    x.data(), y.data(), degree.data(), prefix.data()));
+    std::vector<U64> host(static_cast<std::size_t>(task.count));
+    GPART_CHECK(gpart_memcpy_dtoh(ctx, host.data(), prefix.data(), host.size() * sizeof(U64)));
+    U64 left_weight = 0;
+    const int boundary = choose_split(host, task, capacity, left_weight);
+    const int left_parts = task.parts / 2, right_parts = task.parts - left_parts;
+    GPART_CHECK(gpart_spectral_assign(ctx, order.data(), &task, boundary, left_parts, labels.data()));
+    children.push_back({task.begin + boundary, task.count - boundary, task.first_part + left_parts, right_parts,
+                        task.weight - left_weight});
+    children.push_back({task.begin, boundary, task.first_part, left_parts, left_weight});
+}
+
+}  // namespace
+
+template <typename Types>
+DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeightedGraph<Types>& graph, int parts,
+                                                            double imbalance, std::uint32_t seed, SpectralMode mode,
+                                                            DeviceVector<double>* coords) {
+    const auto start = std::chrono::steady_clock::now();
+    auto* ctx = detail::context();
+    const auto n64 = graph.vertices();
+    if (n64 <= 0 || n64 > std::numeric_limits<int>::max() - 255 || parts < 2 || parts > n64 || parts > 32 ||
+        !std::isfinite(imbalance) || imbalance < 1.0)
+        throw std::invalid_argument("invalid GPU initial partition input");
+    const int n = static_cast<int>(n64);
+    const auto view = graph.view();
+    U64 total = 0, max_weight = 0, min_weight = 0, edge_total = 0, edge_limited = 0;
+    GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &total, &max_weight, &min_weight));
+    GPART_CHECK(gpart_edge_weight_sum(ctx, &view, &edge_total, &edge_limited));
+    if (!total || total == ~U64{0} || edge_total == ~U64{0})
+        throw std::overflow_error("GPU initializer weight accumulation overflow");
+    const U64 capacity = partition_capacity(total, parts, imbalance);
+    if (max_weight > capacity) throw std::invalid_argument("a vertex exceeds partition capacity");
+
+    DeviceVector<std::int32_t> labels(static_cast<std::size_t>(n), 0);
+    DeviceVector<std::int32_t> order(static_cast<std::size_t>(n));
+    GPART_CHECK(gpart_sequence_i32(ctx, order.data(), n));
+    DeviceVector<U64> vertex_weights(static_cast<std::size_t>(n));
+    GPART_CHECK(gpart_widen_weights(ctx, graph.vertex_weights.data(), Types::kWidth, n, vertex_weights.data()));
+    DeviceVector<double> x(static_cast<std::size_t>(n)), y(static_cast<std::size_t>(n)),
+        degree(static_cast<std::size_t>(n));
+    DeviceVector<U64> prefix(static_cast<std::size_t>(n));
+
+    if (mode == SpectralMode::PlainFp64) {
+        std::vector<gpart_bisect_task> tasks{{0, n, 0, parts, total}};
+        while (!tasks.empty()) {
+            const auto task = tasks.back();
+            tasks.pop_back();
+            if (task.parts == 1) {
+                if (task.weight > capacity) throw std::runtime_error("GPU initializer infeasible balance");
+                continue;
+            }
+            plain_task(graph, labels, order, vertex_weights, task, seed, capacity, x, y, degree, prefix, tasks);
+        }
+    } else {
+        const int precision = mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64;
+        const auto max_task = gpart_spectral_depth_max_task();
+        std::vector<gpart_bisect_task> depth{{0, n, 0, parts, total}};
+        DeviceVector<gpart_bisect_task> device_tasks(32), device_next(64);
+        while (!depth.empty()) {
+            std::vector<gpart_bisect_task> batched, next;
+            for (const auto& t : depth) {
+                if (t.parts == 1) {
+                    if (t.weight > capacity) throw std::runtime_error("GPU initializer infeasible balance");
+                } else if (t.count > max_task) {
+                    std::vector<gpart_bisect_task> children;
+                    plain_task(graph, labels, order, vertex_weights, t, seed, capacity, x, y, degree, prefix, children);
+                    next.insert(next.end(), children.begin(), children.end());
+                } else {
+                    batched.push_back(t);
+                }
+            }
+            if (!batched.empty()) {
+                GPART_CHECK(gpart_memcpy_htod(ctx, device_tasks.data(), batched.data(),
+                                              batched.size() * sizeof(gpart_bisect_task)));
+                std::int32_t produced = 0, status = 0;
+                GPART_CHECK(gpart_spectral_depth(ctx, &view, labels.data(), order.data(), vertex_weights.data(),
+                                                 device_tasks.data(), static_cast<std::int32_t>(batched.size()), seed,
+                                                 capacity, precision, x.data(), device_next.data(), &produced,
+                                                 &status));
+                if (status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");
+                if (status == 2) throw std::runtime_error("GPU initializer infeasible balance");
+                std::vector<gpart_bisect_task> produced_tasks(static_cast<std::size_t>(produced));
+                if (produced)
+                    GPART_CHECK(gpart_memcpy_dtoh(ctx, produced_tasks.data(), device_next.data(),
+                                                  produced_tasks.size() * sizeof(gpart_bisect_task)));
+                next.insert(next.end(), produced_tasks.begin(), produced_tasks.end());
+            }
+            depth.swap(next);
+        }
+    }
+    // plain tasks leave their final coordinates in x; batched tasks write them there
+    if (coords) *coords = x;
+    DeviceVector<U64> loads(static_cast<std::size_t>(parts));
+    GPART_CHECK(gpart_part_weights(ctx, &view, labels.data(), parts, loads.data()));
+    const auto host_loads = loads.to_host();
+    if (*std::max_element(host_loads.begin(), host_loads.end()) > capacity)
+        throw std::runtime_error("GPU initializer balance validation failed");
+    U64 cut = 0;
+    GPART_CHECK(gpart_cut(ctx, &view, labels.data(), &cut));
+    static const char* names[] = {"gpu_spectral_plain_fp64", "gpu_spectral_batched_fp64", "gpu_spectral_batched_fp32"};
+    std::cout << "initial_partition_backend=" << names[static_cast<int>(mode)] << '\n'
+              << "coarsest_vertices=" << n << '\n'
+              << "coarsest_edge_entries=" << graph.edges() << '\n'
+              << "initial_partition_parts=" << parts << '\n'
+              << "initial_partition_edgecut=" << cut << '\n'
+              << "initial_partition_seconds="
+              << std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count() << '\n';
+    return labels;
+}
+
+template DeviceVector<std::int32_t> initial_partition_gpu<StandardTypes>(const DeviceWeightedGraph<StandardTypes>&, int,
+                                                                         double, std::uint32_t, SpectralMode,
+                                                                         DeviceVector<double>*);
+template DeviceVector<std::int32_t> initial_partition_gpu<BigTypes>(const DeviceWeightedGraph<BigTypes>&, int, double,
+                                                                    std::uint32_t, SpectralMode, DeviceVector<double>*);
+
+}  // namespace gpart

@@ -1,0 +1,130 @@
+/*
+ * gpart.h -- high-level C ABI of the partitioner (libgpart.so).
+ *
+ * The drop-in surface for foreign-language callers (ctypes, cgo, JNI): the same
+ * operations the reference exposes as C++ templates and CLIs, on plain host
+ * arrays. Status codes as in gpart_hip.h (0 ok, 1 invalid_argument,
+ * 2 overflow_error, 3 runtime_error); message via gpart_last_error().
+ *
+ * Reference counterparts:
+ *   gpart_graph_upload            load_weighted_graph + validate + make_device_weighted
+ *                                 (src/graph.cpp:47-223; app/partition_main.cu:42-66)
+ *   gpart_graph_partition         app/partition_main.cu:67-119 (timed region, production options)
+ *   gpart_graph_coarsen           coarsen() (include/coarsen.hpp:93-101; src/coarsen.cu:1906-2174)
+ *   gpart_graph_initial_partition initial_partition_gpu (include/initial_partition.hpp:9-12)
+ *   gpart_graph_refine            refine_partition (include/refine.hpp:62-66)
+ *   gpart_hierarchy_uncoarsen     uncoarsen (include/uncoarsen.hpp:10-13)
+ *   gpart_hierarchy_write         write_hierarchy (include/hierarchy.hpp:42-126)
+ *   gpart_partition_capacity_c    partition_capacity (include/partition_capacity.hpp:12-23)
+ */
+#ifndef GPART_H
+#define GPART_H
+
+#include <stdint.h>
+
+#include "gpart_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpart_graph gpart_graph;          /* device-resident input graph */
+typedef struct gpart_hierarchy gpart_hierarchy;  /* host copy of a coarsening hierarchy */
+typedef struct gpart_host_csr gpart_host_csr;    /* generated int64 CSR on the host */
+
+int gpart_partition_capacity_c(uint64_t total, int parts, double ratio, uint64_t* out);
+
+/* The calling thread's kernel-library context (for profiling hooks). */
+gpart_ctx* gpart_thread_context(void);
+/* Synchronise and return cached device memory (ScratchPoolSession::finish). */
+int gpart_trim(void);
+
+/* Upload an int64 CSR. edge_weights / vertex_weights may be NULL (unit weights, as
+ * the CLI loader sets them). validate: 0 shape checks, 1 strict (symmetry). */
+int gpart_graph_upload(int width, int64_t n, const int64_t* indptr, int64_t m, const int64_t* indices,
+                       const uint64_t* edge_weights, const uint64_t* vertex_weights, int validate,
+                       gpart_graph** out);
+/* Load the reference's binary int64 files (native byte order). */
+int gpart_graph_load(int width, const char* indptr_path, const char* indices_path, int validate,
+                     gpart_graph** out);
+void gpart_graph_free(gpart_graph* g);
+int64_t gpart_graph_vertices(const gpart_graph* g);
+int64_t gpart_graph_edges(const gpart_graph* g);
+int gpart_graph_cut(gpart_graph* g, const int32_t* labels, uint64_t* cut);
+
+typedef struct gpart_partition_report {
+    uint64_t final_cut;
+    uint64_t max_part_weight;
+    uint64_t capacity;
+    uint64_t initial_cut;
+    int64_t coarsest_vertices;
+    int32_t levels;
+    int32_t refined_levels;
+    double coarsen_seconds;
+    double initial_seconds;
+    double uncoarsen_seconds;
+    double free_seconds;
+    double algorithm_seconds; /* coarsen start -> hierarchy freed (the reference's scope) */
+} gpart_partition_report;
+
+/* Full pipeline with the CLI's production options (cluster_size_factor =
+ * max(64, ceil(512/k)), pack_isolated, 32 refinement rounds, growth 2.0). The
+ * device input is copied first (outside the timed region) so a graph can be
+ * partitioned repeatedly. spectral_mode: 0 plain FP64, 1 batched FP64, 2 batched
+ * FP32. labels (host, n entries) may be NULL. */
+int gpart_graph_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int verify, int spectral_mode,
+                          int32_t* labels, gpart_partition_report* report);
+
+/* Coarsening hierarchy (host copy). */
+int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ratio, int max_levels,
+                        int cluster_size_factor, int pack_isolated, int verify, gpart_hierarchy** out);
+int gpart_hierarchy_read(int width, const char* path, gpart_hierarchy** out);
+int gpart_hierarchy_levels(const gpart_hierarchy* h);
+int gpart_hierarchy_level_size(const gpart_hierarchy* h, int level, int64_t* n, int64_t* m);
+int gpart_hierarchy_level_copy(const gpart_hierarchy* h, int level, int64_t* offsets, int32_t* neighbors,
+                               uint64_t* edge_weights, uint64_t* vertex_weights);
+int gpart_hierarchy_map_copy(const gpart_hierarchy* h, int level, int32_t* map);
+const char* gpart_hierarchy_stop_reason(const gpart_hierarchy* h);
+int gpart_hierarchy_write(const gpart_hierarchy* h, const char* path);
+void gpart_hierarchy_free(gpart_hierarchy* h);
+int gpart_hierarchy_uncoarsen(const gpart_hierarchy* h, const int32_t* coarsest_labels, int parts, double ratio,
+                              int max_rounds, double growth, int verify, int32_t* labels, uint64_t* final_cut);
+
+/* Initial partition of a graph; coords (n doubles) may be NULL. */
+int gpart_graph_initial_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int spectral_mode,
+                                  int32_t* labels, double* coords);
+
+typedef struct gpart_refine_report {
+    int32_t rounds;
+    int32_t admission_iterations;
+    int32_t admission_fallbacks;
+    int32_t pad;
+    uint64_t proposals;
+    uint64_t accepted;
+    uint64_t initial_cut;
+    uint64_t final_cut;
+    uint64_t final_max_part_weight;
+} gpart_refine_report;
+
+/* Refine labels (in/out, host) of a graph. */
+int gpart_graph_refine(gpart_graph* g, int32_t* labels, int parts, double ratio, int max_rounds, int stale_rounds,
+                       int verify, gpart_refine_report* report);
+
+/* Graph generators (host, int64 CSR, rows sorted, no self loops, no duplicates). */
+gpart_host_csr* gpart_generate_grid2d(int64_t rows, int64_t cols);
+gpart_host_csr* gpart_generate_mesh3d(int64_t nx, int64_t ny, int64_t nz);
+/* R-MAT: 2^scale vertices, edge_factor * 2^scale samples, quadrant probabilities
+ * (a, b, c, 1-a-b-c), counter-based RNG seeded by seed; symmetrized, deduplicated,
+ * self loops dropped. */
+gpart_host_csr* gpart_generate_rmat(int scale, int64_t edge_factor, double a, double b, double c, uint64_t seed);
+int64_t gpart_host_csr_vertices(const gpart_host_csr* h);
+int64_t gpart_host_csr_entries(const gpart_host_csr* h);
+const int64_t* gpart_host_csr_indptr(const gpart_host_csr* h);
+const int64_t* gpart_host_csr_indices(const gpart_host_csr* h);
+int gpart_host_csr_write(const gpart_host_csr* h, const char* indptr_path, const char* indices_path);
+void gpart_host_csr_free(gpart_host_csr* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPART_H */

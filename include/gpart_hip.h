@@ -1,0 +1,302 @@
+/*
+ * gpart_hip.h -- C ABI of the gfx950 kernel library (libgpart_hip.so).
+ *
+ * This is the seam between the C++ host scheduling (libgpart.so, built with g++
+ * against this header only) and the hand-written HIP kernels. Every entry point
+ * takes plain pointers and sizes, enqueues work on the context's stream and
+ * returns an int status (0 = success). On failure, gpart_last_error() returns a
+ * thread-local message; no C++ exception crosses this boundary. Entry points
+ * whose outputs are host values ("host out") synchronise the stream; all others
+ * are stream-ordered and asynchronous.
+ *
+ * Each entry point names the reference code it replaces (paths relative to the
+ * reference repository root). The reference has no C ABI: its phases are C++
+ * templates calling CUDA kernels directly; these are the kernel families of
+ * src/coarsen.cu, src/initial_partition_gpu.cu, src/refine.cu and
+ * src/refine_incremental.cuh.
+ *
+ * Widths: GPART_WIDTH_STANDARD = int32 offsets / uint32 weights,
+ *         GPART_WIDTH_BIG      = int64 offsets / uint64 weights
+ * (include/graph_types.hpp:7-17). Vertex ids are always int32.
+ */
+#ifndef GPART_HIP_H
+#define GPART_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPART_OK 0
+#define GPART_ERR_INVALID_ARGUMENT 1 /* std::invalid_argument in the reference */
+#define GPART_ERR_OVERFLOW 2         /* std::overflow_error */
+#define GPART_ERR_RUNTIME 3          /* std::runtime_error, including device failures */
+
+#define GPART_WIDTH_STANDARD 0
+#define GPART_WIDTH_BIG 1
+
+/* ---------------- runtime (include/check.hpp, include/detail/scratch_pool.hpp) ---------- */
+
+typedef struct gpart_ctx gpart_ctx;
+
+const char* gpart_last_error(void);
+/* Record a message for gpart_last_error (used by layers above this ABI). */
+void gpart_hip_set_error(const char* message);
+int gpart_device_count(int* count);
+/* One stream (default priority) and a caching device allocator per context. */
+int gpart_ctx_create(int device, gpart_ctx** out);
+int gpart_ctx_destroy(gpart_ctx* ctx);
+int gpart_ctx_synchronize(gpart_ctx* ctx);
+void* gpart_ctx_stream(gpart_ctx* ctx);
+/* Stream-ordered caching allocator; replaces ScratchAllocator/ScratchPoolSession
+ * (include/detail/scratch_pool.hpp:15-106). Freed blocks are reused by later
+ * allocations on the same stream. */
+int gpart_device_alloc(gpart_ctx* ctx, size_t bytes, void** out);
+int gpart_device_free(gpart_ctx* ctx, void* ptr);
+/* Synchronise, then return every cached block to the device
+ * (ScratchPoolSession::finish, scratch_pool.hpp:42-49). */
+int gpart_release_cache(gpart_ctx* ctx);
+int gpart_memcpy_htod(gpart_ctx* ctx, void* dst, const void* src, size_t bytes); /* host out: sync */
+int gpart_memcpy_dtoh(gpart_ctx* ctx, void* dst, const void* src, size_t bytes); /* host out: sync */
+int gpart_memcpy_dtod(gpart_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Fill with a byte value using a kernel (safe under stream capture). */
+int gpart_memset(gpart_ctx* ctx, void* dst, int value, size_t bytes);
+/* iota: out[i] = i */
+int gpart_sequence_i32(gpart_ctx* ctx, int32_t* out, int64_t n);
+/* Widen vertex weights of either width to uint64. */
+int gpart_widen_weights(gpart_ctx* ctx, const void* weights, int width, int64_t n, uint64_t* out);
+
+/* Live kernel timing for roofline measurement: when enabled, every launch of the
+ * named family's main kernel is bracketed by HIP events on the context stream,
+ * and its algorithmic HBM bytes are accumulated. */
+#define GPART_PROFILE_SCLP_AFFINITY 0
+#define GPART_PROFILE_CONTRACT_SORT 1
+#define GPART_PROFILE_REFINE_CACHE 2
+#define GPART_PROFILE_REFINE_PROPOSALS 3
+#define GPART_PROFILE_REFINE_FILTER 4
+#define GPART_PROFILE_SPECTRAL 5
+#define GPART_PROFILE_FAMILIES 6
+int gpart_profile_enable(gpart_ctx* ctx, int family, int enable);
+/* host out: total milliseconds, launch count, algorithmic bytes */
+int gpart_profile_read(gpart_ctx* ctx, int family, double* total_ms, int64_t* launches,
+                       double* bytes);
+int gpart_profile_reset(gpart_ctx* ctx);
+
+/* ---------------- primitives (P1-P13 of the reference's CUB/Thrust calls) ---------------- */
+
+/* Stable LSD radix sort of (key, value) pairs over key bits [begin_bit, end_bit), in
+ * place (keys/values are overwritten with the sorted result). values may be NULL.
+ * value_bytes is 4 or 8. Replaces cub::DeviceRadixSort::SortKeys/SortPairs
+ * (src/coarsen.cu:898-923,1238-1246,1746-1755). */
+int gpart_sort_pairs_u64(gpart_ctx* ctx, uint64_t* keys, void* values, int value_bytes,
+                         int64_t n, int begin_bit, int end_bit);
+/* Exclusive prefix sum (thrust::exclusive_scan, src/coarsen.cu:1017,1486,1550). */
+int gpart_exclusive_scan_u64(gpart_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n);
+/* Stable compaction of indices with nonzero flags (thrust::copy_if,
+ * src/coarsen.cu:997; src/refine_incremental.cuh:277). host out: count. */
+int gpart_compact_flags(gpart_ctx* ctx, const int32_t* flags, int64_t n, int32_t* out,
+                        int64_t* count);
+
+/* ---------------- graph view ---------------- */
+
+typedef struct gpart_csr_view {
+    int64_t n;                    /* vertices */
+    int64_t m;                    /* directed CSR entries */
+    int32_t width;                /* GPART_WIDTH_* */
+    const void* offsets;          /* n + 1 entries of the width's offset type */
+    const int32_t* neighbors;     /* m */
+    const void* edge_weights;     /* m entries of the width's weight type */
+    const void* vertex_weights;   /* n entries of the width's weight type */
+} gpart_csr_view;
+
+/* Degree classes of one level (sclp_degree_class_kernel, src/coarsen.cu:204-216,
+ * and copy_if :994-1004). Lists are device arrays owned by the caller (n each). */
+typedef struct gpart_degree_classes {
+    int32_t* low;       /* degree <= low_max, 8 lanes per vertex */
+    int32_t* medium;    /* low_max < degree <= medium_max, one wave per vertex */
+    int32_t* high;      /* degree > medium_max, one workgroup per vertex */
+    int64_t low_count;
+    int64_t medium_count;
+    int64_t high_count;
+    int64_t high_edges;  /* sum of degrees of high vertices */
+    int64_t max_degree;
+} gpart_degree_classes;
+
+/* host out: counts. */
+int gpart_degree_classes_build(gpart_ctx* ctx, const gpart_csr_view* g,
+                               gpart_degree_classes* classes);
+
+/* ---------------- SCLP coarsening (src/coarsen.cu) ---------------- */
+
+/* record_bound_kernel (:44-70): saturating row weight sums, their maximum and the
+ * "rows not strictly ascending" flag. host out: *maximum, *unsorted. */
+int gpart_record_bounds(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* row_bounds,
+                        uint64_t* maximum, int* unsorted);
+/* contract_record_bounds (:72-85) with its maximum (:1690-1700). coarse_bounds has
+ * n_coarse entries and is overwritten. host out: *maximum. */
+int gpart_contract_record_bounds(gpart_ctx* ctx, int64_t n, const int32_t* map,
+                                 const uint64_t* fine_bounds, int64_t n_coarse,
+                                 uint64_t* coarse_bounds, uint64_t* maximum);
+
+typedef struct gpart_sclp_round {
+    uint32_t role_salt;        /* src/coarsen.cu:1152-1154 */
+    uint32_t tie_salt;         /* :1155-1157 */
+    uint32_t mover_threshold;  /* :1311-1315 */
+    int32_t filter_roles;
+    int32_t singleton;         /* round 0 on sorted duplicate-free rows (:1161) */
+} gpart_sclp_round;
+
+/* Affinity + decode of one LP round, fused: sclp_singleton_affinity_kernel,
+ * sclp_affinity_low_match_kernel, sclp_affinity_medium_hash_kernel, the hub
+ * sort/reduce path and sclp_decode_gain_kernel (:306-598, :1142-1303).
+ * Writes proposals (INVALID = 0x7fffffff), gains and ties for all n vertices. */
+int gpart_sclp_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* clusters,
+                         const gpart_degree_classes* classes, const gpart_sclp_round* round,
+                         int32_t* proposals, uint64_t* gains, uint32_t* ties);
+
+/* Admission + commit of one LP round: sclp_admission_keys_kernel, copy_if, the
+ * admission radix sort, decode, inclusive_scan_by_key and sclp_commit_kernel
+ * (:600-680, :1086-1121, :1329-1373). gain_bound bounds every gain (row weight
+ * bound). Updates clusters and cluster_weights.
+ * host out: *valid (proposals), *accepted, *nonempty (cluster count after). */
+int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals,
+                     const uint64_t* gains, const uint32_t* ties,
+                     const uint64_t* vertex_weights, uint64_t capacity, uint64_t gain_bound,
+                     int32_t* clusters, uint64_t* cluster_weights, int64_t* valid,
+                     int64_t* accepted, int64_t* nonempty);
+
+/* Isolated-vertex packing (:1410-1441, kernels :741-773).
+ * host out: *isolated (zero-degree count), *nonempty. */
+int gpart_sclp_isolated(gpart_ctx* ctx, const gpart_csr_view* g, const uint64_t* vertex_weights,
+                        uint64_t capacity, int32_t* clusters, uint64_t* cluster_weights,
+                        int64_t* isolated, int64_t* nonempty);
+
+/* Two-hop pairing from round-4 unfiltered proposals (:1443-1539).
+ * host out: *candidates, *pairable (pairs found), *merged, *nonempty. */
+int gpart_sclp_two_hop(gpart_ctx* ctx, int64_t n, const int32_t* proposals,
+                       const uint64_t* gains, const uint32_t* ties,
+                       const uint64_t* vertex_weights, uint64_t capacity, uint64_t gain_bound,
+                       int64_t merge_budget, int32_t* clusters, uint64_t* cluster_weights,
+                       int64_t* candidates, int64_t* pairable, int64_t* merged,
+                       int64_t* nonempty);
+
+/* Compaction to coarse ids (:1541-1577): map[v] = rank of clusters[v] among
+ * nonempty clusters; coarse vertex weights (width type, n_coarse entries).
+ * host out: *maximum_weight. */
+int gpart_sclp_compact(gpart_ctx* ctx, int64_t n, const int32_t* clusters,
+                       const uint64_t* cluster_weights, const gpart_csr_view* g, int32_t* map,
+                       int64_t n_coarse, void* coarse_vertex_weights, uint64_t* maximum_weight);
+
+/* CSR contraction (contract_typed :1674-1814): cross entries keyed by
+ * (map[v], map[u]), sorted, duplicates summed, self entries dropped.
+ * coarse_row_bound bounds every coarse row weight (UINT64_MAX when unknown) and
+ * selects the 8-byte packed record layout when it fits
+ * (include/detail/sclp_record.hpp:16-49). Output arrays are allocated from the
+ * context allocator and owned by the caller (free with gpart_device_free).
+ * host out: *coarse_m. */
+int gpart_contract(gpart_ctx* ctx, const gpart_csr_view* fine,
+                   const gpart_degree_classes* classes, const int32_t* map,
+                   int64_t n_coarse, uint64_t coarse_row_bound, void** coarse_offsets,
+                   int32_t** coarse_neighbors, void** coarse_edge_weights, int64_t* coarse_m);
+
+/* The contraction record layout for n vertices and a row-weight bound: vertex bits,
+ * weight bits (0 = unpacked 8-byte key + separate value) and weight mask
+ * (SclpRecordLayout::checked(n, bound, 1), include/detail/sclp_record.hpp:20-35).
+ * Host only; needs no device. */
+void gpart_record_layout(int64_t vertices, uint64_t bound, int* vertex_bits, int* weight_bits,
+                         uint64_t* weight_mask);
+
+/* ---------------- initial partition (src/initial_partition_gpu.cu) ---------------- */
+
+typedef struct gpart_bisect_task {
+    int32_t begin;       /* segment of order[] */
+    int32_t count;
+    int32_t first_part;  /* label of every vertex of the task */
+    int32_t parts;
+    uint64_t weight;
+} gpart_bisect_task;
+
+#define GPART_SPECTRAL_FP64 0   /* FP64 vectors, reference summation order */
+#define GPART_SPECTRAL_FP32 1   /* FP32 vectors/SpMV, FP64 moments/normalisation/prefix */
+
+/* Plain per-task form (the reference's launch structure :206-229): spectral_start,
+ * finish_moments, 65 normalize and 64 spectral_step + finish_moments launches, then
+ * the (x, id) sort of order[begin, begin+count) and the inclusive vertex-weight
+ * prefix of the sorted segment (prefix: count entries, device). */
+int gpart_spectral_task(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels,
+                        int32_t* order, const uint64_t* vertex_weights,
+                        const gpart_bisect_task* task, uint32_t seed, double* x, double* y,
+                        double* degree, uint64_t* prefix);
+/* assign_split (:149-153). */
+int gpart_spectral_assign(gpart_ctx* ctx, const int32_t* order, const gpart_bisect_task* task,
+                          int32_t boundary, int32_t left_parts, int32_t* labels);
+
+/* Batched form: every task of one recursion depth in one launch, one workgroup
+ * per task, vectors resident in LDS/L2; diffusion, (x, id) sort, weight prefix,
+ * feasible split and label assignment all on the device. tasks/next are device
+ * arrays. precision: GPART_SPECTRAL_FP64 (bit-identical to the plain form) or
+ * GPART_SPECTRAL_FP32. host out: *next_count, *status (0 ok, 1 infeasible split,
+ * 2 leaf over capacity). */
+int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* labels,
+                         int32_t* order, const uint64_t* vertex_weights,
+                         const gpart_bisect_task* tasks, int32_t task_count, uint32_t seed,
+                         uint64_t capacity, int precision, double* coords,
+                         gpart_bisect_task* next, int32_t* next_count, int32_t* status);
+/* Largest task (vertices) the batched form accepts. */
+int64_t gpart_spectral_depth_max_task(void);
+
+/* ---------------- refinement (src/refine.cu, src/refine_incremental.cuh) ---------------- */
+
+/* refine_cut_kernel + device_cut_value (src/refine.cu:52-87,117-142).
+ * host out: *cut (undirected; odd directed total is an error). */
+int gpart_cut(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, uint64_t* cut);
+/* accumulate_part_weights_kernel (src/refine.cu:102-115); part_weights (device,
+ * k entries) is overwritten. */
+int gpart_part_weights(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, int k,
+                       uint64_t* part_weights);
+/* project_partition_kernel (src/refine.cu:89-100). */
+int gpart_project(gpart_ctx* ctx, int64_t n, const int32_t* map, const int32_t* coarse_labels,
+                  int32_t* fine_labels);
+/* incremental_build_affinity (:7-25): cache[v*k + p]. */
+int gpart_refine_build_cache(gpart_ctx* ctx, const gpart_csr_view* g,
+                             const gpart_degree_classes* classes, const int32_t* labels, int k,
+                             uint64_t* cache);
+/* incremental_lp_proposals (:27-46) + incremental_priority_filter (:48-67) +
+ * compaction of movers (:276-279). host out: *count. */
+int gpart_refine_lp_phase(gpart_ctx* ctx, const gpart_csr_view* g,
+                          const gpart_degree_classes* classes, const uint64_t* cache,
+                          const int32_t* labels, int k, double filter, int32_t* locks,
+                          int32_t* targets, int64_t* gains, int32_t* flags, int32_t* movers,
+                          int64_t* count);
+/* incremental_update_cache_and_cut (:69-95) + incremental_commit (:97-109).
+ * host out: *delta (exact cut decrease), part_weights_host (k entries). */
+int gpart_refine_apply(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* movers,
+                       int64_t count, int32_t* labels, const int32_t* targets,
+                       const int32_t* flags, int k, uint64_t* cache, uint64_t* part_weights,
+                       int32_t* locks, int lp, int64_t* delta, uint64_t* part_weights_host);
+/* One rebalance selection: incremental_rebalance_proposals (:111-157) and the
+ * bucket quota/admission kernels (:167-232), ranking bucket members in vertex-id
+ * order. Writes targets and flags, compacts movers. host out: *count. */
+int gpart_refine_rebalance_select(gpart_ctx* ctx, const gpart_csr_view* g,
+                                  const uint64_t* cache, const int32_t* labels, int k,
+                                  const uint64_t* part_weights, uint64_t capacity,
+                                  int32_t* targets, int32_t* flags, int32_t* movers,
+                                  int64_t* count);
+/* Largest edge weight (src/refine_incremental.cuh:332). host out. */
+int gpart_max_edge_weight(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* maximum);
+/* Edge weight sums. host out: *checked saturates at UINT64_MAX (CheckedSum,
+ * src/initial_partition_gpu.cu:31-35); *limited saturates at INT64_MAX/2 + 1
+ * (RefinementWeightSum, src/refine.cu:367-383). */
+int gpart_edge_weight_sum(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* checked,
+                          uint64_t* limited);
+/* Saturating sum, maximum and minimum of vertex weights (src/coarsen.cu:953-958).
+ * host out. */
+int gpart_vertex_weight_stats(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* total,
+                              uint64_t* maximum, uint64_t* minimum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPART_HIP_H */

@@ -1,0 +1,209 @@
+"""GPU parity: every kernel family and the full pipeline against the CPU oracle.
+
+Integer work (SCLP proposals, hierarchies, contraction, refinement labels) must
+be bit-exact. The FP64 diffusion follows the same summation order as the
+oracle and is compared bit-for-bit as well.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*args):
+    print(*args, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu(hx):
+    if hx.device_count() < 1:
+        pytest.skip("no GPU")
+
+
+def hub_graph(n=6000, hub_degree=3000, seed=1):
+    """Ring + one hub of degree > 2048 (global hash path) + a medium hub."""
+    rng = np.random.default_rng(seed)
+    rows = [set() for _ in range(n)]
+
+    def add(a, b):
+        if a != b:
+            rows[a].add(b)
+            rows[b].add(a)
+
+    for v in range(n):
+        add(v, (v + 1) % n)
+    for u in rng.choice(np.arange(1, n), hub_degree, replace=False):
+        add(0, int(u))
+    for u in rng.choice(np.arange(2, n), 600, replace=False):
+        add(1, int(u))
+    indptr = [0]
+    indices = []
+    for r in rows:
+        indices.extend(sorted(r))
+        indptr.append(len(indices))
+    return np.array(indptr, np.int64), np.array(indices, np.int64)
+
+
+GRAPHS = {
+    "grid64": lambda hx: hx.generate_grid2d(64, 64),
+    "grid256": lambda hx: hx.generate_grid2d(256, 256),
+    "rmat12": lambda hx: hx.generate_rmat(12, 16),
+    "hub": lambda hx: hub_graph(),
+}
+
+
+@pytest.mark.parametrize("n", [1, 7, 2048, 2049, 100_000, 1_000_003])
+def test_sort_pairs_stable(hx, n):
+    say(f"case sort n={n}")
+    rng = np.random.default_rng(n)
+    for begin, end in [(0, 64), (0, 17), (20, 44)]:
+        keys = rng.integers(0, 2**63, n, dtype=np.uint64) if n > 8 else rng.integers(0, 4, n).astype(np.uint64)
+        vals = np.arange(n, dtype=np.uint32)
+        k, v = hx.device.sort_pairs(keys, vals, begin, end)
+        mask = np.uint64(((1 << (end - begin)) - 1) if end - begin < 64 else 2**64 - 1)
+        digit = (keys >> np.uint64(begin)) & mask
+        order = np.argsort(digit, kind="stable")
+        assert np.array_equal(v, vals[order])
+        assert np.array_equal(k, keys[order])
+
+
+@pytest.mark.parametrize("n", [1, 5, 2048, 4097, 3_000_001])
+def test_scan_and_compact(hx, n):
+    say(f"case scan n={n}")
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 1000, n, dtype=np.uint64)
+    ex = hx.device.exclusive_scan(x)
+    assert np.array_equal(ex, np.concatenate([[0], np.cumsum(x)[:-1]]).astype(np.uint64))
+    flags = (rng.random(n) < 0.3).astype(np.int32)
+    assert np.array_equal(hx.device.compact(flags), np.nonzero(flags)[0].astype(np.int32))
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+def test_sclp_proposals_match_oracle(hx, oracle, name):
+    say(f"case sclp_proposals graph={name}")
+    indptr, indices = GRAPHS[name](hx)
+    n = len(indptr) - 1
+    rng = np.random.default_rng(7)
+    ew = rng.integers(1, 5, len(indices)).astype(np.uint64)
+    # symmetric weights: weight of (u,v) from the unordered pair
+    src = np.repeat(np.arange(n), np.diff(indptr))
+    lo, hi = np.minimum(src, indices), np.maximum(src, indices)
+    ew = ((lo * 2654435761 + hi * 40503) % 5 + 1).astype(np.uint64)
+    dg = hx.device.DeviceGraph(indptr, indices, ew)
+    og = oracle.Graph(indptr, indices.astype(np.int32), ew)
+    for clusters_kind in ("identity", "random", "blocks"):
+        if clusters_kind == "identity":
+            clusters = np.arange(n, dtype=np.int32)
+        elif clusters_kind == "random":
+            clusters = rng.integers(0, max(1, n // 3), n).astype(np.int32)
+        else:
+            clusters = (np.arange(n) // 5).astype(np.int32)
+        for rnd, filt, thr in [(0, True, 2**31), (1, True, 1_500_000_000), (4, False, 0)]:
+            single = clusters_kind == "identity" and rnd == 0
+            p, g, t = hx.device.sclp_proposals(dg, clusters, 3, 1, rnd, filt, thr, singleton=single)
+            op, ogn, ot = oracle.sclp_proposals(og, clusters, 3, 1, rnd, filt, thr)
+            assert np.array_equal(p, op), (clusters_kind, rnd)
+            assert np.array_equal(g, ogn), (clusters_kind, rnd)
+            assert np.array_equal(t, ot), (clusters_kind, rnd)
+
+
+@pytest.mark.parametrize("name", ["grid64", "rmat12", "hub"])
+@pytest.mark.parametrize("width", ["standard", "big"])
+def test_contract_matches_oracle(hx, oracle, name, width):
+    say(f"case contract graph={name} width={width}")
+    indptr, indices = GRAPHS[name](hx)
+    n = len(indptr) - 1
+    og = oracle.Graph(indptr, indices.astype(np.int32))
+    mp, cw, coarse, cap, _ = oracle.aggregate(og, 4, 0, 0)
+    expect = oracle.contract(og, mp, coarse, cw).arrays()
+    dg = hx.device.DeviceGraph(indptr, indices, width=width)
+    for bound in (64, 2**64 - 1):  # packed and unpacked layouts
+        off, nbr, w = hx.device.contract(dg, mp, coarse, bound)
+        assert np.array_equal(off, expect[0])
+        assert np.array_equal(nbr, expect[1])
+        assert np.array_equal(w, expect[2])
+
+
+def compare_hierarchy(gh, oh):
+    assert len(gh) == len(oh)
+    for level in range(len(gh)):
+        a = gh.level(level)
+        b = oh.level(level).arrays()
+        for x, y, what in zip(a, b, ("offsets", "neighbors", "edge_weights", "vertex_weights")):
+            assert np.array_equal(x, y), (level, what)
+        if level + 1 < len(gh):
+            assert np.array_equal(gh.map(level), oh.map(level)), level
+    assert gh.stop_reason == oh.stop_reason
+
+
+@pytest.mark.parametrize("name", ["grid256", "rmat12", "hub"])
+@pytest.mark.parametrize("production", [False, True])
+def test_hierarchy_bit_exact(hx, oracle, name, production):
+    say(f"case hierarchy graph={name} production={production}")
+    indptr, indices = GRAPHS[name](hx)
+    k = 4
+    csf = max(64, (512 + k - 1) // k) if production else 256
+    g = hx.Graph(indptr, indices)
+    og = oracle.Graph(indptr, indices.astype(np.int32))
+    for seed in (0, 1):
+        gh = g.coarsen(k, seed=seed, cluster_size_factor=csf, pack_isolated=production)
+        oh = oracle.coarsen(og, k, seed=seed, cluster_size_factor=csf, pack_isolated=production)
+        compare_hierarchy(gh, oh)
+
+
+@pytest.mark.parametrize("mode", ["plain_fp64", "batched_fp64"])
+def test_initial_partition_bit_exact(hx, oracle, mode):
+    say(f"case initial mode={mode}")
+    indptr, indices = hx.generate_grid2d(256, 256)
+    og = oracle.Graph(indptr, indices.astype(np.int32))
+    oh = oracle.coarsen(og, 8, cluster_size_factor=64, pack_isolated=True)
+    off, adj, ew, vw = oh.level(len(oh) - 1).arrays()
+    coarse = hx.Graph(off, adj, edge_weights=ew, vertex_weights=vw)
+    oc = oracle.Graph(off, adj, ew, vw)
+    for k in (2, 3, 8):
+        labels, coords = coarse.initial_partition(k, 1.03, 0, mode, with_coords=True)
+        ol, ocoords, _ = oracle.initial_partition(oc, k, 1.03, 0, with_coords=True)
+        assert np.array_equal(labels, ol), k
+        assert np.array_equal(coords.view(np.uint64), ocoords.view(np.uint64)), k
+
+
+def test_refine_kats(hx):
+    say("case refine kats")
+    # tests/test_uncoarsen.cu:454-484 of the reference
+    ex = hx.Graph([0, 2, 5, 7, 10, 11, 12], [1, 3, 0, 2, 4, 1, 3, 0, 2, 5, 1, 3],
+                  edge_weights=[1, 4, 1, 4, 100, 4, 1, 4, 1, 100, 100, 100], vertex_weights=[1] * 6)
+    lab, rep = ex.refine([0, 0, 1, 1, 0, 1], 2, 1.03, 32, verify=True)
+    assert ex.cut(lab) == 2 and (lab == 0).sum() == 3
+    rp = hx.Graph([0, 2, 4, 5, 5, 7, 8, 8, 8], [1, 4, 0, 2, 1, 0, 5, 4],
+                  edge_weights=[1, 4, 1, 100, 100, 4, 100, 100], vertex_weights=[1] * 8)
+    lab, rep = rp.refine([0, 0, 0, 0, 1, 1, 1, 1], 2, 1.03, 32, verify=True)
+    assert rp.cut(lab) == 1 and (lab == 0).sum() == 4 and rep["admission_iterations"] > 0
+
+
+@pytest.mark.parametrize("k", [2, 4, 8, 32])
+def test_refine_matches_oracle(hx, oracle, k):
+    say(f"case refine oracle k={k}")
+    indptr, indices = hx.generate_grid2d(64, 64)
+    n = len(indptr) - 1
+    labels = (np.arange(n) * 7919 % k).astype(np.int32)  # balanced, poor cut
+    g = hx.Graph(indptr, indices)
+    og = oracle.Graph(indptr, indices.astype(np.int32))
+    gl, grep = g.refine(labels, k, 1.03, 32)
+    ol, orep = oracle.refine(og, labels, k, 1.03, 32)
+    assert grep["final_cut"] == orep.final_cut
+    assert np.array_equal(gl, ol)
+
+
+@pytest.mark.parametrize("name,k", [("grid256", 4), ("grid256", 32), ("rmat12", 8), ("hub", 4)])
+def test_partition_matches_oracle(hx, oracle, name, k):
+    say(f"case partition graph={name} k={k}")
+    indptr, indices = GRAPHS[name](hx)
+    g = hx.Graph(indptr, indices)
+    og = oracle.Graph(indptr, indices.astype(np.int32))
+    labels, rep = g.partition(k, 1.03, 0)
+    ol, orep = oracle.partition(og, k, 1.03, 0)
+    cap = oracle.partition_capacity(len(indptr) - 1, k, 1.03)
+    assert np.bincount(labels, minlength=k).max() <= cap
+    assert oracle.cut(og, labels) == rep["final_cut"]
+    assert rep["final_cut"] == orep.final_cut
+    assert np.array_equal(labels, ol)
