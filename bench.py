@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the multilevel k-way partitioner on one MI355X.
+
+A step is one complete partition (coarsening, initial partition, uncoarsening with
+refinement, hierarchy release) of a graph already resident in HBM, timed with the
+reference's scope (app/partition_main.cu:67-93: coarsen start -> hierarchy freed).
+
+Default workload: 3D 7-point mesh 128^3 (2,097,152 vertices, 12,484,608 directed
+entries), unit weights, standard width, k = 32, imbalance 1.03, seed 0.
+
+Prints one JSON line: metric algorithm_total_seconds (lower is better) with the edge
+cut, the CPU oracle's cut on the same input, the live roofline of the dominant
+kernel family and the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "mesh128_k32": dict(kind="mesh3d", size=(128, 128, 128), k=32),
+    "grid256_k4": dict(kind="grid2d", size=(256, 256), k=4),
+    "rmat22_k8": dict(kind="rmat", size=(22, 16), k=8),
+    "rmat18_k8": dict(kind="rmat", size=(18, 16), k=8),
+}
+
+
+def make_graph(hx, spec):
+    if spec["kind"] == "mesh3d":
+        return hx.generate_mesh3d(*spec["size"])
+    if spec["kind"] == "grid2d":
+        return hx.generate_grid2d(*spec["size"])
+    scale, ef = spec["size"]
+    return hx.generate_rmat(scale, ef, 0.57, 0.19, 0.19, 0)
+
+
+def cpu_baseline(indptr, indices, k, ratio, seed, threads):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_py as O
+
+    O.set_threads(threads)
+    g = O.Graph(indptr, indices.astype(np.int32))
+    t = time.perf_counter()
+    labels, res = O.partition(g, k, ratio, seed)
+    wall = time.perf_counter() - t
+    return res, wall, labels
+
+
+def main() -> int:
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="mesh128_k32", choices=sorted(WORKLOADS))
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--ratio", type=float, default=1.03)
+    p.add_argument("--spectral-mode", default="batched_fp64",
+                   choices=["plain_fp64", "batched_fp64", "batched_fp32"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
+    p.add_argument("--no-profile", action="store_true", help="skip the live per-family event timing")
+    args = p.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the partitioner is a single-GPU path; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        return 2
+
+    import horkxg_amd as hx
+
+    if hx.device_count() < 1:
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    spec = WORKLOADS[args.workload]
+    k = spec["k"]
+    indptr, indices = make_graph(hx, spec)
+    n, m = len(indptr) - 1, len(indices)
+    print(f"[bench] workload={args.workload} vertices={n} entries={m} k={k}", file=sys.stderr, flush=True)
+    graph = hx.Graph(indptr, indices)
+
+    families = list(hx.PROFILE_FAMILIES)
+    for _ in range(args.warmup):
+        graph.partition(k, args.ratio, args.seed, spectral_mode=args.spectral_mode, want_labels=False)
+        print("[bench] warmup step done", file=sys.stderr, flush=True)
+    if not args.no_profile:
+        hx.profile_reset()
+        for f in families:
+            hx.profile_enable(f, True)
+    reports = []
+    hx.trim()
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        labels, rep = graph.partition(k, args.ratio, args.seed, spectral_mode=args.spectral_mode,
+                                      want_labels=(step == args.steps - 1))
+        reports.append(rep)
+        print(f"[bench] step {step} algorithm_seconds={rep['algorithm_seconds']:.4f} cut={rep['final_cut']}",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    fam = {}
+    if not args.no_profile:
+        for f in families:
+            fam[f] = hx.profile_read(f)
+            hx.profile_enable(f, False)
+
+    alg = np.array([r["algorithm_seconds"] for r in reports])
+    cut = reports[-1]["final_cut"]
+    cap = reports[-1]["capacity"]
+    loads = np.bincount(labels, minlength=k)
+    independent_cut = graph.cut(labels)
+    assert independent_cut == cut, (independent_cut, cut)
+    assert loads.max() <= cap, (loads.max(), cap)
+
+    roofline = None
+    if fam:
+        dominant = max(fam, key=lambda f: fam[f]["total_ms"] if fam[f]["bytes"] > 0 else -1)
+        d = fam[dominant]
+        per_launch_s = d["total_ms"] / 1e3 / max(d["launches"], 1)
+        bytes_per_launch = d["bytes"] / max(d["launches"], 1)
+        achieved = bytes_per_launch / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
+                    "launches": d["launches"], "avg_launch_us": round(per_launch_s * 1e6, 2),
+                    "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+        roofline["families"] = {
+            f: {"ms": round(v["total_ms"] / max(args.steps, 1), 3), "launches": v["launches"],
+                "GBps": round(v["bytes"] / (v["total_ms"] / 1e3) / 1e9, 1) if v["total_ms"] > 0 and v["bytes"] > 0
+                else None}
+            for f, v in fam.items()}
+
+    cpu = None
+    reference_cut = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        res, cpu_wall, cpu_labels = cpu_baseline(indptr, indices, k, args.ratio, args.seed, threads)
+        reference_cut = int(res.final_cut)
+        cpu = {"value": round(res.total_seconds, 4), "unit": "s", "cores": threads, "kind": "port",
+               "sample": f"one full partition of {args.workload} (seed {args.seed}) by the CPU oracle, "
+                         f"same timed scope", "wall_s": round(cpu_wall, 3),
+               "labels_identical_to_gpu": bool(np.array_equal(cpu_labels, labels))}
+
+    line = {
+        "metric": "algorithm_total_seconds",
+        "value": round(float(alg.mean()), 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / max(args.steps, 1) * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "vertices": n, "directed_entries": m, "k": k,
+                   "imbalance": args.ratio, "seed": args.seed, "width": "standard",
+                   "spectral": args.spectral_mode},
+        "edge_cut": int(cut),
+        "reference_cut": reference_cut,
+        "cut_vs_reference": (round(cut / reference_cut, 5) if reference_cut else None),
+        "max_part_weight": int(loads.max()),
+        "capacity": int(cap),
+        "partitions_per_sec": round(1.0 / float(alg.mean()), 3),
+        "stage_seconds": {s: round(float(np.mean([r[s] for r in reports])), 5)
+                          for s in ("coarsen_seconds", "initial_seconds", "uncoarsen_seconds", "free_seconds")},
+        "algorithm_seconds_min": round(float(alg.min()), 6),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

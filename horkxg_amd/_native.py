@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -144,6 +145,10 @@ def host_lib():
         L.gpart_host_csr_write.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
         L.gpart_host_csr_free.argtypes = [C.c_void_p]
         L.gpart_trim.argtypes = []
+        L.gpart_set_log.argtypes = [C.c_int]
+        L.gpart_set_log.restype = None
+        # Report lines stay off under Python unless HORKXG_LOG=1.
+        L.gpart_set_log(1 if os.environ.get("HORKXG_LOG") == "1" else 0)
         _host = L
     return _host
 
@@ -169,6 +174,11 @@ def record_layout(vertices: int, bound: int):
     vb, wb, mask = C.c_int(), C.c_int(), C.c_uint64()
     hip_lib().gpart_record_layout(vertices, bound, C.byref(vb), C.byref(wb), C.byref(mask))
     return vb.value, wb.value, mask.value
+
+
+def set_log(enabled: bool) -> None:
+    """Switch the library's stdout report lines on or off (process-wide)."""
+    host_lib().gpart_set_log(int(bool(enabled)))
 
 
 def trim() -> None:

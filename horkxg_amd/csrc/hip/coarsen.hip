@@ -12,6 +12,7 @@
 // and fuse the reference's decode step (sclp_decode_gain_kernel) into the
 // reduction, writing proposal/gain/tie directly.
 #include "internal.hpp"
+#include "reduce.hpp"
 #include "scan.hpp"
 
 #include <algorithm>
@@ -98,34 +99,66 @@ __device__ __forceinline__ void decode(std::int32_t v, std::int32_t source, unsi
 
 // ---------------------------------------------------------------- classes
 
+// Three-way stable split of vertex ids by degree class, as one scan.
+struct Class3 {
+    long long c[3];
+};
+struct Class3Op {
+    __device__ __forceinline__ Class3 identity() const { return Class3{{0, 0, 0}}; }
+    __device__ __forceinline__ Class3 operator()(const Class3& a, const Class3& b) const {
+        return Class3{{a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]}};
+    }
+};
+
 template <typename O>
-__global__ void degree_class_kernel(std::int64_t n, const O* offsets, std::int32_t* low, std::int32_t* medium,
-                                    std::int32_t* high, unsigned long long* counters) {
-    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = v < n;
-    const std::int64_t deg = valid ? static_cast<std::int64_t>(offsets[v + 1] - offsets[v]) : 0;
-    const int cls = !valid ? -1 : deg <= kLowDegree ? 0 : deg <= kMediumDegree ? 1 : 2;
-    std::int32_t* lists[3] = {low, medium, high};
-    const std::uint64_t below = lanemask_lt();
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const std::uint64_t mask = __ballot(cls == c);
-        if (!mask) continue;
-        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-        unsigned long long base = 0;
-        if (lane_id() == leader) base = atomicAdd(counters + c, static_cast<unsigned long long>(__popcll(mask)));
-        base = shfl_u64(base, leader, kWave);
-        if (cls == c) lists[c][base + __popcll(mask & below)] = static_cast<std::int32_t>(v);
-    }
-    unsigned long long hub_edges = cls == 2 ? static_cast<unsigned long long>(deg) : 0;
-    unsigned long long max_deg = static_cast<unsigned long long>(deg);
-    hub_edges = group_sum_u64<kWave>(hub_edges);
-    max_deg = group_max_u64<kWave>(max_deg);
-    if (lane_id() == 0) {
-        if (hub_edges) atomicAdd(counters + 3, hub_edges);
-        atomicMax(counters + 4, max_deg);
-    }
+__device__ __forceinline__ int degree_class(const O* offsets, std::int64_t v) {
+    const std::int64_t deg = static_cast<std::int64_t>(offsets[v + 1] - offsets[v]);
+    return deg <= kLowDegree ? 0 : deg <= kMediumDegree ? 1 : 2;
 }
+
+template <typename O>
+struct ClassLoad {
+    const O* offsets;
+    __device__ __forceinline__ Class3 operator()(std::int64_t v) const {
+        Class3 r{{0, 0, 0}};
+        r.c[degree_class(offsets, v)] = 1;
+        return r;
+    }
+};
+
+template <typename O>
+struct ClassStore {
+    const O* offsets;
+    std::int32_t* low;
+    std::int32_t* medium;
+    std::int32_t* high;
+    std::int64_t n;
+    unsigned long long* counts;
+    __device__ __forceinline__ void operator()(std::int64_t v, const Class3& prefix, const Class3& item) const {
+        if (item.c[0]) low[prefix.c[0]] = static_cast<std::int32_t>(v);
+        else if (item.c[1]) medium[prefix.c[1]] = static_cast<std::int32_t>(v);
+        else high[prefix.c[2]] = static_cast<std::int32_t>(v);
+        if (v == n - 1)
+            for (int j = 0; j < 3; ++j) counts[j] = static_cast<unsigned long long>(prefix.c[j] + item.c[j]);
+    }
+};
+
+template <typename O>
+struct DegreeLoad {
+    const O* offsets;
+    __device__ __forceinline__ unsigned long long operator()(std::int64_t v) const {
+        return static_cast<unsigned long long>(offsets[v + 1] - offsets[v]);
+    }
+};
+
+template <typename O>
+struct HubEdgeLoad {
+    const O* offsets;
+    __device__ __forceinline__ unsigned long long operator()(std::int64_t v) const {
+        const auto d = static_cast<unsigned long long>(offsets[v + 1] - offsets[v]);
+        return d > static_cast<unsigned long long>(kMediumDegree) ? d : 0;
+    }
+};
 
 // ---------------------------------------------------------------- affinity
 
@@ -440,22 +473,6 @@ __global__ void admit_apply_kernel(std::int64_t p, const std::int32_t* order, co
     if (lane_id() == 0 && c) atomicAdd(accepted, c);
 }
 
-__global__ void count_nonzero_kernel(std::int64_t n, const unsigned long long* values, unsigned long long* count) {
-    __shared__ unsigned long long waves[kBlock / kWave];
-    unsigned long long local = 0;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x)
-        local += values[i] != 0;
-    local = group_sum_u64<kWave>(local);
-    if (lane_id() == 0) waves[threadIdx.x / kWave] = local;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long total = 0;
-        for (int w = 0; w < kBlock / kWave; ++w) total += waves[w];
-        if (total) atomicAdd(count, total);
-    }
-}
-
 // ---------------------------------------------------------------- isolated / two-hop
 
 template <typename O>
@@ -527,16 +544,21 @@ __global__ void two_hop_commit_kernel(std::int64_t count, const std::int32_t* or
                                       const std::uint64_t* ordered_weights, std::int32_t* clusters,
                                       unsigned long long* cluster_weights, unsigned long long* merged) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= count || !pair_flags[i] || static_cast<std::uint64_t>(positions[i]) >= budget) return;
-    const std::int32_t v = order[i];
-    const std::int32_t source = clusters[v];
-    const std::int32_t target = clusters[order[i - 1]];
-    if (source == target) return;
-    const unsigned long long w = ordered_weights[i];
-    atomicAdd(cluster_weights + source, 0ULL - w);
-    atomicAdd(cluster_weights + target, w);
-    clusters[v] = target;
-    atomicAdd(merged, 1ULL);
+    bool did = false;
+    if (i < count && pair_flags[i] && static_cast<std::uint64_t>(positions[i]) < budget) {
+        const std::int32_t v = order[i];
+        const std::int32_t source = clusters[v];
+        const std::int32_t target = clusters[order[i - 1]];
+        if (source != target) {
+            const unsigned long long w = ordered_weights[i];
+            atomicAdd(cluster_weights + source, 0ULL - w);
+            atomicAdd(cluster_weights + target, w);
+            clusters[v] = target;
+            did = true;
+        }
+    }
+    const unsigned long long c = static_cast<unsigned long long>(__popcll(__ballot(did)));
+    if (lane_id() == 0 && c) atomicAdd(merged, c);
 }
 
 // ---------------------------------------------------------------- compaction
@@ -566,18 +588,6 @@ __global__ void map_and_weights_kernel(std::int64_t n, const std::int32_t* clust
                   static_cast<unsigned long long>(vertex_weights[v]));
 }
 
-template <typename W>
-__global__ void max_weight_kernel(std::int64_t n, const W* weights, unsigned long long* out) {
-    unsigned long long local = 0;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
-        const auto w = static_cast<unsigned long long>(weights[i]);
-        local = w > local ? w : local;
-    }
-    local = group_max_u64<kWave>(local);
-    if (lane_id() == 0 && local) atomicMax(out, local);
-}
-
 // ---------------------------------------------------------------- record bounds
 
 template <typename O, typename W, int G>
@@ -604,10 +614,14 @@ __global__ void record_bound_kernel(std::int64_t count, const std::int32_t* list
     unsorted = group_any<G>(unsorted);
     if (sub == 0) {
         row_bounds[v] = degree;
-        atomicMax(meta, degree);
-        if (unsorted) atomicOr(meta + 1, 1ULL);
+        if (unsorted) atomicOr(meta + 1, 1ULL);  // rare: only unsorted inputs write it
     }
 }
+
+struct U64Load {
+    const unsigned long long* data;
+    __device__ __forceinline__ unsigned long long operator()(std::int64_t i) const { return data[i]; }
+};
 
 // contract_record_bounds (src/coarsen.cu:72-85): saturating add through the map.
 __global__ void contract_bounds_kernel(std::int64_t n, const std::int32_t* map, const unsigned long long* fine,
@@ -667,10 +681,7 @@ std::int32_t* admission_sort(gpart_ctx* ctx, std::int64_t n, std::int64_t p, std
 
 unsigned long long count_nonempty(gpart_ctx* ctx, std::int64_t n, const unsigned long long* weights) {
     Scratch<unsigned long long> c(ctx, 1);
-    fill_bytes(ctx, c.get(), 0, 8);
-    const unsigned blocks = std::min<unsigned>(grid_for(n), 2048);
-    hipLaunchKernelGGL(count_nonzero_kernel, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, weights, c.get());
-    GPART_LAUNCH_CHECK();
+    device_reduce<unsigned long long>(ctx, NonzeroLoad{weights}, n, SumU64{}, c.get());
     return read_scalar(ctx, c.get());
 }
 
@@ -743,9 +754,12 @@ void build_classes(gpart_ctx* ctx, const gpart_csr_view* g, gpart_degree_classes
     Scratch<unsigned long long> counters(ctx, 5);
     fill_bytes(ctx, counters.get(), 0, 40);
     if (g->n > 0) {
-        hipLaunchKernelGGL(degree_class_kernel<O>, dim3(grid_for(g->n)), dim3(kBlock), 0, ctx->stream, g->n,
-                           static_cast<const O*>(g->offsets), c->low, c->medium, c->high, counters.get());
-        GPART_LAUNCH_CHECK();
+        const auto* off = static_cast<const O*>(g->offsets);
+        device_scan<Class3, Class3Op>(ctx, ClassLoad<O>{off},
+                                      ClassStore<O>{off, c->low, c->medium, c->high, g->n, counters.get()}, g->n,
+                                      Class3Op{});
+        device_reduce<unsigned long long>(ctx, HubEdgeLoad<O>{off}, g->n, SumU64{}, counters.get() + 3);
+        device_reduce<unsigned long long>(ctx, DegreeLoad<O>{off}, g->n, MaxU64{}, counters.get() + 4);
     }
     unsigned long long h[5];
     read_back(ctx, h, counters.get(), sizeof(h));
@@ -778,6 +792,7 @@ void record_bounds_impl(gpart_ctx* ctx, const gpart_csr_view* g, std::uint64_t* 
             hipLaunchKernelGGL((record_bound_kernel<O, W, 64>), dim3(grid_for(c.high_count * 64)), dim3(kBlock), 0,
                                ctx->stream, c.high_count, c.high, off, g->neighbors, w, rb, meta.get());
         GPART_LAUNCH_CHECK();
+        device_reduce<unsigned long long>(ctx, U64Load{rb}, g->n, MaxU64{}, meta.get());
     }
     unsigned long long h[2];
     read_back(ctx, h, meta.get(), sizeof(h));
@@ -821,11 +836,7 @@ int gpart_contract_record_bounds(gpart_ctx* ctx, int64_t n, const int32_t* map, 
         }
         Scratch<unsigned long long> mx(ctx, 1);
         fill_bytes(ctx, mx.get(), 0, 8);
-        if (n_coarse > 0) {
-            hipLaunchKernelGGL(max_weight_kernel<unsigned long long>, dim3(std::min<unsigned>(grid_for(n_coarse), 2048)),
-                               dim3(kBlock), 0, ctx->stream, n_coarse, cb, mx.get());
-            GPART_LAUNCH_CHECK();
-        }
+        if (n_coarse > 0) device_reduce<unsigned long long>(ctx, U64Load{cb}, n_coarse, MaxU64{}, mx.get());
         *maximum = read_scalar(ctx, mx.get());
     });
 }
@@ -988,16 +999,16 @@ int gpart_sclp_compact(gpart_ctx* ctx, int64_t n, const int32_t* clusters, const
             hipLaunchKernelGGL(map_and_weights_kernel<std::uint32_t>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
                                n, clusters, compact.get(), static_cast<const std::uint32_t*>(g->vertex_weights), map,
                                static_cast<std::uint32_t*>(coarse_vertex_weights));
-            hipLaunchKernelGGL(max_weight_kernel<std::uint32_t>, dim3(std::min<unsigned>(grid_for(n_coarse), 2048)),
-                               dim3(kBlock), 0, ctx->stream, n_coarse,
-                               static_cast<const std::uint32_t*>(coarse_vertex_weights), mx.get());
+            device_reduce<unsigned long long>(
+                ctx, WidenLoad<std::uint32_t>{static_cast<const std::uint32_t*>(coarse_vertex_weights)}, n_coarse,
+                MaxU64{}, mx.get());
         } else {
             hipLaunchKernelGGL(map_and_weights_kernel<std::uint64_t>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream,
                                n, clusters, compact.get(), static_cast<const std::uint64_t*>(g->vertex_weights), map,
                                static_cast<std::uint64_t*>(coarse_vertex_weights));
-            hipLaunchKernelGGL(max_weight_kernel<std::uint64_t>, dim3(std::min<unsigned>(grid_for(n_coarse), 2048)),
-                               dim3(kBlock), 0, ctx->stream, n_coarse,
-                               static_cast<const std::uint64_t*>(coarse_vertex_weights), mx.get());
+            device_reduce<unsigned long long>(
+                ctx, WidenLoad<std::uint64_t>{static_cast<const std::uint64_t*>(coarse_vertex_weights)}, n_coarse,
+                MaxU64{}, mx.get());
         }
         GPART_LAUNCH_CHECK();
         *maximum_weight = read_scalar(ctx, mx.get());

@@ -40,12 +40,45 @@ Layout choose_layout(std::int64_t vertices, std::uint64_t bound) {
     return l;
 }
 
+// Pass 1: number of cross entries of each vertex (G lanes per vertex, G == kBlock
+// for one workgroup per vertex).
+template <typename O, int G>
+__global__ void __launch_bounds__(kBlock) cross_count_kernel(std::int64_t count, const std::int32_t* list,
+                                                             const O* offsets, const std::int32_t* neighbors,
+                                                             const std::int32_t* map, std::int64_t* cross) {
+    __shared__ unsigned long long waves[kBlock / kWave];
+    const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+    const int sub = threadIdx.x & (G - 1);
+    const bool active = group < count;
+    const std::int32_t v = active ? list[group] : 0;
+    unsigned long long c = 0;
+    if (active) {
+        const std::int32_t source = map[v];
+        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G)
+            c += map[neighbors[e]] != source;
+    }
+    if constexpr (G <= kWave) {
+        c = group_sum_u64<G>(c);
+        if (active && sub == 0) cross[v] = static_cast<std::int64_t>(c);
+    } else {
+        c = group_sum_u64<kWave>(c);
+        if (lane_id() == 0) waves[threadIdx.x / kWave] = c;
+        __syncthreads();
+        if (threadIdx.x == 0 && active) {
+            unsigned long long t = 0;
+            for (int w = 0; w < kBlock / kWave; ++w) t += waves[w];
+            cross[v] = static_cast<std::int64_t>(t);
+        }
+    }
+}
+
+// Pass 2: write each vertex's cross records at base[v] + its rank among them.
 template <typename O, typename W, typename V, int G>
 __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
                                                         const std::int32_t* neighbors, const W* weights,
-                                                        const std::int32_t* map, Layout layout, std::uint64_t* keys,
-                                                        V* values, unsigned long long* cursor) {
-    // G lanes per vertex; G == kBlock means one workgroup per vertex.
+                                                        const std::int32_t* map, const std::int64_t* base,
+                                                        Layout layout, std::uint64_t* keys, V* values) {
+    __shared__ unsigned int wave_counts[kBlock / kWave];
     const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
     const int sub = threadIdx.x & (G - 1);
     const bool active = group < count;
@@ -53,11 +86,11 @@ __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, cons
     const std::int64_t begin = active ? static_cast<std::int64_t>(offsets[v]) : 0;
     const std::int64_t end = active ? static_cast<std::int64_t>(offsets[v + 1]) : 0;
     const std::uint64_t source = active ? static_cast<std::uint32_t>(map[v]) : 0;
-    const std::uint64_t below = lanemask_lt();
-    // iterate in lock-step over the group's longest trip count so the whole wave
-    // takes part in each ballot
-    const std::int64_t my_trips = end - begin > sub ? (end - begin - sub + G - 1) / G : 0;
-    const auto trips = static_cast<std::int64_t>(group_max_u64<kWave>(static_cast<unsigned long long>(my_trips)));
+    std::int64_t next = active ? base[v] : 0;
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    // group-uniform trip count (a workgroup-wide maximum for G == kBlock)
+    const std::int64_t trips = end - begin > 0 ? (end - begin + G - 1) / G : 0;
     for (std::int64_t it = 0; it < trips; ++it) {
         const std::int64_t e = begin + sub + it * G;
         bool emit = false;
@@ -75,15 +108,33 @@ __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, cons
             }
         }
         const std::uint64_t mask = __ballot(emit);
-        if (!mask) continue;
-        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-        unsigned long long base = 0;
-        if (lane_id() == leader) base = atomicAdd(cursor, static_cast<unsigned long long>(__popcll(mask)));
-        base = shfl_u64(base, leader, kWave);
-        if (emit) {
-            const std::uint64_t pos = base + __popcll(mask & below);
-            keys[pos] = key;
-            if (layout.weight_bits == 0) values[pos] = static_cast<V>(w);
+        std::int64_t rank;
+        if constexpr (G <= kWave) {
+            const int first = lane & ~(G - 1);
+            const std::uint64_t group_bits = G == kWave ? ~0ULL : (((1ULL << G) - 1) << first);
+            const std::uint64_t gmask = mask & group_bits;
+            rank = __popcll(gmask & lanemask_lt());
+            if (emit) {
+                keys[next + rank] = key;
+                if (layout.weight_bits == 0) values[next + rank] = static_cast<V>(w);
+            }
+            next += __popcll(gmask);
+        } else {
+            if (lane == 0) wave_counts[wave] = static_cast<unsigned int>(__popcll(mask));
+            __syncthreads();
+            std::int64_t before = 0, total = 0;
+            for (int w2 = 0; w2 < kBlock / kWave; ++w2) {
+                const unsigned int c = wave_counts[w2];
+                before += w2 < wave ? c : 0;
+                total += c;
+            }
+            rank = before + __popcll(mask & lanemask_lt());
+            if (emit) {
+                keys[next + rank] = key;
+                if (layout.weight_bits == 0) values[next + rank] = static_cast<V>(w);
+            }
+            next += total;
+            __syncthreads();
         }
     }
 }
@@ -168,22 +219,36 @@ void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degre
         values.reset(ctx, std::max<std::int64_t>(m, 1));
         values_alt.reset(ctx, std::max<std::int64_t>(m, 1));
     }
-    Scratch<unsigned long long> cursor(ctx, 1);
-    fill_bytes(ctx, cursor.get(), 0, 8);
+    const std::int64_t n_fine = fine->n;
+    Scratch<std::int64_t> base(ctx, n_fine + 1);
+    fill_bytes(ctx, base.get() + n_fine, 0, 8);
+    if (classes->low_count)
+        hipLaunchKernelGGL((cross_count_kernel<O, 8>), dim3(grid_for(classes->low_count * 8)), dim3(kBlock), 0,
+                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, map, base.get());
+    if (classes->medium_count)
+        hipLaunchKernelGGL((cross_count_kernel<O, 64>), dim3(grid_for(classes->medium_count * 64)), dim3(kBlock), 0,
+                           ctx->stream, classes->medium_count, classes->medium, off, fine->neighbors, map, base.get());
+    if (classes->high_count)
+        hipLaunchKernelGGL((cross_count_kernel<O, kBlock>), dim3(static_cast<unsigned>(classes->high_count)),
+                           dim3(kBlock), 0, ctx->stream, classes->high_count, classes->high, off, fine->neighbors, map,
+                           base.get());
+    GPART_LAUNCH_CHECK();
+    // exclusive scan over n + 1 entries: base[n] becomes the cross-entry total
+    exclusive_scan<std::int64_t>(ctx, base.get(), base.get(), n_fine + 1);
     if (classes->low_count)
         hipLaunchKernelGGL((encode_kernel<O, W, V, 8>), dim3(grid_for(classes->low_count * 8)), dim3(kBlock), 0,
-                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, ew, map, layout,
-                           keys.get(), values.get(), cursor.get());
+                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, ew, map, base.get(),
+                           layout, keys.get(), values.get());
     if (classes->medium_count)
         hipLaunchKernelGGL((encode_kernel<O, W, V, 64>), dim3(grid_for(classes->medium_count * 64)), dim3(kBlock), 0,
-                           ctx->stream, classes->medium_count, classes->medium, off, fine->neighbors, ew, map, layout,
-                           keys.get(), values.get(), cursor.get());
+                           ctx->stream, classes->medium_count, classes->medium, off, fine->neighbors, ew, map,
+                           base.get(), layout, keys.get(), values.get());
     if (classes->high_count)
         hipLaunchKernelGGL((encode_kernel<O, W, V, kBlock>), dim3(static_cast<unsigned>(classes->high_count)),
                            dim3(kBlock), 0, ctx->stream, classes->high_count, classes->high, off, fine->neighbors, ew,
-                           map, layout, keys.get(), values.get(), cursor.get());
+                           map, base.get(), layout, keys.get(), values.get());
     GPART_LAUNCH_CHECK();
-    const auto cross = static_cast<std::int64_t>(read_scalar(ctx, cursor.get()));
+    const auto cross = read_scalar(ctx, base.get() + n_fine);
     const int shift = layout.weight_bits;
     const int key_bits = 2 * layout.vertex_bits;
     std::uint64_t* sorted_keys = keys.get();

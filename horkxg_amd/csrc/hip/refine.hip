@@ -11,6 +11,7 @@
 // rank is the weight of the bucket's lower-id members (a stable sort by bucket
 // and a segmented scan), so refinement is reproducible run to run.
 #include "internal.hpp"
+#include "reduce.hpp"
 #include "scan.hpp"
 
 #include <algorithm>
@@ -32,11 +33,13 @@ template <typename O, typename W, int G>
 __global__ void __launch_bounds__(kBlock) cut_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
                                                      const std::int32_t* neighbors, const W* weights,
                                                      const std::int32_t* labels, unsigned long long* directed) {
+    // grid-stride over vertex groups; one partial per workgroup in directed[blockIdx.x]
     __shared__ unsigned long long waves[kBlock / kWave];
-    const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
     const int sub = threadIdx.x & (G - 1);
+    const std::int64_t groups_per_grid = static_cast<std::int64_t>(gridDim.x) * (blockDim.x / G);
     unsigned long long local = 0;
-    if (group < count) {
+    for (std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; group < count;
+         group += groups_per_grid) {
         const std::int32_t v = list ? list[group] : static_cast<std::int32_t>(group);
         const std::int32_t p = labels[v];
         for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G)
@@ -48,7 +51,7 @@ __global__ void __launch_bounds__(kBlock) cut_kernel(std::int64_t count, const s
     if (threadIdx.x == 0) {
         unsigned long long total = 0;
         for (int w = 0; w < kBlock / kWave; ++w) total += waves[w];
-        if (total) atomicAdd(directed, total);
+        directed[blockIdx.x] = total;
     }
 }
 
@@ -287,8 +290,24 @@ __device__ __forceinline__ int gain_bucket(double gain, const PowerTable& t) {
 }
 
 // incremental_rebalance_proposals (:111-157) and the source bucket of each proposal.
+// Vertices of overloaded parts are the only possible proposers; everyone else is
+// reset here and skipped by the row scan below.
+__global__ void overloaded_flags_kernel(std::int64_t n, const std::int32_t* labels,
+                                        const unsigned long long* part_weights, int k, std::uint64_t capacity,
+                                        std::int32_t* flags, std::int32_t* targets, std::int32_t* buckets) {
+    __shared__ unsigned long long pw[32];
+    if (threadIdx.x < 32) pw[threadIdx.x] = threadIdx.x < k ? part_weights[threadIdx.x] : 0;
+    __syncthreads();
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    flags[v] = pw[labels[v]] > capacity;
+    targets[v] = -1;
+    buckets[v] = -1;
+}
+
 template <typename W, int G>
-__global__ void __launch_bounds__(kBlock) rebalance_proposals_kernel(std::int64_t n, const unsigned long long* cache,
+__global__ void __launch_bounds__(kBlock) rebalance_proposals_kernel(std::int64_t count, const std::int32_t* list,
+                                                                     const unsigned long long* cache,
                                                                      const std::int32_t* labels, const W* vertex_weights,
                                                                      int k, const unsigned long long* part_weights,
                                                                      std::uint64_t capacity, PowerTable powers,
@@ -296,18 +315,12 @@ __global__ void __launch_bounds__(kBlock) rebalance_proposals_kernel(std::int64_
     __shared__ unsigned long long pw[32];
     if (threadIdx.x < 32) pw[threadIdx.x] = threadIdx.x < k ? part_weights[threadIdx.x] : 0;
     __syncthreads();
-    const std::int64_t v = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+    const std::int64_t index = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
     const int p = threadIdx.x & (G - 1);
-    if (v >= n) return;  // uniform per group
+    if (index >= count) return;  // uniform per group
+    const std::int64_t v = list[index];
     const std::int32_t source = labels[v];
     const std::uint64_t weight = static_cast<std::uint64_t>(vertex_weights[v]);
-    if (pw[source] <= capacity) {
-        if (p == 0) {
-            targets[v] = -1;
-            buckets[v] = -1;
-        }
-        return;
-    }
     const bool ok = p < k && p != source && pw[p] < capacity && weight <= capacity - pw[p];
     unsigned long long conn = ok ? cache[v * k + p] : 0;
     unsigned long long slack = ok ? capacity - pw[p] : 0;
@@ -456,87 +469,6 @@ __global__ void select_destinations_kernel(std::int64_t n, const std::int32_t* b
     flags[v] = f;
 }
 
-// ---------------------------------------------------------------- reductions
-
-template <typename W>
-__global__ void max_u_kernel(std::int64_t n, const W* values, unsigned long long* out) {
-    unsigned long long local = 0;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
-        const auto w = static_cast<unsigned long long>(values[i]);
-        local = w > local ? w : local;
-    }
-    local = group_max_u64<kWave>(local);
-    if (lane_id() == 0 && local) atomicMax(out, local);
-}
-
-template <typename W>
-__global__ void min_u_kernel(std::int64_t n, const W* values, unsigned long long* out) {
-    unsigned long long local = ~0ULL;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
-        const auto w = static_cast<unsigned long long>(values[i]);
-        local = w < local ? w : local;
-    }
-#pragma unroll
-    for (int m = kWave / 2; m > 0; m /= 2) {
-        const auto o = shfl_xor_u64(local, m, kWave);
-        local = o < local ? o : local;
-    }
-    if (lane_id() == 0 && local != ~0ULL) atomicMin(out, local);
-}
-
-// RefinementWeightSum (src/refine.cu:367-372): saturates at INT64_MAX/2 + 1.
-template <typename W>
-__global__ void limited_sum_kernel(std::int64_t n, const W* values, unsigned long long* out) {
-    constexpr unsigned long long limit = 0x7fffffffffffffffULL / 2;
-    unsigned long long local = 0;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
-        const auto w = static_cast<unsigned long long>(values[i]);
-        local = (local > limit || w > limit || local > limit - w) ? limit + 1 : local + w;
-    }
-#pragma unroll
-    for (int m = kWave / 2; m > 0; m /= 2) {
-        const auto o = shfl_xor_u64(local, m, kWave);
-        local = (local > limit || o > limit || local > limit - o) ? limit + 1 : local + o;
-    }
-    if (lane_id() == 0 && local) {
-        unsigned long long prev = __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while (true) {
-            const unsigned long long sum =
-                (prev > limit || local > limit || prev > limit - local) ? limit + 1 : prev + local;
-            const unsigned long long seen = atomicCAS(out, prev, sum);
-            if (seen == prev) break;
-            prev = seen;
-        }
-    }
-}
-
-template <typename W>
-__global__ void checked_sum_kernel(std::int64_t n, const W* values, unsigned long long* out) {
-    unsigned long long local = 0;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
-        const auto w = static_cast<unsigned long long>(values[i]);
-        local = ~0ULL - local < w ? ~0ULL : local + w;
-    }
-#pragma unroll
-    for (int m = kWave / 2; m > 0; m /= 2) {
-        const auto o = shfl_xor_u64(local, m, kWave);
-        local = ~0ULL - local < o ? ~0ULL : local + o;
-    }
-    if (lane_id() == 0 && local) {
-        unsigned long long prev = __hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while (true) {
-            const unsigned long long sum = ~0ULL - prev < local ? ~0ULL : prev + local;
-            const unsigned long long seen = atomicCAS(out, prev, sum);
-            if (seen == prev) break;
-            prev = seen;
-        }
-    }
-}
-
 // ---------------------------------------------------------------- drivers
 
 int group_for_k(int k) {
@@ -571,9 +503,13 @@ struct Ops {
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
         if (g->n > 0) {
-            // 16 lanes per vertex over every vertex: one pass for validation-grade calls
-            hipLaunchKernelGGL((cut_kernel<O, W, 16>), dim3(grid_for(g->n * 16)), dim3(kBlock), 0, ctx->stream, g->n,
-                               nullptr, off, g->neighbors, w, labels, d.get());
+            // 16 lanes per vertex, capped grid, partials folded by one workgroup
+            const int blocks = static_cast<int>(std::min<std::int64_t>(grid_for(g->n * 16), kReduceMaxBlocks));
+            Scratch<unsigned long long> partials(ctx, blocks);
+            hipLaunchKernelGGL((cut_kernel<O, W, 16>), dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, nullptr, off,
+                               g->neighbors, w, labels, partials.get());
+            hipLaunchKernelGGL((reduce_finish_kernel<unsigned long long, SumU64>), dim3(1), dim3(kReduceThreads), 0,
+                               ctx->stream, partials.get(), blocks, SumU64{}, d.get());
             GPART_LAUNCH_CHECK();
         }
         const auto directed = read_scalar(ctx, d.get());
@@ -685,16 +621,24 @@ struct Ops {
         Scratch<unsigned long long> ranks(ctx, n), counts(ctx, static_cast<std::int64_t>(k) * kBalanceWidth),
             offsets(ctx, static_cast<std::int64_t>(k) * kBalanceWidth);
         Scratch<std::int64_t> d_count(ctx, 1);
-        const int G = group_for_k(k);
-        const unsigned blocks = grid_for(n * G);
-        switch (G) {
-            case 2: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 2>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-            case 4: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 4>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-            case 8: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 8>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-            case 16: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 16>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-            default: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 32>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-        }
+        hipLaunchKernelGGL(overloaded_flags_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, labels,
+                           part_weights, k, capacity, flags, targets, buckets.get());
         GPART_LAUNCH_CHECK();
+        compact_indices(ctx, flags, n, list.get(), d_count.get());
+        const std::int64_t candidates = read_scalar(ctx, d_count.get());
+        const int G = group_for_k(k);
+        const unsigned blocks = grid_for(candidates * G);
+        const std::int32_t* cl = list.get();
+        if (candidates > 0) {
+            switch (G) {
+                case 2: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 2>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
+                case 4: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 4>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
+                case 8: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 8>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
+                case 16: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 16>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
+                default: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 32>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
+            }
+            GPART_LAUNCH_CHECK();
+        }
         const int bucket_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(k) * kBalanceWidth - 1));
         // two ranking rounds: by source bucket, then by destination bucket
         for (int round = 0; round < 2; ++round) {
@@ -835,14 +779,14 @@ int gpart_max_edge_weight(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* max
         Scratch<unsigned long long> out(ctx, 1);
         fill_bytes(ctx, out.get(), 0, 8);
         if (g->m > 0) {
-            const unsigned blocks = std::min<unsigned>(grid_for(g->m), 2048);
             if (g->width == GPART_WIDTH_STANDARD)
-                hipLaunchKernelGGL(max_u_kernel<std::uint32_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->m,
-                                   static_cast<const std::uint32_t*>(g->edge_weights), out.get());
+                device_reduce<unsigned long long>(
+                    ctx, WidenLoad<std::uint32_t>{static_cast<const std::uint32_t*>(g->edge_weights)}, g->m, MaxU64{},
+                    out.get());
             else
-                hipLaunchKernelGGL(max_u_kernel<std::uint64_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->m,
-                                   static_cast<const std::uint64_t*>(g->edge_weights), out.get());
-            GPART_LAUNCH_CHECK();
+                device_reduce<unsigned long long>(
+                    ctx, WidenLoad<std::uint64_t>{static_cast<const std::uint64_t*>(g->edge_weights)}, g->m, MaxU64{},
+                    out.get());
         }
         *maximum = read_scalar(ctx, out.get());
     });
@@ -853,21 +797,15 @@ int gpart_edge_weight_sum(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* che
         Scratch<unsigned long long> out(ctx, 2);
         fill_bytes(ctx, out.get(), 0, 16);
         if (g->m > 0) {
-            const unsigned blocks = std::min<unsigned>(grid_for(g->m), 2048);
             if (g->width == GPART_WIDTH_STANDARD) {
-                const auto* w = static_cast<const std::uint32_t*>(g->edge_weights);
-                hipLaunchKernelGGL(checked_sum_kernel<std::uint32_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->m,
-                                   w, out.get());
-                hipLaunchKernelGGL(limited_sum_kernel<std::uint32_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->m,
-                                   w, out.get() + 1);
+                const WidenLoad<std::uint32_t> load{static_cast<const std::uint32_t*>(g->edge_weights)};
+                device_reduce<unsigned long long>(ctx, load, g->m, CheckedSumU64{}, out.get());
+                device_reduce<unsigned long long>(ctx, load, g->m, LimitedSumU64{}, out.get() + 1);
             } else {
-                const auto* w = static_cast<const std::uint64_t*>(g->edge_weights);
-                hipLaunchKernelGGL(checked_sum_kernel<std::uint64_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->m,
-                                   w, out.get());
-                hipLaunchKernelGGL(limited_sum_kernel<std::uint64_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->m,
-                                   w, out.get() + 1);
+                const WidenLoad<std::uint64_t> load{static_cast<const std::uint64_t*>(g->edge_weights)};
+                device_reduce<unsigned long long>(ctx, load, g->m, CheckedSumU64{}, out.get());
+                device_reduce<unsigned long long>(ctx, load, g->m, LimitedSumU64{}, out.get() + 1);
             }
-            GPART_LAUNCH_CHECK();
         }
         unsigned long long h[2];
         read_back(ctx, h, out.get(), sizeof(h));
@@ -880,34 +818,25 @@ int gpart_vertex_weight_stats(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t*
                               uint64_t* minimum) {
     return guarded([&] {
         Scratch<unsigned long long> out(ctx, 3);
-        fill_bytes(ctx, out.get(), 0, 16);
-        fill_bytes(ctx, out.get() + 2, 0xff, 8);
+        fill_bytes(ctx, out.get(), 0, 24);
         if (g->n > 0) {
-            const unsigned blocks = std::min<unsigned>(grid_for(g->n), 2048);
             if (g->width == GPART_WIDTH_STANDARD) {
-                const auto* w = static_cast<const std::uint32_t*>(g->vertex_weights);
-                hipLaunchKernelGGL(checked_sum_kernel<std::uint32_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, w,
-                                   out.get());
-                hipLaunchKernelGGL(max_u_kernel<std::uint32_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, w,
-                                   out.get() + 1);
-                hipLaunchKernelGGL(min_u_kernel<std::uint32_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, w,
-                                   out.get() + 2);
+                const WidenLoad<std::uint32_t> load{static_cast<const std::uint32_t*>(g->vertex_weights)};
+                device_reduce<unsigned long long>(ctx, load, g->n, CheckedSumU64{}, out.get());
+                device_reduce<unsigned long long>(ctx, load, g->n, MaxU64{}, out.get() + 1);
+                device_reduce<unsigned long long>(ctx, load, g->n, MinU64{}, out.get() + 2);
             } else {
-                const auto* w = static_cast<const std::uint64_t*>(g->vertex_weights);
-                hipLaunchKernelGGL(checked_sum_kernel<std::uint64_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, w,
-                                   out.get());
-                hipLaunchKernelGGL(max_u_kernel<std::uint64_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, w,
-                                   out.get() + 1);
-                hipLaunchKernelGGL(min_u_kernel<std::uint64_t>, dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, w,
-                                   out.get() + 2);
+                const WidenLoad<std::uint64_t> load{static_cast<const std::uint64_t*>(g->vertex_weights)};
+                device_reduce<unsigned long long>(ctx, load, g->n, CheckedSumU64{}, out.get());
+                device_reduce<unsigned long long>(ctx, load, g->n, MaxU64{}, out.get() + 1);
+                device_reduce<unsigned long long>(ctx, load, g->n, MinU64{}, out.get() + 2);
             }
-            GPART_LAUNCH_CHECK();
         }
         unsigned long long h[3];
         read_back(ctx, h, out.get(), sizeof(h));
         *total = h[0];
         *maximum = h[1];
-        *minimum = g->n > 0 ? h[2] : 0;
+        *minimum = h[2];
     });
 }
 

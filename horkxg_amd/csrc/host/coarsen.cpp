@@ -4,6 +4,7 @@
 //
 // The level loop, the LP-round loop and every stop rule are those of the
 // reference; each step below is one kernel-family call (gpart_hip.h).
+#include "log.hpp"
 #include "coarsen.hpp"
 
 #include <algorithm>
@@ -163,17 +164,17 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
         stats.proposal_count += static_cast<std::uint64_t>(valid);
         ++stats.rounds;
         stats.admission_seconds += since(admission_start);
-        std::cout << "ml_gpu_sclp_round level=" << level << " round=" << round << " proposals=" << valid;
+        gpart::detail::log() << "ml_gpu_sclp_round level=" << level << " round=" << round << " proposals=" << valid;
         if (diagnostics) {
             const auto cut_after = cluster_cut(graph, clusters);
             if (cut_after > cut_before) throw std::runtime_error("SCLP synchronous batch increased cluster cut");
-            std::cout << " accepted=" << accepted << " capacity_rejected=" << (valid - accepted)
+            gpart::detail::log() << " accepted=" << accepted << " capacity_rejected=" << (valid - accepted)
                       << " capacity_rejection_ratio="
                       << (valid == 0 ? 0.0 : static_cast<double>(valid - accepted) / static_cast<double>(valid))
                       << " cut_before=" << cut_before << " cut_after=" << cut_after
                       << " actual_gain=" << (cut_before - cut_after);
         }
-        std::cout << " mover_fraction=" << mover_fraction << " clusters=" << cluster_count
+        gpart::detail::log() << " mover_fraction=" << mover_fraction << " clusters=" << cluster_count
                   << " contraction_ratio=" << static_cast<double>(cluster_count) / static_cast<double>(n) << '\n';
         if (static_cast<double>(cluster_count) <= 0.60 * n) break;
     }
@@ -184,7 +185,7 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
                                         cluster_weights.data(), &isolated, &nonempty));
         if (isolated > 1) {
             cluster_count = nonempty;
-            std::cout << "ml_gpu_isolated_pack level=" << level << " isolated_vertices=" << isolated
+            gpart::detail::log() << "ml_gpu_isolated_pack level=" << level << " isolated_vertices=" << isolated
                       << " coarse_vertices=" << cluster_count << '\n';
         }
     }
@@ -208,7 +209,7 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
         if (diagnostics) {
             const auto cut_after = cluster_cut(graph, clusters);
             if (cut_after > cut_before) throw std::runtime_error("SCLP two-hop pairing increased cluster cut");
-            std::cout << "ml_gpu_sclp_two_hop level=" << level << " enabled=1 threshold=" << two_hop_threshold
+            gpart::detail::log() << "ml_gpu_sclp_two_hop level=" << level << " enabled=1 threshold=" << two_hop_threshold
                       << " triggered=1 singleton_with_favorite=" << candidates
                       << " pairable_singletons=" << 2 * pairable << " merged=" << merged
                       << " cut_before=" << cut_before << " cut_after=" << cut_after
@@ -228,14 +229,14 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
     if (out.maximum_weight > stats.capacity) throw std::runtime_error("SCLP coarsening exceeded cluster capacity");
     stats.compact_seconds = since(compact_start);
     stats.aggregate_seconds = since(start);
-    std::cout << "ml_gpu_sclp level=" << level << " fine_vertices=" << n << " coarse_vertices=" << out.coarse_vertices
+    gpart::detail::log() << "ml_gpu_sclp level=" << level << " fine_vertices=" << n << " coarse_vertices=" << out.coarse_vertices
               << " contraction_ratio=" << static_cast<double>(out.coarse_vertices) / static_cast<double>(n)
               << " beta=" << cluster_size_factor << " cluster_cap=" << stats.capacity << " lp_rounds=" << stats.rounds
               << " two_hop_enabled=1 two_hop_threshold=" << two_hop_threshold;
     if (diagnostics)
-        std::cout << " lp_accepted=" << stats.lp_accepted << " capacity_rejected=" << stats.capacity_rejected
+        gpart::detail::log() << " lp_accepted=" << stats.lp_accepted << " capacity_rejected=" << stats.capacity_rejected
                   << " two_hop_merged=" << stats.two_hop_merged;
-    std::cout << " weight_max=" << out.maximum_weight
+    gpart::detail::log() << " weight_max=" << out.maximum_weight
               << " weight_max_over_cap=" << static_cast<double>(out.maximum_weight) / static_cast<double>(stats.capacity)
               << " low_vertices=" << classes.low_count << " medium_vertices=" << classes.medium_count
               << " high_vertices=" << classes.high_count << " hub_edge_entries=" << classes.high_edges
@@ -280,7 +281,7 @@ DeviceWeightedGraph<Types> contract_impl(const DeviceWeightedGraph<Types>& fine,
         GPART_CHECK(gpart_device_free(ctx, weights));
     }
     coarse.vertex_weights = std::move(agg.vertex_weights);
-    std::cout << "ml_gpu_contract_cross_edge_entries=" << coarse_m << " input_edge_entries=" << fine.edges()
+    gpart::detail::log() << "ml_gpu_contract_cross_edge_entries=" << coarse_m << " input_edge_entries=" << fine.edges()
               << " packed_bound=" << coarse_row_bound << '\n';
     if (ws.record_bound_known) {
         ws.maximum_row_weight = coarse_row_bound;
@@ -330,7 +331,7 @@ void validate_coarsening_step(const WeightedGraph<Types>& fine, const WeightedGr
     for (std::size_t v = 0; v < map.size(); ++v) fine_labels[v] = coarse_labels[map[v]];
     if (host_cut(fine, fine_labels) != host_cut(coarse, coarse_labels))
         throw std::runtime_error("cut is not conserved by coarse projection");
-    std::cout << "ml_layer_verify level=" << level << " map=ok weights=ok csr=ok projection_cut=ok\n";
+    gpart::detail::log() << "ml_layer_verify level=" << level << " map=ok weights=ok csr=ok projection_cut=ok\n";
 }
 
 template <typename Types, bool Resident>
@@ -352,7 +353,7 @@ auto coarsen_impl(const WeightedGraph<Types>* graph, DeviceWeightedGraph<Types> 
         else validate_weighted_shape(hierarchy.levels.front(), true);
         input_verify_seconds = since(t);
     }
-    std::cout << "ml_input_verify_seconds=" << input_verify_seconds
+    gpart::detail::log() << "ml_input_verify_seconds=" << input_verify_seconds
               << " status=ok mode=" << (options.strict_verify ? "strict" : "fast") << '\n';
     const auto device_input_start = clock_type::now();
     if constexpr (!Resident) current = make_device_weighted(hierarchy.levels.front());
@@ -367,7 +368,7 @@ auto coarsen_impl(const WeightedGraph<Types>* graph, DeviceWeightedGraph<Types> 
         ws.record_bound_known = true;
         ws.sorted_unique_rows = unsorted == 0;
     }
-    std::cout << "ml_gpu_device_input_seconds=" << since(device_input_start) << " method=sclp\n";
+    gpart::detail::log() << "ml_gpu_device_input_seconds=" << since(device_input_start) << " method=sclp\n";
 
     const std::int64_t cutoff =
         std::max<std::int64_t>(32, options.cluster_size_factor * static_cast<std::int64_t>(options.parts));
@@ -376,23 +377,23 @@ auto coarsen_impl(const WeightedGraph<Types>* graph, DeviceWeightedGraph<Types> 
     int level = 0;
     for (; level < options.max_levels && current.vertices() > cutoff; ++level) {
         const auto level_seed = options.seed + static_cast<std::uint32_t>(level);
-        std::cout << "ml_coarsen_begin level=" << level << " vertices=" << current.vertices()
+        gpart::detail::log() << "ml_coarsen_begin level=" << level << " vertices=" << current.vertices()
                   << " edges=" << current.edges() << " method=sclp seed=" << level_seed << '\n';
         SclpStats stats;
         auto agg = aggregate(current, workspace, options.parts, level_seed, level, stats, diagnostics,
                              options.cluster_size_factor, options.pack_isolated);
         const double contraction = static_cast<double>(agg.coarse_vertices) / static_cast<double>(current.vertices());
-        std::cout << "ml_coarsen_map level=" << level << " coarse_vertices=" << agg.coarse_vertices
+        gpart::detail::log() << "ml_coarsen_map level=" << level << " coarse_vertices=" << agg.coarse_vertices
                   << " ratio=" << contraction << " maximum_weight=" << agg.maximum_weight
                   << " capacity=" << agg.capacity << '\n';
         if (agg.coarse_vertices >= current.vertices() || contraction > options.stop_contraction_ratio) {
-            std::cout << "ml_coarsen_stop reason=insufficient_contraction\n";
+            gpart::detail::log() << "ml_coarsen_stop reason=insufficient_contraction\n";
             hierarchy.stop_reason = "insufficient_contraction";
             break;
         }
         ContractionTimings timings;
         auto coarse = contract_impl(current, std::move(agg), workspace, timings);
-        std::cout << "ml_gpu_contract_seconds level=" << level << " seconds=" << timings.total_seconds
+        gpart::detail::log() << "ml_gpu_contract_seconds level=" << level << " seconds=" << timings.total_seconds
                   << " coarse_edges=" << coarse.edges() << '\n';
         const auto snapshot_start = clock_type::now();
         if constexpr (Resident) {
@@ -419,7 +420,7 @@ auto coarsen_impl(const WeightedGraph<Types>* graph, DeviceWeightedGraph<Types> 
     if constexpr (Resident) hierarchy.levels.push_back(std::move(current));
     hierarchy.hierarchy_loop_seconds = since(core_start);
     hierarchy.snapshot_seconds = snapshot_seconds;
-    std::cout << "ml_hierarchy_loop_seconds=" << hierarchy.hierarchy_loop_seconds
+    gpart::detail::log() << "ml_hierarchy_loop_seconds=" << hierarchy.hierarchy_loop_seconds
               << " includes_snapshot_and_verify=1 method=sclp\n";
     return hierarchy;
 }

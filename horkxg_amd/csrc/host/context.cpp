@@ -4,18 +4,41 @@
 // CUDA device. Here each host thread owns one gpart_ctx (one stream, one caching
 // allocator), created on first use on device 0 or bound explicitly.
 #include "device_vector.hpp"
+#include "gpart.h"
+#include "log.hpp"
+
+#include <atomic>
+#include <iostream>
+#include <streambuf>
+#include <thread>
 
 namespace gpart::detail {
 namespace {
+// The loading thread's context is left to process teardown: destroying a HIP
+// stream from a thread_local destructor at exit races the runtime's own
+// shutdown. Contexts of other threads are destroyed when their thread ends.
+const std::thread::id loading_thread = std::this_thread::get_id();
+
 struct ThreadContext {
     gpart_ctx* ctx = nullptr;
     bool owned = false;
     ~ThreadContext() {
-        if (owned && ctx) gpart_ctx_destroy(ctx);
+        if (owned && ctx && std::this_thread::get_id() != loading_thread) gpart_ctx_destroy(ctx);
     }
 };
 thread_local ThreadContext current;
+
+class NullBuffer : public std::streambuf {
+protected:
+    int overflow(int c) override { return c == traits_type::eof() ? 0 : c; }
+    std::streamsize xsputn(const char*, std::streamsize n) override { return n; }
+};
+NullBuffer null_buffer;
+std::ostream null_stream(&null_buffer);
+std::atomic<bool> log_enabled{true};
 }  // namespace
+
+std::ostream& log() { return log_enabled.load(std::memory_order_relaxed) ? std::cout : null_stream; }
 
 gpart_ctx* context() {
     if (!current.ctx) {
@@ -36,3 +59,5 @@ void release_context_cache() {
 }
 
 }  // namespace gpart::detail
+
+extern "C" void gpart_set_log(int enabled) { gpart::detail::log_enabled.store(enabled != 0); }

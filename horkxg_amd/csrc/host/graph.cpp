@@ -1,5 +1,6 @@
 // graph.cpp -- input boundary: int64 CSR files, validation, host cut, transfers
 // (reference: src/graph.cpp:47-241; same checks, messages and unit weights).
+#include "log.hpp"
 #include "graph.hpp"
 
 #include <algorithm>
@@ -104,7 +105,7 @@ WeightedGraph<Types> load_weighted_graph(const std::string& indptr_path, const s
     }
     graph.edge_weights.assign(static_cast<std::size_t>(edge_count), WeightT{1});
     graph.vertex_weights.assign(static_cast<std::size_t>(vertex_count), WeightT{1});
-    std::cout << "vertices=" << vertex_count << " edge_entries=" << edge_count << " device_count=1\n";
+    gpart::detail::log() << "vertices=" << vertex_count << " edge_entries=" << edge_count << " device_count=1\n";
     return graph;
 }
 

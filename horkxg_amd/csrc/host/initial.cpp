@@ -6,6 +6,7 @@
 // The batched modes run every task of one recursion depth in one launch; tasks
 // at one depth touch disjoint vertices and label values, so the processing
 // order does not change the result (src/initial_partition_gpu.cu:198-255).
+#include "log.hpp"
 #include "initial_partition.hpp"
 #include "partition_capacity.hpp"
 
@@ -156,7 +157,7 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
     U64 cut = 0;
     GPART_CHECK(gpart_cut(ctx, &view, labels.data(), &cut));
     static const char* names[] = {"gpu_spectral_plain_fp64", "gpu_spectral_batched_fp64", "gpu_spectral_batched_fp32"};
-    std::cout << "initial_partition_backend=" << names[static_cast<int>(mode)] << '\n'
+    gpart::detail::log() << "initial_partition_backend=" << names[static_cast<int>(mode)] << '\n'
               << "coarsest_vertices=" << n << '\n'
               << "coarsest_edge_entries=" << graph.edges() << '\n'
               << "initial_partition_parts=" << parts << '\n'

@@ -2,6 +2,7 @@
 // (reference: src/refine_incremental.cuh:235-361 run_incremental_refinement,
 // src/refine.cu:162-660 contexts, gain-range validation, refine_partition and
 // refine_hierarchy_impl).
+#include "log.hpp"
 #include "refine.hpp"
 
 #include "partition_capacity.hpp"
@@ -233,7 +234,7 @@ DeviceUncoarsenResult<Types> refine_hierarchy_impl(H& hierarchy, P&& coarsest_pa
     const auto stats = run_refinement(current, options, options.max_rounds);
     std::uint64_t last_cut = stats.final_cut;
     auto last_refined = current.graph.vertices();
-    std::cout << "refine_coarsest vertices=" << current.graph.vertices() << " initial_cut=" << stats.initial_cut
+    gpart::detail::log() << "refine_coarsest vertices=" << current.graph.vertices() << " initial_cut=" << stats.initial_cut
               << " final_cut=" << stats.final_cut << " rounds=" << stats.rounds << " seconds=" << stats.seconds << '\n';
     for (std::size_t coarse_level = hierarchy.levels.size() - 1; coarse_level > 0; --coarse_level) {
         const auto fine_level = coarse_level - 1;

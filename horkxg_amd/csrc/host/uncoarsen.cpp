@@ -1,5 +1,6 @@
 // uncoarsen.cpp -- projection + refinement driver and its report lines
 // (reference: src/uncoarsen.cu:39-112).
+#include "log.hpp"
 #include "uncoarsen.hpp"
 
 #include <algorithm>
@@ -36,7 +37,7 @@ DeviceUncoarsenResult<Types> uncoarsen_impl(H& hierarchy, P&& coarsest_partition
     for (const auto& level : result.levels) {
         const auto& s = level.refinement;
         const auto& t = level.timings;
-        std::cout << std::setprecision(10) << "uncoarsen_level=" << level.level << " vertices=" << level.vertices
+        gpart::detail::log() << std::setprecision(10) << "uncoarsen_level=" << level.level << " vertices=" << level.vertices
                   << " edge_entries=" << level.edge_entries << " projection_cut=" << level.projection_cut
                   << " projection_max_part_weight=" << level.projection_max_part_weight
                   << " projection_imbalance=" << level.projection_imbalance << " refine_rounds=" << s.rounds
@@ -52,7 +53,7 @@ DeviceUncoarsenResult<Types> uncoarsen_impl(H& hierarchy, P&& coarsest_partition
     }
     const auto total = sum_weights(result.final_part_weights);
     const auto maximum = *std::max_element(result.final_part_weights.begin(), result.final_part_weights.end());
-    std::cout << std::setprecision(10) << "final_edge_cut=" << result.final_cut << '\n'
+    gpart::detail::log() << std::setprecision(10) << "final_edge_cut=" << result.final_cut << '\n'
               << "final_cut_ratio="
               << (result.total_edge_weight == 0
                       ? 0.0

@@ -38,6 +38,9 @@ int gpart_partition_capacity_c(uint64_t total, int parts, double ratio, uint64_t
 gpart_ctx* gpart_thread_context(void);
 /* Synchronise and return cached device memory (ScratchPoolSession::finish). */
 int gpart_trim(void);
+/* Progress/report lines on stdout (the reference's drivers always print them):
+ * 1 on (default), 0 off. Process-wide. */
+void gpart_set_log(int enabled);
 
 /* Upload an int64 CSR. edge_weights / vertex_weights may be NULL (unit weights, as
  * the CLI loader sets them). validate: 0 shape checks, 1 strict (symmetry). */
