@@ -23,6 +23,7 @@ PROFILE_FAMILIES = {
     "refine_proposals": 3,
     "refine_filter": 4,
     "spectral": 5,
+    "rebalance": 6,
 }
 
 
@@ -114,6 +115,8 @@ def host_lib():
                                             C.c_void_p, C.POINTER(PartitionReport)]
         L.gpart_graph_coarsen.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_double, C.c_int, C.c_int, C.c_int,
                                           C.c_int, C.POINTER(C.c_void_p)]
+        L.gpart_graph_aggregate.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, _i32p, _u64p,
+                                            C.POINTER(C.c_int64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.gpart_hierarchy_read.argtypes = [C.c_int, C.c_char_p, C.POINTER(C.c_void_p)]
         L.gpart_hierarchy_levels.argtypes = [C.c_void_p]
         L.gpart_hierarchy_level_size.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -359,6 +362,16 @@ class Graph:
         _check(host_lib().gpart_graph_coarsen(self._h, parts, seed, stop_ratio, max_levels, cluster_size_factor,
                                               int(pack_isolated), int(verify), C.byref(h)))
         return Hierarchy(h.value)
+
+    def aggregate(self, parts: int, seed: int = 0, level: int = 0, cluster_size_factor: int = 256,
+                  pack_isolated: bool = False):
+        """One SCLP aggregation: (map, cluster weights, coarse count, capacity, max weight)."""
+        mp = np.empty(self.n, np.int32)
+        cw = np.empty(max(self.n, 1), np.uint64)
+        coarse, cap, mx = C.c_int64(), C.c_uint64(), C.c_uint64()
+        _check(host_lib().gpart_graph_aggregate(self._h, parts, seed, level, cluster_size_factor, int(pack_isolated),
+                                                mp, cw, C.byref(coarse), C.byref(cap), C.byref(mx)))
+        return mp, cw[: coarse.value].copy(), coarse.value, cap.value, mx.value
 
     def initial_partition(self, parts: int, ratio: float = 1.03, seed: int = 0,
                           spectral_mode: str = "batched_fp64", with_coords: bool = False):

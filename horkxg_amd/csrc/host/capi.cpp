@@ -275,6 +275,30 @@ int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ra
     });
 }
 
+int gpart_graph_aggregate(gpart_graph* g, int parts, uint32_t seed, int level, int cluster_size_factor,
+                          int pack_isolated, int32_t* map, uint64_t* cluster_weights, int64_t* coarse_vertices,
+                          uint64_t* capacity, uint64_t* maximum_weight) {
+    return guard([&] {
+        std::visit(
+            [&](auto& x) {
+                using T = std::conditional_t<std::is_same_v<std::decay_t<decltype(x)>, GraphOf<gpart::StandardTypes>>,
+                                             gpart::StandardTypes, gpart::BigTypes>;
+                gpart::SclpWorkspace<T> workspace;
+                gpart::SclpStats stats;
+                auto result = gpart::aggregate(x.device, workspace, parts, seed, level, stats, false,
+                                               cluster_size_factor, pack_isolated != 0);
+                const auto mapping = result.map.to_host();
+                const auto weights = result.vertex_weights.to_host();
+                std::copy(mapping.begin(), mapping.end(), map);
+                for (std::size_t i = 0; i < weights.size(); ++i) cluster_weights[i] = static_cast<uint64_t>(weights[i]);
+                *coarse_vertices = static_cast<int64_t>(result.coarse_vertices);
+                *capacity = result.capacity;
+                *maximum_weight = result.maximum_weight;
+            },
+            g->g);
+    });
+}
+
 int gpart_hierarchy_read(int width, const char* path, gpart_hierarchy** out) {
     return guard([&] {
         if (width == GPART_WIDTH_STANDARD)

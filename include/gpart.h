@@ -11,6 +11,7 @@
  *                                 (src/graph.cpp:47-223; app/partition_main.cu:42-66)
  *   gpart_graph_partition         app/partition_main.cu:67-119 (timed region, production options)
  *   gpart_graph_coarsen           coarsen() (include/coarsen.hpp:93-101; src/coarsen.cu:1906-2174)
+ *   gpart_graph_aggregate         aggregate() (include/coarsen.hpp:80-85; src/coarsen.cu:937-1659)
  *   gpart_graph_initial_partition initial_partition_gpu (include/initial_partition.hpp:9-12)
  *   gpart_graph_refine            refine_partition (include/refine.hpp:62-66)
  *   gpart_hierarchy_uncoarsen     uncoarsen (include/uncoarsen.hpp:10-13)
@@ -81,6 +82,12 @@ int gpart_graph_partition(gpart_graph* g, int parts, double ratio, uint32_t seed
 /* Coarsening hierarchy (host copy). */
 int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ratio, int max_levels,
                         int cluster_size_factor, int pack_isolated, int verify, gpart_hierarchy** out);
+/* One SCLP aggregation of a graph (aggregate, include/coarsen.hpp:80-85 of the
+ * reference; src/coarsen.cu:937-1659). map (n entries) and cluster_weights (at
+ * least coarse entries; n suffices) are host arrays. */
+int gpart_graph_aggregate(gpart_graph* g, int parts, uint32_t seed, int level, int cluster_size_factor,
+                          int pack_isolated, int32_t* map, uint64_t* cluster_weights, int64_t* coarse_vertices,
+                          uint64_t* capacity, uint64_t* maximum_weight);
 int gpart_hierarchy_read(int width, const char* path, gpart_hierarchy** out);
 int gpart_hierarchy_levels(const gpart_hierarchy* h);
 int gpart_hierarchy_level_size(const gpart_hierarchy* h, int level, int64_t* n, int64_t* m);

@@ -1,0 +1,115 @@
+"""CPU: the C-ABI libraries, the CLIs' argument handling and the input generators.
+
+Nothing here launches a kernel: library loading and symbol exports, CLI usage and
+load errors (which happen before any device call), and the host-side graph
+generators against their closed-form sizes and CSR invariants.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB = ROOT / "horkxg_amd" / "lib"
+BIN = ROOT / "horkxg_amd" / "bin"
+
+HEADERS = {
+    "gpart_hip.h": "libgpart_hip.so",
+    "gpart.h": "libgpart.so",
+}
+
+
+def declared_functions(header: Path):
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set()
+    for match in re.finditer(r"\b(gpart_\w+)\s*\(", text):
+        start = text.rfind("\n", 0, match.start()) + 1
+        line = text[start:match.start()]
+        if "typedef" in line or "#define" in line:
+            continue
+        names.add(match.group(1))
+    return sorted(names)
+
+
+@pytest.mark.parametrize("header", sorted(HEADERS))
+def test_library_exports_every_declared_symbol(header):
+    """Every function include/<header> declares is exported by its library."""
+    lib = ctypes.CDLL(str(LIB / HEADERS[header]))
+    names = declared_functions(ROOT / "include" / header)
+    assert len(names) > 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("cli", ["gpart_partition_standard", "gpart_partition_big",
+                                 "gpart_coarsen_standard", "gpart_coarsen_big"])
+def test_cli_usage_exit_code(cli):
+    """Wrong argument counts print the usage line and exit 2 (app/partition_main.cu:27-33,
+    app/coarsen_main.cu:18-23)."""
+    run = subprocess.run([str(BIN / cli)], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 2
+    assert "Usage:" in run.stderr
+
+
+def test_cli_missing_input_exit_code(tmp_path):
+    """A load failure is reported on stderr with exit 1, before any device work."""
+    run = subprocess.run([str(BIN / "gpart_partition_standard"), str(tmp_path / "none.bin"),
+                          str(tmp_path / "none2.bin"), "4", str(tmp_path / "out.bin")],
+                         capture_output=True, text=True, timeout=60)
+    assert run.returncode == 1
+    assert "gpart_partition:" in run.stderr
+
+
+def check_csr(indptr, indices):
+    n = len(indptr) - 1
+    assert indptr[0] == 0 and indptr[-1] == len(indices)
+    assert np.all(np.diff(indptr) >= 0)
+    assert indices.min() >= 0 and indices.max() < n
+    src = np.repeat(np.arange(n), np.diff(indptr))
+    assert not np.any(src == indices), "self loop"
+    key = src.astype(np.int64) * n + indices
+    assert np.all(np.diff(key) > 0), "rows not strictly sorted (duplicate or unsorted entry)"
+    rev = np.sort(indices.astype(np.int64) * n + src)
+    assert np.array_equal(rev, key), "not symmetric"
+
+
+def test_grid_generator(hx):
+    """256 x 256 4-neighbour grid: 2 * 2 * 256 * 255 = 261,120 directed entries."""
+    ip, ix = hx.generate_grid2d(256, 256)
+    assert len(ip) - 1 == 65536 and len(ix) == 261120
+    check_csr(ip, ix)
+    v = 300  # r = 1, c = 44
+    assert list(ix[ip[v]:ip[v + 1]]) == [44, 299, 301, 556]
+
+
+def test_mesh_generator(hx):
+    """s^3 mesh with 6 face neighbours: 2 * 3 * s^2 * (s - 1) directed entries
+    (12,484,608 for s = 128)."""
+    for s in (5, 16):
+        ip, ix = hx.generate_mesh3d(s, s, s)
+        assert len(ip) - 1 == s ** 3 and len(ix) == 6 * s * s * (s - 1)
+        check_csr(ip, ix)
+    ip, ix = hx.generate_mesh3d(3, 4, 5)
+    assert len(ip) - 1 == 60 and len(ix) == 2 * (2 * 4 * 5 + 3 * 3 * 5 + 3 * 4 * 4)
+
+
+def test_rmat_generator(hx):
+    """R-MAT: symmetric, deduplicated, no self loops, deterministic in the seed, and
+    skewed (the quadrant probabilities favour low ids)."""
+    ip, ix = hx.generate_rmat(12, 16)
+    check_csr(ip, ix)
+    assert len(ip) - 1 == 4096
+    ip2, ix2 = hx.generate_rmat(12, 16)
+    assert np.array_equal(ip, ip2) and np.array_equal(ix, ix2)
+    ip3, ix3 = hx.generate_rmat(12, 16, seed=1)
+    assert not (len(ix3) == len(ix) and np.array_equal(ix3, ix))
+    deg = np.diff(ip)
+    assert deg[:64].mean() > 4 * deg[-2048:].mean()
+    assert 0.5 * 2 * 16 * 4096 < len(ix) <= 2 * 16 * 4096
