@@ -27,6 +27,16 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = ROOT / "profiles"
+
+# The kernel each live-timed family brackets (what a rocprofv3 --pmc run selects).
+FAMILY_KERNELS = {
+    "rebalance": "balance_proposals_kernel",
+    "refine_proposals": "lp_proposals_kernel",
+    "refine_filter": "priority_filter_kernel",
+    "refine_cache": "build_cache_kernel",
+    "spectral": "spectral_depth_kernel",
+}
 
 WORKLOADS = {
     "mesh128_k32": dict(kind="mesh3d", size=(128, 128, 128), k=32),
@@ -57,6 +67,85 @@ def cpu_baseline(indptr, indices, k, ratio, seed, threads):
     return res, wall, labels
 
 
+def copy_calibration(mib: int) -> int:
+    """Five device-to-device copies of `mib` MiB on the library stream: a plain copy
+    kernel of known size, whose FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
+    counters (run under rocprofv3 --pmc)."""
+    import ctypes as C
+
+    import horkxg_amd as hx
+    from horkxg_amd import _native as N
+
+    nbytes = mib << 20
+    src = hx.device.DeviceArray(dtype=np.uint8, count=nbytes)
+    dst = hx.device.DeviceArray(dtype=np.uint8, count=nbytes)
+    H = N.hip_lib()
+    H.gpart_memcpy_dtod.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    for _ in range(5):
+        N._check(H.gpart_memcpy_dtod(N._ctx(), C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes))
+    N._check(H.gpart_ctx_synchronize(N._ctx()))
+    print(f"[bench] copied {mib} MiB five times", file=sys.stderr, flush=True)
+    return 0
+
+
+def pmc_per_dispatch(csv_path: Path, kernel_regex: str, counter: str):
+    """Mean of `counter` per dispatch of kernels matching kernel_regex in a
+    rocprofv3 counter_collection.csv (summed over the counter's instances)."""
+    import csv
+    import re
+
+    per_dispatch = {}
+    pattern = re.compile(kernel_regex)
+    with open(csv_path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter or not pattern.search(row.get("Kernel_Name", "")):
+                continue
+            key = row.get("Dispatch_Id") or row.get("Correlation_Id")
+            per_dispatch[key] = per_dispatch.get(key, 0.0) + float(row["Counter_Value"])
+    if not per_dispatch:
+        raise SystemExit(f"no {counter} rows for /{kernel_regex}/ in {csv_path}")
+    values = list(per_dispatch.values())
+    return sum(values) / len(values), len(values)
+
+
+def traffic_summary(args) -> int:
+    """Writes profiles/traffic_<workload>.json from two PMC runs of the bench
+    (FETCH_SIZE, WRITE_SIZE) and two of the copy calibration."""
+    copy_bytes = args.copy_mib << 20
+    cf, _ = pmc_per_dispatch(Path(args.copy_fetch), "copyBuffer", "FETCH_SIZE")
+    cw, _ = pmc_per_dispatch(Path(args.copy_write), "copyBuffer", "WRITE_SIZE")
+    f_scale = copy_bytes / (cf * 1024.0)
+    w_scale = copy_bytes / (cw * 1024.0)
+    kf, nf = pmc_per_dispatch(Path(args.fetch), args.kernel, "FETCH_SIZE")
+    kw, nw = pmc_per_dispatch(Path(args.write), args.kernel, "WRITE_SIZE")
+    out = {
+        "workload": args.workload,
+        "kernel": args.kernel,
+        "dispatches": {"fetch": nf, "write": nw},
+        "fetch_kib_per_dispatch": round(kf, 1),
+        "write_kib_per_dispatch": round(kw, 1),
+        "copy_calibration": {"bytes": copy_bytes, "fetch_kib": round(cf, 1), "write_kib": round(cw, 1),
+                             "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4)},
+        "traffic_bytes_per_dispatch": round(kf * 1024.0 * f_scale + kw * 1024.0 * w_scale),
+    }
+    PROFILES.mkdir(exist_ok=True)
+    path = PROFILES / f"traffic_{args.workload}.json"
+    path.write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out))
+    return 0
+
+
+def recorded_traffic(workload: str, family: str):
+    """HBM bytes per launch of the family's kernel from the committed PMC summary."""
+    path = PROFILES / f"traffic_{workload}.json"
+    if not path.exists() or family not in FAMILY_KERNELS:
+        return None
+    rec = json.loads(path.read_text())
+    if FAMILY_KERNELS[family] not in rec.get("kernel", ""):
+        return None
+    return rec["traffic_bytes_per_dispatch"]
+
+
 def main() -> int:
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
@@ -70,7 +159,18 @@ def main() -> int:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
     p.add_argument("--no-profile", action="store_true", help="skip the live per-family event timing")
+    p.add_argument("--copy-calibration", type=int, metavar="MIB", help="only run the copy calibration")
+    p.add_argument("--traffic-summary", action="store_true",
+                   help="only summarise rocprofv3 --pmc CSVs into profiles/traffic_<workload>.json")
+    p.add_argument("--kernel", default="lp_proposals_kernel", help="kernel regex for --traffic-summary")
+    p.add_argument("--fetch", help="counter_collection.csv of the FETCH_SIZE run")
+    p.add_argument("--write", help="counter_collection.csv of the WRITE_SIZE run")
+    p.add_argument("--copy-fetch", help="counter_collection.csv of the calibration FETCH_SIZE run")
+    p.add_argument("--copy-write", help="counter_collection.csv of the calibration WRITE_SIZE run")
+    p.add_argument("--copy-mib", type=int, default=1024)
     args = p.parse_args()
+    if args.traffic_summary:
+        return traffic_summary(args)
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(the partitioner is a single-GPU path; see DESIGN.md for how it would shard)",
@@ -82,6 +182,8 @@ def main() -> int:
     if hx.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     spec = WORKLOADS[args.workload]
     k = spec["k"]
     indptr, indices = make_graph(hx, spec)
@@ -129,7 +231,9 @@ def main() -> int:
         bytes_per_launch = d["bytes"] / max(d["launches"], 1)
         achieved = bytes_per_launch / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": recorded_traffic(args.workload, dominant), "family": dominant,
+                    "kernel": FAMILY_KERNELS.get(dominant, dominant),
                     "launches": d["launches"], "avg_launch_us": round(per_launch_s * 1e6, 2),
                     "algorithmic_bytes_per_launch": round(bytes_per_launch)}
         roofline["families"] = {

@@ -30,6 +30,7 @@ from ._native import (  # noqa: F401
     profile_read,
     profile_reset,
     record_layout,
+    cache_stats,
     set_log,
     trim,
 )

@@ -87,6 +87,8 @@ def hip_lib():
         H.gpart_memcpy_htod.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         H.gpart_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         H.gpart_ctx_synchronize.argtypes = [C.c_void_p]
+        H.gpart_cache_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         _hip = H
     return _hip
 
@@ -177,6 +179,17 @@ def record_layout(vertices: int, bound: int):
     vb, wb, mask = C.c_int(), C.c_int(), C.c_uint64()
     hip_lib().gpart_record_layout(vertices, bound, C.byref(vb), C.byref(wb), C.byref(mask))
     return vb.value, wb.value, mask.value
+
+
+def cache_stats() -> dict:
+    """The calling thread's allocator: cached blocks/bytes, live bytes and the last
+    release's block count and seconds (returning blocks, stream wait, pool trim)."""
+    blocks, cached, live, released = C.c_int64(), C.c_uint64(), C.c_uint64(), C.c_int64()
+    secs = (C.c_double * 3)()
+    _check(hip_lib().gpart_cache_stats(_ctx(), C.byref(blocks), C.byref(cached), C.byref(live),
+                                       C.byref(released), secs))
+    return {"cached_blocks": blocks.value, "cached_bytes": cached.value, "live_bytes": live.value,
+            "released_blocks": released.value, "release_seconds": list(secs)}
 
 
 def set_log(enabled: bool) -> None:

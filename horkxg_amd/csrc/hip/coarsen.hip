@@ -454,23 +454,42 @@ __global__ void admit_decide_kernel(std::int64_t p, const std::int32_t* ordered_
     accept[i] = base <= capacity && prefix[i] <= capacity - base;
 }
 
-__global__ void admit_apply_kernel(std::int64_t p, const std::int32_t* order, const std::int32_t* ordered_targets,
-                                   const std::uint64_t* ordered_weights, const std::int32_t* accept,
-                                   std::int32_t* clusters, unsigned long long* cluster_weights,
-                                   unsigned long long* accepted) {
+// sclp_commit_kernel (:652-680): totals[0] counts accepted moves, totals[1] sums
+// their gains (the diagnostics' predicted gain); one atomic per workgroup each.
+__global__ void __launch_bounds__(kBlock) admit_apply_kernel(std::int64_t p, const std::int32_t* order,
+                                                             const std::int32_t* ordered_targets,
+                                                             const std::uint64_t* ordered_weights,
+                                                             const std::int32_t* accept,
+                                                             const unsigned long long* gains,
+                                                             std::int32_t* clusters,
+                                                             unsigned long long* cluster_weights,
+                                                             unsigned long long* totals) {
+    __shared__ unsigned long long partial[2][kBlock / kWave];
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     bool ok = false;
+    unsigned long long gain = 0;
     if (i < p && accept[i]) {
         ok = true;
         const std::int32_t v = order[i];
         const std::int32_t t = ordered_targets[i];
         const unsigned long long w = ordered_weights[i];
+        gain = gains[v];
         atomicAdd(cluster_weights + clusters[v], 0ULL - w);
         atomicAdd(cluster_weights + t, w);
         clusters[v] = t;
     }
     const unsigned long long c = static_cast<unsigned long long>(__popcll(__ballot(ok)));
-    if (lane_id() == 0 && c) atomicAdd(accepted, c);
+    gain = group_sum_u64<kWave>(gain);
+    if (lane_id() == 0) {
+        partial[0][threadIdx.x / kWave] = c;
+        partial[1][threadIdx.x / kWave] = gain;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        unsigned long long total = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) total += partial[threadIdx.x][w];
+        if (total) atomicAdd(totals + threadIdx.x, total);
+    }
 }
 
 // ---------------------------------------------------------------- isolated / two-hop
@@ -856,7 +875,7 @@ int gpart_sclp_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t*
 int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const uint64_t* gains,
                      const uint32_t* ties, const uint64_t* vertex_weights, uint64_t capacity, uint64_t gain_bound,
                      int32_t* clusters, uint64_t* cluster_weights, int64_t* valid, int64_t* accepted,
-                     int64_t* nonempty) {
+                     int64_t* nonempty, uint64_t* predicted_gain) {
     return guarded([&] {
         auto* cw = reinterpret_cast<unsigned long long*>(cluster_weights);
         const auto* gn = reinterpret_cast<const unsigned long long*>(gains);
@@ -882,12 +901,16 @@ int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const 
             hipLaunchKernelGGL(admit_decide_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
                                ordered_targets.get(), prefix.get(), cw, capacity, accept.get());
             GPART_LAUNCH_CHECK();
-            Scratch<unsigned long long> acc(ctx, 1);
-            fill_bytes(ctx, acc.get(), 0, 8);
+            Scratch<unsigned long long> acc(ctx, 2);
+            fill_bytes(ctx, acc.get(), 0, 16);
             hipLaunchKernelGGL(admit_apply_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
-                               ordered_targets.get(), ordered_weights.get(), accept.get(), clusters, cw, acc.get());
+                               ordered_targets.get(), ordered_weights.get(), accept.get(),
+                               reinterpret_cast<const unsigned long long*>(gains), clusters, cw, acc.get());
             GPART_LAUNCH_CHECK();
-            *accepted = static_cast<std::int64_t>(read_scalar(ctx, acc.get()));
+            unsigned long long h[2];
+            read_back(ctx, h, acc.get(), sizeof(h));
+            *accepted = static_cast<std::int64_t>(h[0]);
+            if (predicted_gain) *predicted_gain = h[1];
         }
         *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));
     });

@@ -54,6 +54,12 @@ __device__ __forceinline__ unsigned long long shfl_down_u64(unsigned long long v
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
+__device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long v, int delta, int width) {
+    const unsigned lo = __shfl_up(static_cast<unsigned>(v), delta, width);
+    const unsigned hi = __shfl_up(static_cast<unsigned>(v >> 32), delta, width);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ long long shfl_xor_i64(long long v, int mask, int width) {
     return static_cast<long long>(shfl_xor_u64(static_cast<unsigned long long>(v), mask, width));
 }

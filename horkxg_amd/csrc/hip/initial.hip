@@ -30,6 +30,7 @@ namespace {
 constexpr int kHalf = 32;
 constexpr int kMaxBatchedTask = 4096;
 constexpr int kBatchThreads = 1024;
+constexpr int kRowsPerPass = 4;
 
 struct Moments {
     double mass, sum, squares;
@@ -317,39 +318,69 @@ __global__ void __launch_bounds__(kBatchThreads) spectral_depth_kernel(
         __threadfence_block();
         __syncthreads();
         if (iteration == 64) break;
-        for (int base = 0; base < count; base += kHalves) {
-            const int i = base + hw;
-            T sum = 0;
-            if (i < count) {
-                const int v = ids[i];
-                for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32) {
-                    const int u = neighbors[e];
-                    if (labels[u] == group) sum = VecTraits<T>::fmadd(static_cast<T>(weights[e]), x[u], sum);
-                }
-            }
-            double sd;
-            if constexpr (sizeof(T) == 8) {
-                sd = tree32(static_cast<double>(sum));
-            } else {
-                float s = sum;
+        // Each half-wave carries kRowsPerPass vertices per pass so their gather
+        // chains (offsets -> neighbours/weights -> labels/x) are in flight together;
+        // every lane still accumulates its entries e = begin + lane + 32j in
+        // increasing e, so the sums are unchanged.
+        for (int base = 0; base < count; base += kHalves * kRowsPerPass) {
+            int rows[kRowsPerPass];
+            std::int64_t first[kRowsPerPass], last[kRowsPerPass];
+            T sum[kRowsPerPass];
 #pragma unroll
-                for (int s2 = 16; s2; s2 >>= 1) s += __shfl_down(s, s2, kHalf);
-                sd = static_cast<double>(s);
+            for (int r = 0; r < kRowsPerPass; ++r) {
+                const int i = base + r * kHalves + hw;
+                rows[r] = i < count ? ids[i] : -1;
+                first[r] = rows[r] >= 0 ? static_cast<std::int64_t>(offsets[rows[r]]) : 0;
+                last[r] = rows[r] >= 0 ? static_cast<std::int64_t>(offsets[rows[r] + 1]) : 0;
+                sum[r] = 0;
             }
-            if (lane == 0 && i < count) {
-                const int v = ids[i];
-                const double d = degree[v];
-                double a;
-                if constexpr (sizeof(T) == 8) {
-                    a = d > 0 ? 0.5 * (static_cast<double>(x[v]) + sd / d) : static_cast<double>(x[v]);
-                } else {
-                    const float df = static_cast<float>(d);
-                    const float af = d > 0 ? 0.5f * (x[v] + static_cast<float>(sd) / df) : x[v];
-                    a = static_cast<double>(af);
+            int nbr[kRowsPerPass];
+            T w[kRowsPerPass];
+#pragma unroll
+            for (int r = 0; r < kRowsPerPass; ++r) {
+                const std::int64_t e = first[r] + lane;
+                nbr[r] = e < last[r] ? neighbors[e] : -1;
+                w[r] = e < last[r] ? static_cast<T>(weights[e]) : T(0);
+            }
+#pragma unroll
+            for (int r = 0; r < kRowsPerPass; ++r) {
+                const int u = nbr[r];
+                const int lab = u >= 0 ? labels[u] : -1;
+                const T xu = u >= 0 ? x[u] : T(0);
+                if (lab == group) sum[r] = VecTraits<T>::fmadd(w[r], xu, sum[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < kRowsPerPass; ++r)
+                for (std::int64_t e = first[r] + lane + 32; e < last[r]; e += 32) {
+                    const int u = neighbors[e];
+                    if (labels[u] == group) sum[r] = VecTraits<T>::fmadd(static_cast<T>(weights[e]), x[u], sum[r]);
                 }
-                y[v] = static_cast<T>(a);
-                const double mass = d > 0 ? d : 1.0;
-                tvm[i] = Moments{mass, mass * a, mass * a * a};
+#pragma unroll
+            for (int r = 0; r < kRowsPerPass; ++r) {
+                double sd;
+                if constexpr (sizeof(T) == 8) {
+                    sd = tree32(static_cast<double>(sum[r]));
+                } else {
+                    float s = sum[r];
+#pragma unroll
+                    for (int s2 = 16; s2; s2 >>= 1) s += __shfl_down(s, s2, kHalf);
+                    sd = static_cast<double>(s);
+                }
+                if (lane == 0 && rows[r] >= 0) {
+                    const int v = rows[r];
+                    const double d = degree[v];
+                    double a;
+                    if constexpr (sizeof(T) == 8) {
+                        a = d > 0 ? 0.5 * (static_cast<double>(x[v]) + sd / d) : static_cast<double>(x[v]);
+                    } else {
+                        const float df = static_cast<float>(d);
+                        const float af = d > 0 ? 0.5f * (x[v] + static_cast<float>(sd) / df) : x[v];
+                        a = static_cast<double>(af);
+                    }
+                    y[v] = static_cast<T>(a);
+                    const double mass = d > 0 ? d : 1.0;
+                    tvm[base + r * kHalves + hw] = Moments{mass, mass * a, mass * a * a};
+                }
             }
         }
         __threadfence_block();

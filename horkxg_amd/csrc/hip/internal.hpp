@@ -16,12 +16,27 @@
 struct gpart_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // Caching allocator: blocks keyed by rounded size; never returned to HIP until
-    // gpart_release_cache or destruction.
-    std::multimap<std::size_t, void*> free_blocks;
-    std::map<void*, std::size_t> live_blocks;
+    // Caching sub-allocator. Blocks (rounded sizes) are carved from large slabs
+    // taken stream-ordered from the context's own HIP memory pool (release
+    // threshold UINT64_MAX); a freed block is cached by size. A release frees the
+    // slabs that hold no live block and trims the pool: a few HIP calls instead
+    // of one per block (each costs tens of microseconds of host time).
+    struct Slab {
+        char* base = nullptr;
+        std::size_t size = 0;
+        std::size_t used = 0;
+        std::int64_t live = 0;
+    };
+    hipMemPool_t pool = nullptr;
+    std::map<int, Slab> slabs;
+    int next_slab = 0;
+    std::multimap<std::size_t, std::pair<void*, int>> free_blocks;    // size -> (block, slab)
+    std::map<void*, std::pair<std::size_t, int>> live_blocks;         // block -> (size, slab)
     std::size_t cached_bytes = 0;
     std::size_t live_bytes = 0;
+    // Host seconds of the last release: returning blocks, synchronising, trimming.
+    double release_seconds[3] = {0, 0, 0};
+    std::int64_t released_blocks = 0;
     // Small pinned staging buffer for host-out scalars.
     void* pinned = nullptr;
     // Profiling per family.
@@ -125,6 +140,8 @@ void fill_bytes(gpart_ctx* ctx, void* dst, int value, std::size_t bytes);
 // Profiling hooks around a family's main kernel launches.
 void profile_begin(gpart_ctx* ctx, int family);
 void profile_end(gpart_ctx* ctx, int family, double algorithmic_bytes);
+// Adds bytes to the family's last launch once its size is known on the host.
+void profile_add_bytes(gpart_ctx* ctx, int family, double algorithmic_bytes);
 
 // ---- primitives (primitives.hip) ----
 // Stable LSD radix sort over key bits [begin_bit, end_bit); alternate buffers

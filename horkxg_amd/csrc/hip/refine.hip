@@ -3,13 +3,17 @@
 //
 // Per-vertex edge loops use the level's degree classes: 8 lanes for degree <= 8,
 // a wave for <= 256 and a workgroup above. Row scans of the n x k connectivity
-// cache use a group of next_pow2(k) lanes per vertex so every row is one
-// contiguous, coalesced read.
+// cache use 1, 2 or 4 lanes per row, each owning 8 consecutive parts, so a wave
+// has 16-64 rows in flight per pass.
 //
 // The reference ranks vertices inside a rebalance bucket by the return value of
 // atomicAdd (src/refine_incremental.cuh:185,216): any order is legal. Here the
-// rank is the weight of the bucket's lower-id members (a stable sort by bucket
-// and a segmented scan), so refinement is reproducible run to run.
+// rank is the weight of the bucket's lower-id members, so refinement is
+// reproducible run to run.
+//
+// A round (LP or rebalance) keeps its counts on the device: kernels after a
+// compaction read the length from a RoundState and loop grid-stride, and the
+// host reads one RoundState back at the end of the round.
 #include "internal.hpp"
 #include "reduce.hpp"
 #include "scan.hpp"
@@ -101,33 +105,48 @@ __global__ void __launch_bounds__(kBlock) build_cache_kernel(std::int64_t count,
         for (int p = sub; p < k; p += G) cache[static_cast<std::int64_t>(v) * k + p] = rows[local_group][p];
 }
 
-// incremental_lp_proposals (:27-46), one group of G >= k lanes per vertex.
+// Rows of the n x k cache are read by groups of G lanes, lane j owning the
+// kRowSpan consecutive parts [kRowSpan*j, kRowSpan*j + kRowSpan): G = 1, 2 or 4
+// for k <= 8, 16, 32. A wave covers 64/G rows per pass.
+constexpr int kRowSpan = 8;
+
+// incremental_lp_proposals (:27-46): argmax over p != source of cache > 0,
+// lowest index on ties; accepted if best >= current or within the filter.
 template <int G>
 __global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, const unsigned long long* cache,
                                                               const std::int32_t* labels, int k, double filter,
                                                               std::int32_t* locks, std::int32_t* targets,
                                                               long long* gains) {
     const std::int64_t v = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
-    const int p = threadIdx.x & (G - 1);
+    const int j = threadIdx.x & (G - 1);
     if (v >= n) return;  // uniform per group
-    const bool locked = locks[v] != 0;
     const std::int32_t source = labels[v];
-    const unsigned long long value = p < k ? cache[v * k + p] : 0;
-    // argmax over p != source of value > 0, lowest index on ties
-    unsigned long long best = (p < k && p != source) ? value : 0;
-    int best_p = best > 0 ? p : 0x7fffffff;
+    const unsigned long long* row = cache + v * k;
+    unsigned long long best = 0, current = 0;
+    int best_p = 0x7fffffff;
+#pragma unroll
+    for (int e = 0; e < kRowSpan; ++e) {
+        const int p = j * kRowSpan + e;
+        const unsigned long long value = p < k ? row[p] : 0;
+        if (p == source) current = value;
+        else if (value > best) {  // strict: the lowest index keeps a tie
+            best = value;
+            best_p = p;
+        }
+    }
 #pragma unroll
     for (int m = G / 2; m > 0; m /= 2) {
         const auto ob = shfl_xor_u64(best, m, G);
         const int op = __shfl_xor(best_p, m, G);
+        const auto oc = shfl_xor_u64(current, m, G);
         if (ob > best || (ob == best && op < best_p)) {
             best = ob;
             best_p = op;
         }
+        current = oc > current ? oc : current;  // only the source's lane holds it
     }
-    const unsigned long long current = shfl_u64(value, source, G);
-    if (p != 0) return;
-    if (locked) {
+    if (j != 0) return;
+    if (locks[v] != 0) {
         locks[v] = 0;
         targets[v] = -1;
         gains[v] = static_cast<long long>(0x8000000000000000ULL);
@@ -189,19 +208,37 @@ __global__ void __launch_bounds__(kBlock) priority_filter_kernel(std::int64_t co
     }
 }
 
-// incremental_update_cache_and_cut (:69-95)
+// ---------------------------------------------------------------- round state
+
+// Device record of one round, read back once at its end.
+struct RoundState {
+    long long light;       // movers with degree <= kHeavyDegree
+    long long heavy;       // movers above
+    long long delta;       // exact cut decrease
+    long long candidates;  // rebalance: vertices of overloaded parts
+    unsigned long long max_weight;
+    unsigned long long pad[3];
+    unsigned long long pw[32];
+};
+
+constexpr int kHeavyDegree = 64;
+constexpr int kRoundBlocks = 1024;  // cap of grid-stride launches whose count lives on the device
+
+// incremental_update_cache_and_cut (:69-95) over a mover list whose length is on the device.
 template <typename O, typename W, int G>
-__global__ void __launch_bounds__(kBlock) update_cache_kernel(std::int64_t count, const std::int32_t* movers,
+__global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* count_ptr, const std::int32_t* movers,
                                                               const O* offsets, const std::int32_t* neighbors,
                                                               const W* weights, const std::int32_t* labels,
                                                               const std::int32_t* targets,
                                                               const std::int32_t* flags, int k,
                                                               unsigned long long* cache, long long* delta) {
     __shared__ long long block_sum[kBlock / kWave];
-    const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+    const long long count = *count_ptr;
     const int sub = threadIdx.x & (G - 1);
+    const long long groups = static_cast<long long>(gridDim.x) * (kBlock / G);
     long long gain = 0;
-    if (group < count) {
+    for (long long group = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) / G; group < count;
+         group += groups) {
         const std::int32_t v = movers[group];
         const std::int32_t source = labels[v];
         const std::int32_t target = targets[v];
@@ -229,45 +266,95 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(std::int64_t count
     }
 }
 
-// incremental_commit (:97-109)
+// incremental_commit (:97-109) over light then heavy movers; part weight changes
+// are summed per workgroup before they reach the k global counters.
 template <typename W>
-__global__ void commit_kernel(std::int64_t count, const std::int32_t* movers, std::int32_t* labels,
-                              const std::int32_t* targets, const W* vertex_weights, unsigned long long* part_weights,
-                              std::int32_t* locks, bool lp) {
-    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const std::int32_t v = movers[i];
-    const std::int32_t source = labels[v];
-    const std::int32_t target = targets[v];
-    const auto w = static_cast<unsigned long long>(vertex_weights[v]);
-    atomicAdd(part_weights + source, 0ULL - w);
-    atomicAdd(part_weights + target, w);
-    labels[v] = target;
-    if (lp) locks[v] = 1;
-}
-
-template <typename O>
-__global__ void split_movers_kernel(std::int64_t count, const std::int32_t* movers, const O* offsets,
-                                    std::int32_t* light, std::int32_t* heavy, unsigned long long* counters) {
-    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < count;
-    const std::int32_t v = valid ? movers[i] : 0;
-    const bool heavy_row = valid && (offsets[v + 1] - offsets[v]) > 64;
-    const std::uint64_t below = lanemask_lt();
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const bool mine = valid && (c == 1) == heavy_row;
-        const std::uint64_t mask = __ballot(mine);
-        if (!mask) continue;
-        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-        unsigned long long base = 0;
-        if (lane_id() == leader) base = atomicAdd(counters + c, static_cast<unsigned long long>(__popcll(mask)));
-        base = shfl_u64(base, leader, kWave);
-        if (mine) (c ? heavy : light)[base + __popcll(mask & below)] = v;
+__global__ void __launch_bounds__(kBlock) commit_kernel(const RoundState* st, const std::int32_t* light,
+                                                        const std::int32_t* heavy, std::int32_t* labels,
+                                                        const std::int32_t* targets, const W* vertex_weights,
+                                                        unsigned long long* part_weights, std::int32_t* locks,
+                                                        bool lp, int k) {
+    __shared__ unsigned long long gained[32], lost[32];
+    if (threadIdx.x < 32) gained[threadIdx.x] = lost[threadIdx.x] = 0;
+    __syncthreads();
+    const long long nl = st->light, total = nl + st->heavy;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const std::int32_t v = i < nl ? light[i] : heavy[i - nl];
+        const std::int32_t source = labels[v];
+        const std::int32_t target = targets[v];
+        const auto w = static_cast<unsigned long long>(vertex_weights[v]);
+        atomicAdd(&lost[source], w);
+        atomicAdd(&gained[target], w);
+        labels[v] = target;
+        if (lp) locks[v] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x < k) {
+        const unsigned long long change = gained[threadIdx.x] - lost[threadIdx.x];
+        if (change) atomicAdd(part_weights + threadIdx.x, change);
     }
 }
 
+__global__ void round_status_kernel(RoundState* st, const unsigned long long* part_weights, int k) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long maximum = 0;
+    for (int p = 0; p < k; ++p) {
+        st->pw[p] = part_weights[p];
+        maximum = part_weights[p] > maximum ? part_weights[p] : maximum;
+    }
+    st->max_weight = maximum;
+}
+
+// Movers are split by degree so the cache update can give heavy rows a workgroup.
+struct MoverPair {
+    long long light, heavy;
+};
+struct MoverPairOp {
+    __device__ __forceinline__ MoverPair identity() const { return MoverPair{0, 0}; }
+    __device__ __forceinline__ MoverPair operator()(const MoverPair& a, const MoverPair& b) const {
+        return MoverPair{a.light + b.light, a.heavy + b.heavy};
+    }
+};
+template <typename O>
+struct MoverLoad {
+    const std::int32_t* flags;
+    const O* offsets;
+    __device__ __forceinline__ MoverPair operator()(std::int64_t v) const {
+        if (!flags[v]) return MoverPair{0, 0};
+        const bool heavy = offsets[v + 1] - offsets[v] > kHeavyDegree;
+        return MoverPair{heavy ? 0 : 1, heavy ? 1 : 0};
+    }
+};
+struct MoverStore {
+    std::int32_t* light;
+    std::int32_t* heavy;
+    std::int64_t n;
+    RoundState* st;
+    __device__ __forceinline__ void operator()(std::int64_t v, const MoverPair& prefix, const MoverPair& item) const {
+        if (item.light) light[prefix.light] = static_cast<std::int32_t>(v);
+        if (item.heavy) heavy[prefix.heavy] = static_cast<std::int32_t>(v);
+        if (v == n - 1) {
+            st->light = prefix.light + item.light;
+            st->heavy = prefix.heavy + item.heavy;
+        }
+    }
+};
+
 // ---------------------------------------------------------------- rebalance
+//
+// One iteration of the reference's two-sided bucketed rebalance:
+//   proposals   every vertex of an overloaded part picks a destination and lands
+//               in a source bucket (part, gain class, section);
+//   sources     a vertex leaves when the weight ranked before it in its part's
+//               bucket order is below the part's overload;
+//   destinations it is admitted when the weight ranked before it plus its own
+//               fits the destination's slack.
+// Buckets are visited in a fixed order, so within one part every bucket is
+// either wholly taken, wholly refused, or the single boundary bucket that
+// straddles the part's threshold. Only boundary members need a rank; they get
+// the weight of the lower-id members of their bucket (a blocked multisplit),
+// one legal outcome of the reference's atomicAdd ranking.
 
 struct PowerTable {
     double value[48];  // correctly rounded 1.5^j, j = -23..24
@@ -289,139 +376,292 @@ __device__ __forceinline__ int gain_bucket(double gain, const PowerTable& t) {
     return max(2, min(49, 25 + j));
 }
 
-// incremental_rebalance_proposals (:111-157) and the source bucket of each proposal.
-// Vertices of overloaded parts are the only possible proposers; everyone else is
-// reset here and skipped by the row scan below.
-__global__ void overloaded_flags_kernel(std::int64_t n, const std::int32_t* labels,
-                                        const unsigned long long* part_weights, int k, std::uint64_t capacity,
-                                        std::int32_t* flags, std::int32_t* targets, std::int32_t* buckets) {
-    __shared__ unsigned long long pw[32];
-    if (threadIdx.x < 32) pw[threadIdx.x] = threadIdx.x < k ? part_weights[threadIdx.x] : 0;
-    __syncthreads();
-    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    flags[v] = pw[labels[v]] > capacity;
-    targets[v] = -1;
-    buckets[v] = -1;
+// Per-workgroup histogram of bucket weights in LDS (open addressing); a full
+// table spills to the global counters. Workgroups flush into one of kReplicas
+// copies of the counter array (blockIdx % kReplicas): a few hundred hot buckets
+// would otherwise take every workgroup's atomics on the same addresses. The
+// bucket scan sums the copies.
+constexpr int kReplicas = 16;
+constexpr int kTableSlots = 2048;
+constexpr unsigned kTableEmpty = 0xffffffffU;
+struct BucketTable {
+    unsigned key[kTableSlots];
+    unsigned long long value[kTableSlots];
+};
+
+__device__ __forceinline__ void table_clear(BucketTable& t) {
+    for (int i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
+        t.key[i] = kTableEmpty;
+        t.value[i] = 0;
+    }
 }
 
-template <typename W, int G>
-__global__ void __launch_bounds__(kBlock) rebalance_proposals_kernel(std::int64_t count, const std::int32_t* list,
-                                                                     const unsigned long long* cache,
-                                                                     const std::int32_t* labels, const W* vertex_weights,
-                                                                     int k, const unsigned long long* part_weights,
-                                                                     std::uint64_t capacity, PowerTable powers,
-                                                                     std::int32_t* targets, std::int32_t* buckets) {
-    __shared__ unsigned long long pw[32];
-    if (threadIdx.x < 32) pw[threadIdx.x] = threadIdx.x < k ? part_weights[threadIdx.x] : 0;
-    __syncthreads();
-    const std::int64_t index = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
-    const int p = threadIdx.x & (G - 1);
-    if (index >= count) return;  // uniform per group
-    const std::int64_t v = list[index];
-    const std::int32_t source = labels[v];
-    const std::uint64_t weight = static_cast<std::uint64_t>(vertex_weights[v]);
-    const bool ok = p < k && p != source && pw[p] < capacity && weight <= capacity - pw[p];
-    unsigned long long conn = ok ? cache[v * k + p] : 0;
-    unsigned long long slack = ok ? capacity - pw[p] : 0;
-    int bp = ok ? p : 0x7fffffff;
-    // (conn, slack) lexicographic max, lowest index on ties
+__device__ __forceinline__ void table_add(BucketTable& t, unsigned key, unsigned long long w,
+                                          unsigned long long* global) {
+    const unsigned h = (key * 2654435761U) >> 21;  // 11 bits
+#pragma unroll 1
+    for (int probe = 0; probe < 64; ++probe) {
+        const unsigned slot = (h + probe) & (kTableSlots - 1);
+        const unsigned prev = atomicCAS(&t.key[slot], kTableEmpty, key);
+        if (prev == kTableEmpty || prev == key) {
+            atomicAdd(&t.value[slot], w);
+            return;
+        }
+    }
+    atomicAdd(global + key, w);
+}
+
+__device__ __forceinline__ void table_flush(const BucketTable& t, unsigned long long* global) {
+    for (int i = threadIdx.x; i < kTableSlots; i += blockDim.x)
+        if (t.key[i] != kTableEmpty && t.value[i]) atomicAdd(global + t.key[i], t.value[i]);
+}
+
+// Contiguous slice [begin, end) of count items owned by this workgroup.
+__device__ __forceinline__ void block_slice(long long count, long long& begin, long long& end) {
+    const long long per = (count + gridDim.x - 1) / gridDim.x;
+    begin = min(count, static_cast<long long>(blockIdx.x) * per);
+    end = min(count, begin + per);
+}
+
+// For the workgroup's threads in index order: a pending thread with key (< 32)
+// and weight w receives running[key] plus the weight of the pending threads
+// with the same key and a lower index; running[] then advances by the pass.
+// Called by every thread of the workgroup.
+__device__ unsigned long long block_key_prefix(bool pending, int key, unsigned long long w,
+                                               unsigned long long (*wave_total)[32], unsigned long long* running) {
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    if (lane < 32) wave_total[wave][lane] = 0;
+    unsigned long long mine = 0;
+    unsigned long long open = __ballot(pending);
+    while (open) {
+        const int leader = __ffsll(static_cast<unsigned long long>(open)) - 1;
+        const int current = __shfl(key, leader, kWave);
+        const bool member = pending && key == current;
+        const unsigned long long members = __ballot(member);
+        unsigned long long x = member ? w : 0;
 #pragma unroll
-    for (int m = G / 2; m > 0; m /= 2) {
-        const auto oc = shfl_xor_u64(conn, m, G);
-        const auto os = shfl_xor_u64(slack, m, G);
-        const int op = __shfl_xor(bp, m, G);
-        const bool ob = op != 0x7fffffff;
-        const bool mine = bp != 0x7fffffff;
-        if (ob && (!mine || oc > conn || (oc == conn && (os > slack || (os == slack && op < bp))))) {
-            conn = oc;
-            slack = os;
-            bp = op;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const auto other = shfl_up_u64(x, d, kWave);
+            if (lane >= d) x += other;
         }
+        if (member) mine = x - w;
+        const auto total = shfl_u64(x, kWave - 1, kWave);
+        if (lane == 0) wave_total[wave][current] = total;
+        open &= ~members;
     }
-    const unsigned long long current = shfl_u64(p < k ? cache[v * k + p] : 0, source, G);
-    if (p != 0) return;
-    int best = bp == 0x7fffffff ? -1 : bp;
-    if (best >= 0 && conn == 0) {
-        std::uint64_t total_slack = 0;
-        for (int q = 0; q < k; ++q)
-            if (q != source && pw[q] < capacity && weight <= capacity - pw[q])
-                total_slack = ~0ULL - total_slack < capacity - pw[q] ? ~0ULL : total_slack + capacity - pw[q];
-        std::uint64_t ticket = static_cast<std::uint64_t>(mix32(static_cast<std::uint32_t>(v))) % total_slack;
-        for (int q = 0; q < k; ++q) {
-            if (q == source || pw[q] >= capacity || weight > capacity - pw[q]) continue;
-            const std::uint64_t s = capacity - pw[q];
-            if (ticket < s) {
-                best = q;
-                break;
-            }
-            ticket -= s;
+    __syncthreads();
+    if (pending) {
+        for (int q = 0; q < wave; ++q) mine += wave_total[q][key];
+        mine += running[key];
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        unsigned long long pass = 0;
+        for (int q = 0; q < kBlock / kWave; ++q) pass += wave_total[q][threadIdx.x];
+        running[threadIdx.x] += pass;
+    }
+    __syncthreads();
+    return mine;
+}
+
+// Appends the taking threads' vertices to the light/heavy mover lists (one
+// global atomic per list per workgroup pass) and flags them. Called by every
+// thread of the workgroup.
+template <typename O>
+__device__ void append_movers(bool take, std::int32_t v, const O* offsets, std::int32_t* light,
+                              std::int32_t* heavy, std::int32_t* flags, RoundState* st, unsigned* counts,
+                              unsigned long long* base) {
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const bool is_heavy = take && offsets[v + 1] - offsets[v] > kHeavyDegree;
+    const bool is_light = take && !is_heavy;
+    const unsigned long long bl = __ballot(is_light), bh = __ballot(is_heavy);
+    if (lane == 0) {
+        counts[wave] = static_cast<unsigned>(__popcll(bl));
+        counts[kBlock / kWave + wave] = static_cast<unsigned>(__popcll(bh));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tl = 0, th = 0;
+        for (int q = 0; q < kBlock / kWave; ++q) {
+            tl += counts[q];
+            th += counts[kBlock / kWave + q];
         }
+        base[0] = tl ? atomicAdd(reinterpret_cast<unsigned long long*>(&st->light), tl) : 0;
+        base[1] = th ? atomicAdd(reinterpret_cast<unsigned long long*>(&st->heavy), th) : 0;
     }
-    if (best >= 0) {
-        const double gain = conn >= current ? static_cast<double>(conn - current) : -static_cast<double>(current - conn);
-        const double per_weight = gain / static_cast<double>(weight);
-        buckets[v] = source * kBalanceWidth + gain_bucket(per_weight, powers) * kBalanceSections +
-                     static_cast<int>(mix32(static_cast<std::uint32_t>(v)) % kBalanceSections);
-        targets[v] = best;
-    } else {
-        buckets[v] = -1;
-        targets[v] = -1;
+    __syncthreads();
+    if (take) {
+        unsigned before = 0;
+        const unsigned* row = is_heavy ? counts + kBlock / kWave : counts;
+        for (int q = 0; q < wave; ++q) before += row[q];
+        const unsigned long long mask = is_heavy ? bh : bl;
+        before += static_cast<unsigned>(__popcll(mask & lanemask_lt()));
+        (is_heavy ? heavy : light)[(is_heavy ? base[1] : base[0]) + before] = v;
+        flags[v] = 1;
     }
+    __syncthreads();
 }
 
-__global__ void bucket_flags_kernel(std::int64_t n, const std::int32_t* buckets, std::int32_t* flags) {
-    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v < n) flags[v] = buckets[v] >= 0;
-}
-
-template <typename W>
-__global__ void bucket_keys_kernel(std::int64_t p, const std::int32_t* list, const std::int32_t* buckets,
-                                   std::uint64_t* keys) {
-    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < p) keys[i] = static_cast<std::uint64_t>(buckets[list[i]]);
-}
-
-// Segmented exclusive rank of weights by bucket; bucket totals at segment ends.
-template <typename W>
-struct RankLoad {
-    const std::int32_t* order;
-    const std::int32_t* buckets;
-    const W* weights;
-    __device__ __forceinline__ SegValue operator()(std::int64_t i) const {
-        const std::int32_t v = order[i];
-        const bool head = i == 0 || buckets[order[i - 1]] != buckets[v];
-        return SegValue{static_cast<unsigned long long>(weights[v]), head ? 1U : 0U};
+// Candidates: vertices of overloaded parts, in id order. Every flag is cleared
+// on the way (movers are flagged later in the iteration).
+struct OverloadedLoad {
+    const std::int32_t* labels;
+    const unsigned long long* part_weights;
+    std::uint64_t capacity;
+    __device__ __forceinline__ long long operator()(std::int64_t v) const {
+        return part_weights[labels[v]] > capacity ? 1 : 0;
     }
 };
-template <typename W>
-struct RankStore {
-    const std::int32_t* order;
-    const std::int32_t* buckets;
-    const W* weights;
+struct CandidateStore {
+    std::int32_t* candidates;
+    std::int32_t* flags;
     std::int64_t n;
-    unsigned long long* ranks;   // by vertex
-    unsigned long long* counts;  // by bucket
-    __device__ __forceinline__ void operator()(std::int64_t i, const SegValue& prefix, const SegValue& item) const {
-        const SegValue incl = SegOp{}(prefix, item);
-        const std::int32_t v = order[i];
-        ranks[v] = incl.value - static_cast<unsigned long long>(weights[v]);
-        const bool last = i == n - 1 || buckets[order[i + 1]] != buckets[v];
-        if (last) counts[buckets[v]] = incl.value;
+    RoundState* st;
+    __device__ __forceinline__ void operator()(std::int64_t v, long long prefix, long long item) const {
+        flags[v] = 0;
+        if (item) candidates[prefix] = static_cast<std::int32_t>(v);
+        if (v == n - 1) st->candidates = prefix + item;
     }
 };
 
-// balance_scan_buckets (:188-202): exclusive sum over the 800 buckets of each part.
-__global__ void __launch_bounds__(kBlock) scan_buckets_kernel(const unsigned long long* counts,
-                                                              unsigned long long* offsets) {
+// incremental_rebalance_proposals (:111-157) + balance_source_histogram (:174-186).
+// One group of G lanes per candidate, lane j owning parts [8j, 8j + 8).
+template <typename W, int G>
+__global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundState* st, const std::int32_t* cand,
+                                                                   const unsigned long long* cache,
+                                                                   const std::int32_t* labels, const W* vertex_weights,
+                                                                   int k, const unsigned long long* part_weights,
+                                                                   std::uint64_t capacity, PowerTable powers,
+                                                                   std::int32_t* targets, std::int32_t* cbucket,
+                                                                   unsigned long long* replicas) {
+    __shared__ unsigned long long pw[32];
+    __shared__ BucketTable table;
+    if (threadIdx.x < 32) pw[threadIdx.x] = threadIdx.x < k ? part_weights[threadIdx.x] : 0;
+    table_clear(table);
+    __syncthreads();
+    unsigned long long* counts = replicas + static_cast<long long>(blockIdx.x % kReplicas) * k * kBalanceWidth;
+    const long long count = st->candidates;
+    const int j = threadIdx.x & (G - 1);
+    const long long groups = static_cast<long long>(gridDim.x) * (kBlock / G);
+    for (long long index = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) / G; index < count;
+         index += groups) {
+        const std::int64_t v = cand[index];
+        const std::int32_t source = labels[v];
+        const std::uint64_t weight = static_cast<std::uint64_t>(vertex_weights[v]);
+        const unsigned long long* row = cache + v * k;
+        unsigned long long slack[kRowSpan];
+        unsigned long long conn = 0, best_slack = 0, current = 0;
+        int bp = 0x7fffffff;
+        // lane-local best by (conn, slack), lowest index on ties (ascending p)
+#pragma unroll
+        for (int e = 0; e < kRowSpan; ++e) {
+            const int p = j * kRowSpan + e;
+            const unsigned long long value = p < k ? row[p] : 0;
+            if (p == source) current = value;
+            const bool ok = p < k && p != source && pw[p] < capacity && weight <= capacity - pw[p];
+            slack[e] = ok ? capacity - pw[p] : 0;
+            if (ok && (bp == 0x7fffffff || value > conn || (value == conn && slack[e] > best_slack))) {
+                conn = value;
+                best_slack = slack[e];
+                bp = p;
+            }
+        }
+#pragma unroll
+        for (int m = G / 2; m > 0; m /= 2) {
+            const auto oc = shfl_xor_u64(conn, m, G);
+            const auto os = shfl_xor_u64(best_slack, m, G);
+            const int op = __shfl_xor(bp, m, G);
+            const auto ocur = shfl_xor_u64(current, m, G);
+            const bool ob = op != 0x7fffffff;
+            const bool mine = bp != 0x7fffffff;
+            if (ob && (!mine || oc > conn || (oc == conn && (os > best_slack || (os == best_slack && op < bp))))) {
+                conn = oc;
+                best_slack = os;
+                bp = op;
+            }
+            current = ocur > current ? ocur : current;  // only the source's lane holds it
+        }
+        int best = bp == 0x7fffffff ? -1 : bp;
+        if (best >= 0 && conn == 0) {
+            // No affinity anywhere: a ticket mix32(v) mod total slack walks the
+            // eligible parts in index order. Saturating sums of non-negative terms do
+            // not depend on the order of addition, so the running total before part p
+            // is the lane's running prefix plus the saturated total of lower lanes;
+            // the walk stops at the first part whose running total exceeds the ticket.
+            unsigned long long running[kRowSpan];
+            unsigned long long lane_total = 0;
+#pragma unroll
+            for (int e = 0; e < kRowSpan; ++e) {
+                lane_total = ~0ULL - lane_total < slack[e] ? ~0ULL : lane_total + slack[e];
+                running[e] = lane_total;
+            }
+            unsigned long long inclusive = lane_total;
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) {
+                const auto other = shfl_up_u64(inclusive, d, G);
+                if (j >= d) inclusive = ~0ULL - inclusive < other ? ~0ULL : inclusive + other;
+            }
+            const unsigned long long before = G > 1 ? shfl_up_u64(inclusive, 1, G) : 0;
+            const unsigned long long lower = j > 0 ? before : 0;
+            const unsigned long long total = shfl_u64(inclusive, G - 1, G);  // > 0: some part is eligible
+            const std::uint32_t hash = mix32(static_cast<std::uint32_t>(v));
+            const unsigned long long ticket =
+                total > 0xffffffffULL ? hash : hash % static_cast<std::uint32_t>(total);
+            int hit = 0x7fffffff;
+#pragma unroll
+            for (int e = kRowSpan - 1; e >= 0; --e) {
+                const unsigned long long upto = ~0ULL - lower < running[e] ? ~0ULL : lower + running[e];
+                if (upto > ticket) hit = j * kRowSpan + e;
+            }
+#pragma unroll
+            for (int m = G / 2; m > 0; m /= 2) {
+                const int other = __shfl_xor(hit, m, G);
+                hit = other < hit ? other : hit;
+            }
+            best = hit;
+        }
+        if (j == 0) {
+            int bucket = -1;
+            if (best >= 0) {
+                const double gain =
+                    conn >= current ? static_cast<double>(conn - current) : -static_cast<double>(current - conn);
+                const double per_weight = gain / static_cast<double>(weight);
+                bucket = source * kBalanceWidth + gain_bucket(per_weight, powers) * kBalanceSections +
+                         static_cast<int>(mix32(static_cast<std::uint32_t>(v)) % kBalanceSections);
+                table_add(table, static_cast<unsigned>(bucket), weight, counts);
+            }
+            targets[v] = best;
+            cbucket[index] = bucket;
+        }
+    }
+    __syncthreads();
+    table_flush(table, counts);
+}
+
+// balance_scan_buckets (:188-202): bucket totals (the kReplicas copies summed),
+// the exclusive sum over the 800 buckets of each part, and the part's threshold:
+// its overload (sources) or slack (destinations).
+__global__ void __launch_bounds__(kBlock) balance_scan_kernel(const unsigned long long* replicas, int k,
+                                                              unsigned long long* counts,
+                                                              unsigned long long* offsets,
+                                                              const unsigned long long* part_weights,
+                                                              std::uint64_t capacity, bool destinations,
+                                                              unsigned long long* thresholds) {
     __shared__ unsigned long long waves[kBlock / kWave + 1];
     constexpr int kPer = (kBalanceWidth + kBlock - 1) / kBlock;
+    const long long width = static_cast<long long>(k) * kBalanceWidth;
     unsigned long long items[kPer];
     unsigned long long local = 0;
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
         const int b = threadIdx.x * kPer + i;
-        items[i] = b < kBalanceWidth ? counts[static_cast<std::int64_t>(blockIdx.x) * kBalanceWidth + b] : 0;
+        unsigned long long total = 0;
+        if (b < kBalanceWidth) {
+            const long long bucket = static_cast<long long>(blockIdx.x) * kBalanceWidth + b;
+            for (int r = 0; r < kReplicas; ++r) total += replicas[r * width + bucket];
+            counts[bucket] = total;
+        }
+        items[i] = total;
         local += items[i];
     }
     unsigned long long total;
@@ -432,48 +672,233 @@ __global__ void __launch_bounds__(kBlock) scan_buckets_kernel(const unsigned lon
         if (b < kBalanceWidth) offsets[static_cast<std::int64_t>(blockIdx.x) * kBalanceWidth + b] = prefix;
         prefix += items[i];
     }
-}
-
-// balance_select_sources (:204-217)
-__global__ void select_sources_kernel(std::int64_t p, const std::int32_t* list, std::int32_t* buckets,
-                                      const unsigned long long* ranks, const unsigned long long* offsets,
-                                      const unsigned long long* part_weights, std::uint64_t capacity,
-                                      const std::int32_t* targets) {
-    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= p) return;
-    const std::int32_t v = list[i];
-    const int b = buckets[v];
-    const int source = b / kBalanceWidth;
-    if (offsets[b] + ranks[v] >= part_weights[source] - capacity) {
-        buckets[v] = -1;
-        return;
+    if (threadIdx.x == 0) {
+        const unsigned long long w = part_weights[blockIdx.x];
+        thresholds[blockIdx.x] =
+            destinations ? (w < capacity ? capacity - w : 0) : (w > capacity ? w - capacity : 0);
     }
-    buckets[v] = targets[v] * kBalanceWidth + b % kBalanceWidth;
 }
 
-// balance_select_destinations (:219-232)
+// Where a bucket stands against its part's threshold T, exactly for any
+// non-negative weights (ranks of members lie in [0, count]):
+//   sources      member leaves iff offset + rank < T
+//                taken iff offset + count < T, refused iff offset >= T
+//   destinations member enters iff offset + rank + weight <= T
+//                taken iff offset + count <= T, refused iff offset > T
+// The in-between intervals of a part's non-empty buckets are disjoint, so at
+// most one bucket per part is a boundary bucket.
+enum BucketSide { kTaken, kRefused, kBoundary };
+__device__ __forceinline__ BucketSide source_side(unsigned long long offset, unsigned long long count,
+                                                  unsigned long long threshold) {
+    if (offset + count < threshold) return kTaken;
+    if (offset >= threshold) return kRefused;
+    return kBoundary;
+}
+__device__ __forceinline__ BucketSide destination_side(unsigned long long offset, unsigned long long count,
+                                                       unsigned long long threshold) {
+    if (offset + count <= threshold) return kTaken;
+    if (offset > threshold) return kRefused;
+    return kBoundary;
+}
+
+// Destination bucket codes in dbucket[i]: d >= 0 selected; -1 none; -2 source
+// boundary (rank pending); -3 - d destination boundary (rank pending).
+constexpr int kNoBucket = -1;
+constexpr int kSourcePending = -2;
+
+// balance_select_sources (:204-217) for non-boundary buckets; boundary members
+// are summed per part and workgroup for the multisplit.
 template <typename W>
-__global__ void select_destinations_kernel(std::int64_t n, const std::int32_t* buckets,
-                                           const unsigned long long* ranks, const unsigned long long* offsets,
-                                           const unsigned long long* part_weights, std::uint64_t capacity,
-                                           const W* weights, std::int32_t* flags) {
-    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const int b = buckets[v];
-    std::int32_t f = 0;
-    if (b >= 0) {
-        const unsigned long long before = offsets[b] + ranks[v];
-        const unsigned long long slack = capacity - part_weights[b / kBalanceWidth];
-        f = before <= slack && static_cast<unsigned long long>(weights[v]) <= slack - before;
+__global__ void __launch_bounds__(kBlock) balance_sources_kernel(const RoundState* st, const std::int32_t* cand,
+                                                                 const std::int32_t* cbucket, std::int32_t* dbucket,
+                                                                 const std::int32_t* targets,
+                                                                 const W* vertex_weights,
+                                                                 const unsigned long long* counts,
+                                                                 const unsigned long long* offsets,
+                                                                 const unsigned long long* thresholds,
+                                                                 unsigned long long* block_sums,
+                                                                 unsigned long long* dst_replicas, long long width) {
+    __shared__ BucketTable table;
+    __shared__ unsigned long long sums[32];
+    unsigned long long* dst_counts = dst_replicas + static_cast<long long>(blockIdx.x % kReplicas) * width;
+    table_clear(table);
+    if (threadIdx.x < 32) sums[threadIdx.x] = 0;
+    __syncthreads();
+    long long begin, end;
+    block_slice(st->candidates, begin, end);
+    for (long long i = begin + threadIdx.x; i < end; i += blockDim.x) {
+        const int b = cbucket[i];
+        int d = kNoBucket;
+        if (b >= 0) {
+            const int s = b / kBalanceWidth;
+            const std::int32_t v = cand[i];
+            const auto w = static_cast<unsigned long long>(vertex_weights[v]);
+            const BucketSide side = source_side(offsets[b], counts[b], thresholds[s]);
+            if (side == kTaken) {
+                d = targets[v] * kBalanceWidth + b % kBalanceWidth;
+                table_add(table, static_cast<unsigned>(d), w, dst_counts);
+            } else if (side == kBoundary) {
+                d = kSourcePending;
+                atomicAdd(&sums[s], w);
+            }
+        }
+        dbucket[i] = d;
     }
-    flags[v] = f;
+    __syncthreads();
+    table_flush(table, dst_counts);
+    if (threadIdx.x < 32) block_sums[static_cast<long long>(blockIdx.x) * 32 + threadIdx.x] = sums[threadIdx.x];
+}
+
+// Exclusive scan of each part's column of per-workgroup sums (blocks <= 1024).
+__global__ void __launch_bounds__(kBlock) column_scan_kernel(const unsigned long long* sums, int blocks,
+                                                             unsigned long long* prefix) {
+    __shared__ unsigned long long waves[kBlock / kWave + 1];
+    constexpr int kPer = kRoundBlocks / kBlock;
+    const int column = blockIdx.x;
+    unsigned long long items[kPer];
+    unsigned long long local = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int b = threadIdx.x * kPer + i;
+        items[i] = b < blocks ? sums[static_cast<long long>(b) * 32 + column] : 0;
+        local += items[i];
+    }
+    unsigned long long total;
+    unsigned long long running = block_exclusive_scan(local, SumOp<unsigned long long>{}, &total, waves);
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int b = threadIdx.x * kPer + i;
+        if (b < blocks) prefix[static_cast<long long>(b) * 32 + column] = running;
+        running += items[i];
+    }
+}
+
+// Source boundary members: taken while offset + rank < overload.
+template <typename W>
+__global__ void __launch_bounds__(kBlock) balance_source_ranks_kernel(const RoundState* st, const std::int32_t* cand,
+                                                                      const std::int32_t* cbucket,
+                                                                      std::int32_t* dbucket,
+                                                                      const std::int32_t* targets,
+                                                                      const W* vertex_weights,
+                                                                      const unsigned long long* offsets,
+                                                                      const unsigned long long* thresholds,
+                                                                      const unsigned long long* column_prefix,
+                                                                      unsigned long long* dst_replicas,
+                                                                      long long width) {
+    __shared__ BucketTable table;
+    unsigned long long* dst_counts = dst_replicas + static_cast<long long>(blockIdx.x % kReplicas) * width;
+    __shared__ unsigned long long wave_total[kBlock / kWave][32];
+    __shared__ unsigned long long running[32];
+    table_clear(table);
+    if (threadIdx.x < 32) running[threadIdx.x] = column_prefix[static_cast<long long>(blockIdx.x) * 32 + threadIdx.x];
+    __syncthreads();
+    long long begin, end;
+    block_slice(st->candidates, begin, end);
+    for (long long base = begin; base < end; base += blockDim.x) {
+        const long long i = base + threadIdx.x;
+        const bool pending = i < end && dbucket[i] == kSourcePending;
+        if (!__syncthreads_or(pending)) continue;
+        int b = 0, s = 0;
+        std::int32_t v = 0;
+        unsigned long long w = 0;
+        if (pending) {
+            b = cbucket[i];
+            s = b / kBalanceWidth;
+            v = cand[i];
+            w = static_cast<unsigned long long>(vertex_weights[v]);
+        }
+        const unsigned long long rank = block_key_prefix(pending, s, w, wave_total, running);
+        if (pending) {
+            if (offsets[b] + rank < thresholds[s]) {
+                const int d = targets[v] * kBalanceWidth + b % kBalanceWidth;
+                dbucket[i] = d;
+                table_add(table, static_cast<unsigned>(d), w, dst_counts);
+            } else {
+                dbucket[i] = kNoBucket;
+            }
+        }
+    }
+    __syncthreads();
+    table_flush(table, dst_counts);
+}
+
+// balance_select_destinations (:219-232) for non-boundary buckets: taken ones
+// become movers; boundary members are summed for the multisplit.
+template <typename O, typename W>
+__global__ void __launch_bounds__(kBlock) balance_destinations_kernel(
+    RoundState* st, const std::int32_t* cand, std::int32_t* dbucket, const W* vertex_weights, const O* offsets_csr,
+    const unsigned long long* counts, const unsigned long long* offsets, const unsigned long long* thresholds,
+    unsigned long long* block_sums, std::int32_t* light, std::int32_t* heavy, std::int32_t* flags) {
+    __shared__ unsigned long long sums[32];
+    __shared__ unsigned counts_lds[2 * (kBlock / kWave)];
+    __shared__ unsigned long long base_lds[2];
+    if (threadIdx.x < 32) sums[threadIdx.x] = 0;
+    __syncthreads();
+    long long begin, end;
+    block_slice(st->candidates, begin, end);
+    for (long long base = begin; base < end; base += blockDim.x) {
+        const long long i = base + threadIdx.x;
+        bool take = false;
+        std::int32_t v = 0;
+        if (i < end) {
+            const int d = dbucket[i];
+            if (d >= 0) {
+                const int part = d / kBalanceWidth;
+                v = cand[i];
+                const BucketSide side = destination_side(offsets[d], counts[d], thresholds[part]);
+                if (side == kTaken) {
+                    take = true;
+                } else if (side == kBoundary) {
+                    dbucket[i] = -3 - d;
+                    atomicAdd(&sums[part], static_cast<unsigned long long>(vertex_weights[v]));
+                }
+            }
+        }
+        append_movers(take, v, offsets_csr, light, heavy, flags, st, counts_lds, base_lds);
+    }
+    if (threadIdx.x < 32) block_sums[static_cast<long long>(blockIdx.x) * 32 + threadIdx.x] = sums[threadIdx.x];
+}
+
+// Destination boundary members: admitted while offset + rank + weight <= slack.
+template <typename O, typename W>
+__global__ void __launch_bounds__(kBlock) balance_destination_ranks_kernel(
+    RoundState* st, const std::int32_t* cand, const std::int32_t* dbucket, const W* vertex_weights,
+    const O* offsets_csr, const unsigned long long* offsets, const unsigned long long* thresholds,
+    const unsigned long long* column_prefix, std::int32_t* light, std::int32_t* heavy, std::int32_t* flags) {
+    __shared__ unsigned long long wave_total[kBlock / kWave][32];
+    __shared__ unsigned long long running[32];
+    __shared__ unsigned counts_lds[2 * (kBlock / kWave)];
+    __shared__ unsigned long long base_lds[2];
+    if (threadIdx.x < 32) running[threadIdx.x] = column_prefix[static_cast<long long>(blockIdx.x) * 32 + threadIdx.x];
+    __syncthreads();
+    long long begin, end;
+    block_slice(st->candidates, begin, end);
+    for (long long base = begin; base < end; base += blockDim.x) {
+        const long long i = base + threadIdx.x;
+        const int code = i < end ? dbucket[i] : kNoBucket;
+        const bool pending = code <= -3;
+        if (!__syncthreads_or(pending)) continue;
+        int d = 0, part = 0;
+        std::int32_t v = 0;
+        unsigned long long w = 0;
+        if (pending) {
+            d = -3 - code;
+            part = d / kBalanceWidth;
+            v = cand[i];
+            w = static_cast<unsigned long long>(vertex_weights[v]);
+        }
+        const unsigned long long rank = block_key_prefix(pending, part, w, wave_total, running);
+        const bool take = pending && offsets[d] + rank + w <= thresholds[part];
+        append_movers(take, v, offsets_csr, light, heavy, flags, st, counts_lds, base_lds);
+    }
 }
 
 // ---------------------------------------------------------------- drivers
 
+// Lanes per cache row: G * kRowSpan >= k.
 int group_for_k(int k) {
-    int g = 2;
-    while (g < k) g <<= 1;
+    int g = 1;
+    while (g * kRowSpan < k) g <<= 1;
     return g;
 }
 
@@ -538,25 +963,11 @@ struct Ops {
                     (n + 1) * sizeof(O) + m * (4 + sizeof(W)) + 4 * m + 8.0 * n * k);
     }
 
-    static void lp_phase(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
-                         const unsigned long long* cache, const std::int32_t* labels, int k, double filter,
-                         std::int32_t* locks, std::int32_t* targets, long long* gains, std::int32_t* flags,
-                         std::int32_t* movers, std::int64_t* count) {
-        const std::int64_t n = g->n;
+    static void filter_movers(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
+                              const std::int32_t* labels, const std::int32_t* targets, const long long* gains,
+                              std::int32_t* flags) {
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
-        const int G = group_for_k(k);
-        profile_begin(ctx, GPART_PROFILE_REFINE_PROPOSALS);
-        const unsigned blocks = grid_for(n * G);
-        switch (G) {
-            case 2: hipLaunchKernelGGL(lp_proposals_kernel<2>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            case 4: hipLaunchKernelGGL(lp_proposals_kernel<4>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            case 8: hipLaunchKernelGGL(lp_proposals_kernel<8>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            case 16: hipLaunchKernelGGL(lp_proposals_kernel<16>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            default: hipLaunchKernelGGL(lp_proposals_kernel<32>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-        }
-        GPART_LAUNCH_CHECK();
-        profile_end(ctx, GPART_PROFILE_REFINE_PROPOSALS, 8.0 * n * k + 24.0 * n);
         profile_begin(ctx, GPART_PROFILE_REFINE_FILTER);
         if (c->low_count)
             hipLaunchKernelGGL((priority_filter_kernel<O, W, 8>), dim3(grid_for(c->low_count * 8)), dim3(kBlock), 0,
@@ -570,116 +981,150 @@ struct Ops {
                                dim3(kBlock), 0, ctx->stream, c->high_count, c->high, off, g->neighbors, w, labels,
                                targets, gains, flags);
         GPART_LAUNCH_CHECK();
-        const double nd = static_cast<double>(n), m = static_cast<double>(g->m);
+        const double nd = static_cast<double>(g->n), m = static_cast<double>(g->m);
         profile_end(ctx, GPART_PROFILE_REFINE_FILTER, (nd + 1) * sizeof(O) + m * (4 + sizeof(W)) + 16 * m + 20 * nd);
-        Scratch<std::int64_t> d_count(ctx, 1);
-        compact_indices(ctx, flags, n, movers, d_count.get());
-        *count = read_scalar(ctx, d_count.get());
     }
 
-    static void apply(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_t* movers, std::int64_t count,
-                      std::int32_t* labels, const std::int32_t* targets, const std::int32_t* flags, int k,
-                      unsigned long long* cache, unsigned long long* part_weights, std::int32_t* locks, bool lp,
-                      std::int64_t* delta, std::uint64_t* pw_host) {
+    // incremental_update_cache_and_cut + incremental_commit over the round's movers.
+    static void apply(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c, RoundState* st,
+                      const std::int32_t* light, const std::int32_t* heavy, std::int32_t* labels,
+                      const std::int32_t* targets, const std::int32_t* flags, int k, unsigned long long* cache,
+                      unsigned long long* part_weights, std::int32_t* locks, bool lp) {
+        const std::int64_t n = g->n;
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
-        Scratch<long long> d(ctx, 1);
-        fill_bytes(ctx, d.get(), 0, 8);
-        if (count > 0) {
-            Scratch<std::int32_t> light(ctx, count), heavy(ctx, count);
-            Scratch<unsigned long long> counters(ctx, 2);
-            fill_bytes(ctx, counters.get(), 0, 16);
-            hipLaunchKernelGGL(split_movers_kernel<O>, dim3(grid_for(count)), dim3(kBlock), 0, ctx->stream, count,
-                               movers, off, light.get(), heavy.get(), counters.get());
-            GPART_LAUNCH_CHECK();
-            unsigned long long h[2];
-            read_back(ctx, h, counters.get(), sizeof(h));
-            const auto nl = static_cast<std::int64_t>(h[0]), nh = static_cast<std::int64_t>(h[1]);
-            if (nl)
-                hipLaunchKernelGGL((update_cache_kernel<O, W, 8>), dim3(grid_for(nl * 8)), dim3(kBlock), 0, ctx->stream,
-                                   nl, light.get(), off, g->neighbors, w, labels, targets, flags, k, cache, d.get());
-            if (nh)
-                hipLaunchKernelGGL((update_cache_kernel<O, W, kBlock>), dim3(static_cast<unsigned>(nh)), dim3(kBlock), 0,
-                                   ctx->stream, nh, heavy.get(), off, g->neighbors, w, labels, targets, flags, k, cache,
-                                   d.get());
-            hipLaunchKernelGGL(commit_kernel<W>, dim3(grid_for(count)), dim3(kBlock), 0, ctx->stream, count, movers,
-                               labels, targets, static_cast<const W*>(g->vertex_weights), part_weights, locks, lp);
-            GPART_LAUNCH_CHECK();
-        }
-        *delta = read_scalar(ctx, d.get());
-        read_back(ctx, pw_host, part_weights, sizeof(std::uint64_t) * k);
+        const auto light_blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * 8), 4 * kRoundBlocks));
+        hipLaunchKernelGGL((update_cache_kernel<O, W, 8>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
+                           &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta);
+        const std::int64_t heavy_bound = c->medium_count + c->high_count;
+        if (heavy_bound > 0)
+            hipLaunchKernelGGL((update_cache_kernel<O, W, kBlock>),
+                               dim3(static_cast<unsigned>(std::min<std::int64_t>(heavy_bound, kRoundBlocks))),
+                               dim3(kBlock), 0, ctx->stream, &st->heavy, heavy, off, g->neighbors, w, labels, targets,
+                               flags, k, cache, &st->delta);
+        hipLaunchKernelGGL(commit_kernel<W>,
+                           dim3(static_cast<unsigned>(std::min<std::int64_t>(grid_for(n), kRoundBlocks))),
+                           dim3(kBlock), 0, ctx->stream, st, light, heavy, labels, targets,
+                           static_cast<const W*>(g->vertex_weights), part_weights, locks, lp, k);
+        GPART_LAUNCH_CHECK();
     }
 
-    static void rebalance_select(gpart_ctx* ctx, const gpart_csr_view* g, const unsigned long long* cache,
-                                 const std::int32_t* labels, int k, const unsigned long long* part_weights,
-                                 std::uint64_t capacity, std::int32_t* targets, std::int32_t* flags,
-                                 std::int32_t* movers, std::int64_t* count) {
+    static void finish(gpart_ctx* ctx, RoundState* st, const unsigned long long* part_weights, int k,
+                       gpart_refine_status* out) {
+        hipLaunchKernelGGL(round_status_kernel, dim3(1), dim3(kWave), 0, ctx->stream, st, part_weights, k);
+        GPART_LAUNCH_CHECK();
+        RoundState h;
+        read_back(ctx, &h, st, sizeof(RoundState));
+        out->movers = h.light + h.heavy;
+        out->cut_delta = h.delta;
+        out->candidates = h.candidates;
+        out->max_part_weight = h.max_weight;
+        for (int p = 0; p < 32; ++p) out->part_weights[p] = p < k ? h.pw[p] : 0;
+    }
+
+    // One LP round: proposals, priority filter, movers, apply; one synchronisation.
+    static void lp_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
+                         unsigned long long* cache, std::int32_t* labels, int k, double filter,
+                         std::int32_t* locks, std::int32_t* targets, long long* gains, std::int32_t* flags,
+                         unsigned long long* part_weights, gpart_refine_status* out) {
         const std::int64_t n = g->n;
+        const auto* off = static_cast<const O*>(g->offsets);
+        Scratch<RoundState> st(ctx, 1);
+        fill_bytes(ctx, st.get(), 0, sizeof(RoundState));
+        const int G = group_for_k(k);
+        profile_begin(ctx, GPART_PROFILE_REFINE_PROPOSALS);
+        const unsigned blocks = grid_for(n * G);
+        switch (G) {
+            case 1: hipLaunchKernelGGL(lp_proposals_kernel<1>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
+            case 2: hipLaunchKernelGGL(lp_proposals_kernel<2>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
+            default: hipLaunchKernelGGL(lp_proposals_kernel<4>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
+        }
+        GPART_LAUNCH_CHECK();
+        profile_end(ctx, GPART_PROFILE_REFINE_PROPOSALS, static_cast<double>(n) * (20.0 + 8.0 * k));
+        filter_movers(ctx, g, c, labels, targets, gains, flags);
+        Scratch<std::int32_t> light(ctx, n), heavy(ctx, n);
+        device_scan<MoverPair, MoverPairOp>(ctx, MoverLoad<O>{flags, off}, MoverStore{light.get(), heavy.get(), n,
+                                                                                      st.get()},
+                                            n, MoverPairOp{});
+        apply(ctx, g, c, st.get(), light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, locks,
+              true);
+        finish(ctx, st.get(), part_weights, k, out);
+    }
+
+    // One rebalance iteration: proposals, source and destination bucket quotas,
+    // apply; one synchronisation.
+    static void rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
+                                unsigned long long* cache, std::int32_t* labels, int k,
+                                unsigned long long* part_weights, std::uint64_t capacity, std::int32_t* targets,
+                                std::int32_t* flags, gpart_refine_status* out) {
+        const std::int64_t n = g->n;
+        const auto* off = static_cast<const O*>(g->offsets);
         const auto* vw = static_cast<const W*>(g->vertex_weights);
         static const PowerTable powers = make_power_table();
-        Scratch<std::int32_t> buckets(ctx, n), list(ctx, n), list_alt(ctx, n);
-        Scratch<unsigned long long> ranks(ctx, n), counts(ctx, static_cast<std::int64_t>(k) * kBalanceWidth),
-            offsets(ctx, static_cast<std::int64_t>(k) * kBalanceWidth);
-        Scratch<std::int64_t> d_count(ctx, 1);
-        hipLaunchKernelGGL(overloaded_flags_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, labels,
-                           part_weights, k, capacity, flags, targets, buckets.get());
-        GPART_LAUNCH_CHECK();
-        compact_indices(ctx, flags, n, list.get(), d_count.get());
-        const std::int64_t candidates = read_scalar(ctx, d_count.get());
+        const std::int64_t width = static_cast<std::int64_t>(k) * kBalanceWidth;
+        Scratch<RoundState> st(ctx, 1);
+        Scratch<unsigned long long> replicas(ctx, 2 * kReplicas * width), counts(ctx, 2 * width),
+            offsets(ctx, 2 * width), thresholds(ctx, 64);
+        fill_bytes(ctx, st.get(), 0, sizeof(RoundState));
+        fill_bytes(ctx, replicas.get(), 0, static_cast<std::size_t>(2 * kReplicas * width) * 8);
+        unsigned long long* src_replicas = replicas.get();
+        unsigned long long* dst_replicas = replicas.get() + kReplicas * width;
+        unsigned long long* src_counts = counts.get();
+        unsigned long long* dst_counts = counts.get() + width;
+        unsigned long long* src_offsets = offsets.get();
+        unsigned long long* dst_offsets = offsets.get() + width;
+        unsigned long long* src_thresholds = thresholds.get();
+        unsigned long long* dst_thresholds = thresholds.get() + 32;
+        Scratch<std::int32_t> cand(ctx, n), cbucket(ctx, n), dbucket(ctx, n), light(ctx, n), heavy(ctx, n);
+        const int slices = static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(grid_for(n), kRoundBlocks)));
+        Scratch<unsigned long long> sums(ctx, static_cast<std::int64_t>(slices) * 32),
+            prefix(ctx, static_cast<std::int64_t>(slices) * 32);
+
+        device_scan<long long, SumOp<long long>>(ctx, OverloadedLoad{labels, part_weights, capacity},
+                                                 CandidateStore{cand.get(), flags, n, st.get()}, n,
+                                                 SumOp<long long>{});
         const int G = group_for_k(k);
-        const unsigned blocks = grid_for(candidates * G);
-        const std::int32_t* cl = list.get();
-        if (candidates > 0) {
-            switch (G) {
-                case 2: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 2>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-                case 4: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 4>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-                case 8: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 8>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-                case 16: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 16>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-                default: hipLaunchKernelGGL((rebalance_proposals_kernel<W, 32>), dim3(blocks), dim3(kBlock), 0, ctx->stream, candidates, cl, cache, labels, vw, k, part_weights, capacity, powers, targets, buckets.get()); break;
-            }
-            GPART_LAUNCH_CHECK();
+        const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), 2 * kRoundBlocks));
+        profile_begin(ctx, GPART_PROFILE_REBALANCE);
+#define GPART_BALANCE_PROPOSALS(GG)                                                                              \
+    hipLaunchKernelGGL((balance_proposals_kernel<W, GG>), dim3(blocks), dim3(kBlock), 0, ctx->stream, st.get(),   \
+                       cand.get(), cache, labels, vw, k, part_weights, capacity, powers, targets, cbucket.get(), \
+                       src_replicas)
+        switch (G) {
+            case 1: GPART_BALANCE_PROPOSALS(1); break;
+            case 2: GPART_BALANCE_PROPOSALS(2); break;
+            default: GPART_BALANCE_PROPOSALS(4); break;
         }
-        const int bucket_bits = std::max(1, bit_width_u64(static_cast<std::uint64_t>(k) * kBalanceWidth - 1));
-        // two ranking rounds: by source bucket, then by destination bucket
-        for (int round = 0; round < 2; ++round) {
-            hipLaunchKernelGGL(bucket_flags_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, buckets.get(),
-                               flags);
-            GPART_LAUNCH_CHECK();
-            compact_indices(ctx, flags, n, list.get(), d_count.get());
-            const std::int64_t p = read_scalar(ctx, d_count.get());
-            fill_bytes(ctx, counts.get(), 0, static_cast<std::size_t>(k) * kBalanceWidth * 8);
-            std::int32_t* sorted = list.get();
-            if (p > 0) {
-                if (p > 1) {
-                    Scratch<std::uint64_t> keys(ctx, p), keys_alt(ctx, p);
-                    hipLaunchKernelGGL(bucket_keys_kernel<W>, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
-                                       list.get(), buckets.get(), keys.get());
-                    GPART_LAUNCH_CHECK();
-                    if (radix_sort_pairs(ctx, keys.get(), keys_alt.get(), reinterpret_cast<std::uint32_t*>(list.get()),
-                                         reinterpret_cast<std::uint32_t*>(list_alt.get()), p, 0, bucket_bits))
-                        sorted = list_alt.get();
-                }
-                device_scan<SegValue, SegOp>(ctx, RankLoad<W>{sorted, buckets.get(), vw},
-                                             RankStore<W>{sorted, buckets.get(), vw, p, ranks.get(), counts.get()}, p,
-                                             SegOp{});
-            }
-            hipLaunchKernelGGL(scan_buckets_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, counts.get(), offsets.get());
-            GPART_LAUNCH_CHECK();
-            if (round == 0) {
-                if (p > 0) {
-                    hipLaunchKernelGGL(select_sources_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
-                                       sorted, buckets.get(), ranks.get(), offsets.get(), part_weights, capacity,
-                                       targets);
-                    GPART_LAUNCH_CHECK();
-                }
-            } else {
-                hipLaunchKernelGGL(select_destinations_kernel<W>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n,
-                                   buckets.get(), ranks.get(), offsets.get(), part_weights, capacity, vw, flags);
-                GPART_LAUNCH_CHECK();
-            }
-        }
-        compact_indices(ctx, flags, n, movers, d_count.get());
-        *count = read_scalar(ctx, d_count.get());
+#undef GPART_BALANCE_PROPOSALS
+        GPART_LAUNCH_CHECK();
+        profile_end(ctx, GPART_PROFILE_REBALANCE, 0.0);
+        hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, src_replicas, k, src_counts,
+                           src_offsets, part_weights, capacity, false, src_thresholds);
+        hipLaunchKernelGGL(balance_sources_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st.get(),
+                           cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_counts, src_offsets,
+                           src_thresholds, sums.get(), dst_replicas, width);
+        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, sums.get(), slices,
+                           prefix.get());
+        hipLaunchKernelGGL(balance_source_ranks_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st.get(),
+                           cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_offsets, src_thresholds,
+                           prefix.get(), dst_replicas, width);
+        hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, dst_replicas, k, dst_counts,
+                           dst_offsets, part_weights, capacity, true, dst_thresholds);
+        hipLaunchKernelGGL((balance_destinations_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
+                           st.get(), cand.get(), dbucket.get(), vw, off, dst_counts, dst_offsets, dst_thresholds,
+                           sums.get(), light.get(), heavy.get(), flags);
+        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, sums.get(), slices,
+                           prefix.get());
+        hipLaunchKernelGGL((balance_destination_ranks_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
+                           st.get(), cand.get(), dbucket.get(), vw, off, dst_offsets, dst_thresholds, prefix.get(),
+                           light.get(), heavy.get(), flags);
+        GPART_LAUNCH_CHECK();
+        apply(ctx, g, c, st.get(), light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, nullptr,
+              false);
+        finish(ctx, st.get(), part_weights, k, out);
+        // per candidate: id, label, weight, cache row, target and bucket writes
+        profile_add_bytes(ctx, GPART_PROFILE_REBALANCE,
+                          static_cast<double>(out->candidates) * (16.0 + sizeof(W) + 8.0 * k));
     }
 };
 
@@ -740,37 +1185,29 @@ int gpart_refine_build_cache(gpart_ctx* ctx, const gpart_csr_view* g, const gpar
     });
 }
 
-int gpart_refine_lp_phase(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes,
-                          const uint64_t* cache, const int32_t* labels, int k, double filter, int32_t* locks,
-                          int32_t* targets, int64_t* gains, int32_t* flags, int32_t* movers, int64_t* count) {
+int gpart_refine_lp_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes,
+                          uint64_t* cache, int32_t* labels, int k, double filter, int32_t* locks, int32_t* targets,
+                          int64_t* gains, int32_t* flags, uint64_t* part_weights, gpart_refine_status* status) {
     return guarded([&] {
+        if (k < 1 || k > 32) throw Error(GPART_ERR_INVALID_ARGUMENT, "part count must be in [1, 32]");
+        if (g->n <= 0) throw Error(GPART_ERR_INVALID_ARGUMENT, "refinement round needs vertices");
         GPART_WIDTH_DISPATCH(g->width,
-                             OpsT::lp_phase(ctx, g, classes, reinterpret_cast<const unsigned long long*>(cache), labels,
-                                            k, filter, locks, targets, reinterpret_cast<long long*>(gains), flags,
-                                            movers, count));
+                             OpsT::lp_round(ctx, g, classes, reinterpret_cast<unsigned long long*>(cache), labels, k,
+                                            filter, locks, targets, reinterpret_cast<long long*>(gains), flags,
+                                            reinterpret_cast<unsigned long long*>(part_weights), status));
     });
 }
 
-int gpart_refine_apply(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* movers, int64_t count, int32_t* labels,
-                       const int32_t* targets, const int32_t* flags, int k, uint64_t* cache, uint64_t* part_weights,
-                       int32_t* locks, int lp, int64_t* delta, uint64_t* part_weights_host) {
+int gpart_refine_rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes,
+                                 uint64_t* cache, int32_t* labels, int k, uint64_t* part_weights, uint64_t capacity,
+                                 int32_t* targets, int32_t* flags, gpart_refine_status* status) {
     return guarded([&] {
+        if (k < 1 || k > 32) throw Error(GPART_ERR_INVALID_ARGUMENT, "part count must be in [1, 32]");
+        if (g->n <= 0) throw Error(GPART_ERR_INVALID_ARGUMENT, "refinement round needs vertices");
         GPART_WIDTH_DISPATCH(g->width,
-                             OpsT::apply(ctx, g, movers, count, labels, targets, flags, k,
-                                         reinterpret_cast<unsigned long long*>(cache),
-                                         reinterpret_cast<unsigned long long*>(part_weights), locks, lp != 0, delta,
-                                         part_weights_host));
-    });
-}
-
-int gpart_refine_rebalance_select(gpart_ctx* ctx, const gpart_csr_view* g, const uint64_t* cache,
-                                  const int32_t* labels, int k, const uint64_t* part_weights, uint64_t capacity,
-                                  int32_t* targets, int32_t* flags, int32_t* movers, int64_t* count) {
-    return guarded([&] {
-        GPART_WIDTH_DISPATCH(g->width,
-                             OpsT::rebalance_select(ctx, g, reinterpret_cast<const unsigned long long*>(cache), labels,
-                                                    k, reinterpret_cast<const unsigned long long*>(part_weights),
-                                                    capacity, targets, flags, movers, count));
+                             OpsT::rebalance_round(ctx, g, classes, reinterpret_cast<unsigned long long*>(cache),
+                                                   labels, k, reinterpret_cast<unsigned long long*>(part_weights),
+                                                   capacity, targets, flags, status));
     });
 }
 

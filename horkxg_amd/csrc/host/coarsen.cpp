@@ -155,9 +155,10 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
         const auto admission_start = clock_type::now();
         const auto cut_before = diagnostics ? cluster_cut(graph, clusters) : 0;
         std::int64_t valid = 0, accepted = 0, nonempty = 0;
+        std::uint64_t predicted_gain = 0;
         GPART_CHECK(gpart_sclp_admit(ctx, n, proposals.data(), gains.data(), ties.data(), vertex_weights64.data(),
                                      stats.capacity, gain_bound, clusters.data(), cluster_weights.data(), &valid,
-                                     &accepted, &nonempty));
+                                     &accepted, &nonempty, &predicted_gain));
         cluster_count = nonempty;
         stats.lp_accepted += static_cast<std::uint64_t>(accepted);
         stats.capacity_rejected += static_cast<std::uint64_t>(valid - accepted);
@@ -169,7 +170,7 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
             const auto cut_after = cluster_cut(graph, clusters);
             if (cut_after > cut_before) throw std::runtime_error("SCLP synchronous batch increased cluster cut");
             gpart::detail::log() << " accepted=" << accepted << " capacity_rejected=" << (valid - accepted)
-                      << " capacity_rejection_ratio="
+                      << " predicted_gain=" << predicted_gain << " capacity_rejection_ratio="
                       << (valid == 0 ? 0.0 : static_cast<double>(valid - accepted) / static_cast<double>(valid))
                       << " cut_before=" << cut_before << " cut_after=" << cut_after
                       << " actual_gain=" << (cut_before - cut_after);
@@ -275,10 +276,9 @@ DeviceWeightedGraph<Types> contract_impl(const DeviceWeightedGraph<Types>& fine,
     coarse.edge_weights = DeviceVector<typename Types::WeightT>::adopt(static_cast<typename Types::WeightT*>(weights),
                                                                        static_cast<std::size_t>(coarse_m));
     if (coarse_m == 0) {
+        // no cross entries: clear() returns the two (unused) blocks to the context
         coarse.neighbors.clear();
         coarse.edge_weights.clear();
-        GPART_CHECK(gpart_device_free(ctx, neighbors));
-        GPART_CHECK(gpart_device_free(ctx, weights));
     }
     coarse.vertex_weights = std::move(agg.vertex_weights);
     gpart::detail::log() << "ml_gpu_contract_cross_edge_entries=" << coarse_m << " input_edge_entries=" << fine.edges()

@@ -112,7 +112,12 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
         }
     } else {
         const int precision = mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64;
-        const auto max_task = gpart_spectral_depth_max_task();
+        // A batched task runs on one workgroup, so its time grows with its size;
+        // the plain form spreads one task over the whole GPU at ~200 launches per
+        // task. Large tasks therefore take the plain form (measured crossover on
+        // MI355X: about a thousand vertices).
+        constexpr std::int64_t kPlainAbove = 1024;
+        const auto max_task = std::min<std::int64_t>(gpart_spectral_depth_max_task(), kPlainAbove);
         std::vector<gpart_bisect_task> depth{{0, n, 0, parts, total}};
         DeviceVector<gpart_bisect_task> device_tasks(32), device_next(64);
         while (!depth.empty()) {

@@ -86,7 +86,7 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     gpart_degree_classes classes{low.data(), medium.data(), high.data(), 0, 0, 0, 0, 0};
     GPART_CHECK(gpart_degree_classes_build(ctx, &view, &classes));
     DeviceVector<std::uint64_t> cache(N * static_cast<std::size_t>(k));
-    DeviceVector<std::int32_t> targets(N), locks(N, 0), flags(N, 0), movers(N);
+    DeviceVector<std::int32_t> targets(N), locks(N, 0), flags(N, 0);
     DeviceVector<std::int64_t> gains(N);
     GPART_CHECK(gpart_refine_build_cache(ctx, &view, &classes, level.partition.data(), k, cache.data()));
 
@@ -99,18 +99,17 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
         GPART_CHECK(gpart_memcpy_dtod(ctx, best_partition.data(), level.partition.data(), N * sizeof(std::int32_t)));
         GPART_CHECK(gpart_memcpy_dtod(ctx, best_part_weights.data(), level.part_weights.data(), k * sizeof(std::uint64_t)));
     };
-    auto apply = [&](std::int64_t count, bool lp) {
-        std::int64_t delta = 0;
-        GPART_CHECK(gpart_refine_apply(ctx, &view, movers.data(), count, level.partition.data(), targets.data(),
-                                       flags.data(), k, cache.data(), level.part_weights.data(), locks.data(),
-                                       lp ? 1 : 0, &delta, level.host_part_weights.data()));
+    // Book-keeping of an applied round (the reference's apply, :280-301).
+    auto absorb = [&](const gpart_refine_status& status) {
+        const std::int64_t delta = status.cut_delta;
         if (delta >= 0) {
             if (static_cast<std::uint64_t>(delta) > current_cut) throw std::runtime_error("incremental cut underflow");
             current_cut -= static_cast<std::uint64_t>(delta);
         } else {
             current_cut += static_cast<std::uint64_t>(-delta);
         }
-        stats.accepted += static_cast<std::uint64_t>(count);
+        stats.accepted += static_cast<std::uint64_t>(status.movers);
+        level.host_part_weights.assign(status.part_weights, status.part_weights + k);
         if (options.strict_verify) {
             if (level.cut() != current_cut) throw std::runtime_error("incremental cut differs from CSR scan");
             DeviceVector<std::uint64_t> check(N * static_cast<std::size_t>(k));
@@ -122,12 +121,12 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     auto rebalance = [&]() {
         for (int iteration = 0; iteration < 2 * k + 4 && level.maximum() > level.capacity; ++iteration) {
             ++stats.admission_iterations;
-            std::int64_t count = 0;
-            GPART_CHECK(gpart_refine_rebalance_select(ctx, &view, cache.data(), level.partition.data(), k,
-                                                      level.part_weights.data(), level.capacity, targets.data(),
-                                                      flags.data(), movers.data(), &count));
-            if (!count) return false;
-            apply(count, false);
+            gpart_refine_status status{};
+            GPART_CHECK(gpart_refine_rebalance_round(ctx, &view, &classes, cache.data(), level.partition.data(), k,
+                                                     level.part_weights.data(), level.capacity, targets.data(),
+                                                     flags.data(), &status));
+            if (!status.movers) return false;
+            absorb(status);
         }
         return level.maximum() <= level.capacity;
     };
@@ -137,18 +136,18 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     int stale = 0, empty = 0;
     for (int phase = 0; phase < rounds && stale < options.stale_rounds; ++phase) {
         stats.rounds = phase + 1;
-        std::int64_t count = 0;
-        GPART_CHECK(gpart_refine_lp_phase(ctx, &view, &classes, cache.data(), level.partition.data(), k, filter,
-                                          locks.data(), targets.data(), gains.data(), flags.data(), movers.data(),
-                                          &count));
-        stats.proposals += static_cast<std::uint64_t>(count);
-        if (!count) {
+        gpart_refine_status status{};
+        GPART_CHECK(gpart_refine_lp_round(ctx, &view, &classes, cache.data(), level.partition.data(), k, filter,
+                                          locks.data(), targets.data(), gains.data(), flags.data(),
+                                          level.part_weights.data(), &status));
+        stats.proposals += static_cast<std::uint64_t>(status.movers);
+        if (!status.movers) {
             if (++empty >= 2) break;
             ++stale;
             continue;
         }
         empty = 0;
-        apply(count, true);
+        absorb(status);
         if (!rebalance()) {
             ++stats.admission_fallbacks;
             break;

@@ -58,6 +58,11 @@ int gpart_device_free(gpart_ctx* ctx, void* ptr);
 /* Synchronise, then return every cached block to the device
  * (ScratchPoolSession::finish, scratch_pool.hpp:42-49). */
 int gpart_release_cache(gpart_ctx* ctx);
+/* Allocator state: cached (free) blocks and bytes, live bytes, and for the last
+ * release the number of slabs freed and host seconds spent returning them, waiting for
+ * the stream and trimming the pool (release_seconds[3]). */
+int gpart_cache_stats(gpart_ctx* ctx, int64_t* cached_blocks, uint64_t* cached_bytes, uint64_t* live_bytes,
+                      int64_t* released_blocks, double* release_seconds);
 int gpart_memcpy_htod(gpart_ctx* ctx, void* dst, const void* src, size_t bytes); /* host out: sync */
 int gpart_memcpy_dtoh(gpart_ctx* ctx, void* dst, const void* src, size_t bytes); /* host out: sync */
 int gpart_memcpy_dtod(gpart_ctx* ctx, void* dst, const void* src, size_t bytes);
@@ -77,7 +82,8 @@ int gpart_widen_weights(gpart_ctx* ctx, const void* weights, int width, int64_t 
 #define GPART_PROFILE_REFINE_PROPOSALS 3
 #define GPART_PROFILE_REFINE_FILTER 4
 #define GPART_PROFILE_SPECTRAL 5
-#define GPART_PROFILE_FAMILIES 6
+#define GPART_PROFILE_REBALANCE 6
+#define GPART_PROFILE_FAMILIES 7
 int gpart_profile_enable(gpart_ctx* ctx, int family, int enable);
 /* host out: total milliseconds, launch count, algorithmic bytes */
 int gpart_profile_read(gpart_ctx* ctx, int family, double* total_ms, int64_t* launches,
@@ -160,12 +166,13 @@ int gpart_sclp_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t*
  * admission radix sort, decode, inclusive_scan_by_key and sclp_commit_kernel
  * (:600-680, :1086-1121, :1329-1373). gain_bound bounds every gain (row weight
  * bound). Updates clusters and cluster_weights.
- * host out: *valid (proposals), *accepted, *nonempty (cluster count after). */
+ * host out: *valid (proposals), *accepted, *nonempty (cluster count after),
+ * *predicted_gain (sum of accepted gains; may be NULL). */
 int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals,
                      const uint64_t* gains, const uint32_t* ties,
                      const uint64_t* vertex_weights, uint64_t capacity, uint64_t gain_bound,
                      int32_t* clusters, uint64_t* cluster_weights, int64_t* valid,
-                     int64_t* accepted, int64_t* nonempty);
+                     int64_t* accepted, int64_t* nonempty, uint64_t* predicted_gain);
 
 /* Isolated-vertex packing (:1410-1441, kernels :741-773).
  * host out: *isolated (zero-degree count), *nonempty. */
@@ -263,27 +270,35 @@ int gpart_project(gpart_ctx* ctx, int64_t n, const int32_t* map, const int32_t* 
 int gpart_refine_build_cache(gpart_ctx* ctx, const gpart_csr_view* g,
                              const gpart_degree_classes* classes, const int32_t* labels, int k,
                              uint64_t* cache);
-/* incremental_lp_proposals (:27-46) + incremental_priority_filter (:48-67) +
- * compaction of movers (:276-279). host out: *count. */
-int gpart_refine_lp_phase(gpart_ctx* ctx, const gpart_csr_view* g,
-                          const gpart_degree_classes* classes, const uint64_t* cache,
-                          const int32_t* labels, int k, double filter, int32_t* locks,
-                          int32_t* targets, int64_t* gains, int32_t* flags, int32_t* movers,
-                          int64_t* count);
-/* incremental_update_cache_and_cut (:69-95) + incremental_commit (:97-109).
- * host out: *delta (exact cut decrease), part_weights_host (k entries). */
-int gpart_refine_apply(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* movers,
-                       int64_t count, int32_t* labels, const int32_t* targets,
-                       const int32_t* flags, int k, uint64_t* cache, uint64_t* part_weights,
-                       int32_t* locks, int lp, int64_t* delta, uint64_t* part_weights_host);
-/* One rebalance selection: incremental_rebalance_proposals (:111-157) and the
- * bucket quota/admission kernels (:167-232), ranking bucket members in vertex-id
- * order. Writes targets and flags, compacts movers. host out: *count. */
-int gpart_refine_rebalance_select(gpart_ctx* ctx, const gpart_csr_view* g,
-                                  const uint64_t* cache, const int32_t* labels, int k,
-                                  const uint64_t* part_weights, uint64_t capacity,
-                                  int32_t* targets, int32_t* flags, int32_t* movers,
-                                  int64_t* count);
+/* Outcome of one refinement round, read back at its end. */
+typedef struct gpart_refine_status {
+    int64_t movers;              /* vertices moved (0: nothing was applied) */
+    int64_t cut_delta;           /* exact cut decrease; negative when the cut grew */
+    int64_t candidates;          /* rebalance: vertices of overloaded parts */
+    uint64_t max_part_weight;    /* after the round */
+    uint64_t part_weights[32];   /* after the round; first k valid */
+} gpart_refine_status;
+
+/* One LP round of run_incremental_refinement (src/refine_incremental.cuh:338-352):
+ * incremental_lp_proposals (:27-46), incremental_priority_filter (:48-67),
+ * mover compaction (:276-279), incremental_update_cache_and_cut (:69-95) and
+ * incremental_commit (:97-109) with locks set. cache, labels, locks, targets,
+ * gains, flags and part_weights are device arrays the round updates in place;
+ * one host synchronisation. host out: *status. */
+int gpart_refine_lp_round(gpart_ctx* ctx, const gpart_csr_view* g,
+                          const gpart_degree_classes* classes, uint64_t* cache, int32_t* labels,
+                          int k, double filter, int32_t* locks, int32_t* targets, int64_t* gains,
+                          int32_t* flags, uint64_t* part_weights, gpart_refine_status* status);
+/* One iteration of the rebalance loop (src/refine_incremental.cuh:303-331):
+ * incremental_rebalance_proposals (:111-157), the source and destination bucket
+ * quotas (:167-232) and the same apply without locks. Members of a bucket are
+ * ranked by the weight of its lower-id members, one legal outcome of the
+ * reference's atomicAdd ranking (:185,216). One host synchronisation.
+ * host out: *status (movers == 0: nothing could move). */
+int gpart_refine_rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g,
+                                 const gpart_degree_classes* classes, uint64_t* cache,
+                                 int32_t* labels, int k, uint64_t* part_weights, uint64_t capacity,
+                                 int32_t* targets, int32_t* flags, gpart_refine_status* status);
 /* Largest edge weight (src/refine_incremental.cuh:332). host out. */
 int gpart_max_edge_weight(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* maximum);
 /* Edge weight sums. host out: *checked saturates at UINT64_MAX (CheckedSum,

@@ -190,3 +190,21 @@ def test_sclp_small_hierarchy(hx, oracle):
             assert np.array_equal(x, y), i
         if i + 1 < len(gh):
             assert np.array_equal(gh.map(i), oh.map(i))
+
+
+@pytest.mark.parametrize("verify", [False, True])
+def test_edgeless_coarse_level(hx, oracle, verify):
+    """A perfect matching contracts to levels without any cross entries (every
+    edge internal); the hierarchy still equals the oracle's (regression: the
+    empty coarse CSR once released its blocks twice)."""
+    say(f"case kat edgeless coarse level verify={verify}")
+    g = K.pairs(4096)
+    gh = upload(hx, g).coarsen(4, 0, 1.0, 24, verify=verify)
+    og = oracle.Graph(g["indptr"], g["indices"].astype(np.int32), g["edge_weights"], g["vertex_weights"])
+    oh = oracle.coarsen(og, 4, 0, 1.0, 24)
+    assert len(gh) == len(oh) and any(len(gh.level(i)[1]) == 0 for i in range(len(gh)))
+    for i in range(len(gh)):
+        for x, y in zip(gh.level(i), oh.level(i).arrays()):
+            assert np.array_equal(x, y), i
+        if i + 1 < len(gh):
+            assert np.array_equal(gh.map(i), oh.map(i))
