@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -141,7 +142,7 @@ def recorded_traffic(workload: str, family: str):
     if not path.exists() or family not in FAMILY_KERNELS:
         return None
     rec = json.loads(path.read_text())
-    if FAMILY_KERNELS[family] not in rec.get("kernel", ""):
+    if not re.search(rec.get("kernel", "^$"), FAMILY_KERNELS[family]):
         return None
     return rec["traffic_bytes_per_dispatch"]
 
