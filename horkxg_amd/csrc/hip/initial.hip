@@ -13,11 +13,13 @@
 // Multiply-adds are fused only where written with fma() (the SpMV accumulation
 // and the variance); the library is built with -ffp-contract=off.
 //
-// Plain form: the reference's launch structure, one task at a time.
-// Batched form: all tasks of one recursion depth in one launch, one 1024-thread
-// workgroup per task: start, 64 diffusion steps, the (x, id) sort (LDS bitonic),
-// the weight prefix, the feasible split and the label assignment, emitting the
-// child tasks for the next depth.
+// Plain form: the reference's launch structure, one task at a time (start, then
+// per step: SpMV, finish_moments, normalize).
+// Batched form: all tasks of one recursion depth share every launch. A depth is
+// 66 launches whatever its task count: start, 64 steps (each also finishing the
+// previous step's moments and normalising on read), and one 1024-thread
+// workgroup per task for the (x, id) sort (LDS bitonic), the weight prefix, the
+// feasible split and the label assignment, emitting the child tasks.
 #include "internal.hpp"
 #include "scan.hpp"
 
@@ -247,7 +249,6 @@ struct VecTraits<float> {
 };
 
 struct BatchShared {
-    Moments partials[kMaxBatchedTask / 8];
     std::uint64_t keys[kMaxBatchedTask];
     int ids[kMaxBatchedTask];
     double shared8[24];
@@ -264,143 +265,154 @@ __device__ __forceinline__ std::uint64_t split_target(std::uint64_t weight, int 
     return q * static_cast<std::uint64_t>(lp) + (r * static_cast<std::uint64_t>(lp)) / static_cast<std::uint64_t>(parts);
 }
 
+// All tasks of one depth share each launch: the grid is the concatenation of the
+// tasks' 8-vertex blocks (the reference's own mapping), so every task, small or
+// large, is spread over the whole GPU. One step is one launch: each workgroup
+// first recomputes its task's normalisation from the previous step's block
+// partials (finish_block, the reference's fixed order) and normalises x values as
+// it reads them, (x - mean) * scale, the same expression the reference stores.
+constexpr int kMaxDepthTasks = 32;
+
+struct DepthTasks {
+    int count;
+    int first_block[kMaxDepthTasks + 1];  // prefix of ceil(task.count / 8)
+    gpart_bisect_task task[kMaxDepthTasks];
+};
+
+__device__ __forceinline__ int task_of_block(const DepthTasks& dt, int block) {
+    int t = 0;
+    while (t + 1 < dt.count && dt.first_block[t + 1] <= block) ++t;
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ T normalized(const T* x, int v, const Norm& norm) {
+    return static_cast<T>((static_cast<double>(x[v]) - norm.mean) * norm.scale);
+}
+
+// spectral_start (:81-107) for every task of the depth.
 template <typename O, typename W, typename T>
-__global__ void __launch_bounds__(kBatchThreads) spectral_depth_kernel(
-    const gpart_bisect_task* tasks, int task_count, const O* offsets, const int* neighbors, const W* weights,
-    int* labels, int* order, const std::uint64_t* vertex_weights, unsigned seed, std::uint64_t capacity,
-    double* degree, T* xa, T* xb, Moments* vm, double* coords, gpart_bisect_task* next, int* next_count,
-    int* status) {
-    __shared__ BatchShared sh;
-    const int task_index = blockIdx.x;
-    if (task_index >= task_count) return;  // uniform per block
-    const gpart_bisect_task task = tasks[task_index];
-    const int count = task.count;
-    int* ids = order + task.begin;
-    Moments* tvm = vm + task.begin;
+__global__ void __launch_bounds__(256) depth_start_kernel(DepthTasks dt, const int* order, const int* labels,
+                                                          const O* offsets, const int* neighbors, const W* weights,
+                                                          unsigned seed, double* degree, T* x, Moments* partials) {
+    __shared__ Moments vm[8];
+    const int t = task_of_block(dt, blockIdx.x);
+    const gpart_bisect_task task = dt.task[t];
+    const int* ids = order + task.begin;
     const int group = task.first_part;
     const unsigned tseed = seed ^ static_cast<unsigned>(task.first_part);
     const int lane = threadIdx.x & 31, hw = threadIdx.x >> 5;
-    constexpr int kHalves = kBatchThreads / kHalf;
-    const int blocks8 = (count + 7) / 8;
-    T* x = xa;
-    T* y = xb;
-
-    // spectral_start
-    for (int base = 0; base < count; base += kHalves) {
-        const int i = base + hw;
-        double d = 0;
-        if (i < count) {
-            const int v = ids[i];
-            for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32)
-                if (labels[neighbors[e]] == group) d += static_cast<double>(weights[e]);
-        }
-        d = tree32(d);
-        if (lane == 0 && i < count) {
+    const int i = (blockIdx.x - dt.first_block[t]) * 8 + hw;
+    double d = 0;
+    if (i < task.count) {
+        const int v = ids[i];
+        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32)
+            if (labels[neighbors[e]] == group) d += static_cast<double>(weights[e]);
+    }
+    d = tree32(d);
+    if (lane == 0) {
+        Moments m{0, 0, 0};
+        if (i < task.count) {
             const int v = ids[i];
             const double a = static_cast<double>(mix32(static_cast<unsigned>(v) ^ tseed)) / 4294967296.0 - 0.5;
             degree[v] = d;
             x[v] = static_cast<T>(a);
             const double av = static_cast<double>(static_cast<T>(a));
             const double mass = d > 0 ? d : 1.0;
-            tvm[i] = Moments{mass, mass * av, mass * av * av};
+            m = Moments{mass, mass * av, mass * av * av};
+        }
+        vm[hw] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = block_partial(vm, 0, 8);
+}
+
+// finish_moments (:65-79) of the previous step, normalize (:138-142) on read, and
+// spectral_step (:109-136), for every task of the depth.
+template <typename O, typename W, typename T>
+__global__ void __launch_bounds__(256) depth_step_kernel(DepthTasks dt, const int* order, const int* labels,
+                                                         const O* offsets, const int* neighbors, const W* weights,
+                                                         const double* degree, const T* x, T* y,
+                                                         const Moments* previous, Moments* partials) {
+    __shared__ Moments vm[8];
+    __shared__ double shared8[24];
+    __shared__ Norm shared_norm;
+    const int t = task_of_block(dt, blockIdx.x);
+    const gpart_bisect_task task = dt.task[t];
+    const int first = dt.first_block[t];
+    const Norm norm = finish_block(previous + first, dt.first_block[t + 1] - first, shared8, &shared_norm);
+    const int* ids = order + task.begin;
+    const int group = task.first_part;
+    const int lane = threadIdx.x & 31, hw = threadIdx.x >> 5;
+    const int i = (blockIdx.x - first) * 8 + hw;
+    T sum = 0;
+    if (i < task.count) {
+        const int v = ids[i];
+        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + lane; e < offsets[v + 1]; e += 32) {
+            const int u = neighbors[e];
+            if (labels[u] == group) sum = VecTraits<T>::fmadd(static_cast<T>(weights[e]), normalized(x, u, norm), sum);
         }
     }
-    __threadfence_block();
-    __syncthreads();
-    for (int b = threadIdx.x; b < blocks8; b += kBatchThreads) sh.partials[b] = block_partial(tvm, b * 8, count);
-    __syncthreads();
-    Norm norm = finish_block(sh.partials, blocks8, sh.shared8, &sh.norm);
-    for (int iteration = 0; iteration <= 64; ++iteration) {
-        for (int i = threadIdx.x; i < count; i += kBatchThreads) {
+    double sd;
+    if constexpr (sizeof(T) == 8) {
+        sd = tree32(static_cast<double>(sum));
+    } else {
+        float f = sum;
+#pragma unroll
+        for (int s2 = 16; s2; s2 >>= 1) f += __shfl_down(f, s2, kHalf);
+        sd = static_cast<double>(f);
+    }
+    if (lane == 0) {
+        Moments m{0, 0, 0};
+        if (i < task.count) {
             const int v = ids[i];
-            x[v] = static_cast<T>((static_cast<double>(x[v]) - norm.mean) * norm.scale);
+            const double d = degree[v];
+            const T xv = normalized(x, v, norm);
+            double a;
+            if constexpr (sizeof(T) == 8) {
+                a = d > 0 ? 0.5 * (static_cast<double>(xv) + sd / d) : static_cast<double>(xv);
+            } else {
+                const float df = static_cast<float>(d);
+                const float af = d > 0 ? 0.5f * (xv + static_cast<float>(sd) / df) : xv;
+                a = static_cast<double>(af);
+            }
+            y[v] = static_cast<T>(a);
+            const double mass = d > 0 ? d : 1.0;
+            m = Moments{mass, mass * a, mass * a * a};
         }
-        __threadfence_block();
-        __syncthreads();
-        if (iteration == 64) break;
-        // Each half-wave carries kRowsPerPass vertices per pass so their gather
-        // chains (offsets -> neighbours/weights -> labels/x) are in flight together;
-        // every lane still accumulates its entries e = begin + lane + 32j in
-        // increasing e, so the sums are unchanged.
-        for (int base = 0; base < count; base += kHalves * kRowsPerPass) {
-            int rows[kRowsPerPass];
-            std::int64_t first[kRowsPerPass], last[kRowsPerPass];
-            T sum[kRowsPerPass];
-#pragma unroll
-            for (int r = 0; r < kRowsPerPass; ++r) {
-                const int i = base + r * kHalves + hw;
-                rows[r] = i < count ? ids[i] : -1;
-                first[r] = rows[r] >= 0 ? static_cast<std::int64_t>(offsets[rows[r]]) : 0;
-                last[r] = rows[r] >= 0 ? static_cast<std::int64_t>(offsets[rows[r] + 1]) : 0;
-                sum[r] = 0;
-            }
-            int nbr[kRowsPerPass];
-            T w[kRowsPerPass];
-#pragma unroll
-            for (int r = 0; r < kRowsPerPass; ++r) {
-                const std::int64_t e = first[r] + lane;
-                nbr[r] = e < last[r] ? neighbors[e] : -1;
-                w[r] = e < last[r] ? static_cast<T>(weights[e]) : T(0);
-            }
-#pragma unroll
-            for (int r = 0; r < kRowsPerPass; ++r) {
-                const int u = nbr[r];
-                const int lab = u >= 0 ? labels[u] : -1;
-                const T xu = u >= 0 ? x[u] : T(0);
-                if (lab == group) sum[r] = VecTraits<T>::fmadd(w[r], xu, sum[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < kRowsPerPass; ++r)
-                for (std::int64_t e = first[r] + lane + 32; e < last[r]; e += 32) {
-                    const int u = neighbors[e];
-                    if (labels[u] == group) sum[r] = VecTraits<T>::fmadd(static_cast<T>(weights[e]), x[u], sum[r]);
-                }
-#pragma unroll
-            for (int r = 0; r < kRowsPerPass; ++r) {
-                double sd;
-                if constexpr (sizeof(T) == 8) {
-                    sd = tree32(static_cast<double>(sum[r]));
-                } else {
-                    float s = sum[r];
-#pragma unroll
-                    for (int s2 = 16; s2; s2 >>= 1) s += __shfl_down(s, s2, kHalf);
-                    sd = static_cast<double>(s);
-                }
-                if (lane == 0 && rows[r] >= 0) {
-                    const int v = rows[r];
-                    const double d = degree[v];
-                    double a;
-                    if constexpr (sizeof(T) == 8) {
-                        a = d > 0 ? 0.5 * (static_cast<double>(x[v]) + sd / d) : static_cast<double>(x[v]);
-                    } else {
-                        const float df = static_cast<float>(d);
-                        const float af = d > 0 ? 0.5f * (x[v] + static_cast<float>(sd) / df) : x[v];
-                        a = static_cast<double>(af);
-                    }
-                    y[v] = static_cast<T>(a);
-                    const double mass = d > 0 ? d : 1.0;
-                    tvm[base + r * kHalves + hw] = Moments{mass, mass * a, mass * a * a};
-                }
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-        for (int b = threadIdx.x; b < blocks8; b += kBatchThreads) sh.partials[b] = block_partial(tvm, b * 8, count);
-        __syncthreads();
-        norm = finish_block(sh.partials, blocks8, sh.shared8, &sh.norm);
-        T* t = x;
-        x = y;
-        y = t;
+        vm[hw] = m;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = block_partial(vm, 0, 8);
+}
+
+// The last normalisation, the (x, id) sort (:143-148, :230), the weight prefix and
+// feasible split (:231-248) and the label assignment (:149-153): one workgroup per
+// task, emitting the child tasks of the next depth.
+template <typename T>
+__global__ void __launch_bounds__(kBatchThreads) depth_finish_kernel(DepthTasks dt, int* labels, int* order,
+                                                                     const std::uint64_t* vertex_weights,
+                                                                     std::uint64_t capacity, const T* x,
+                                                                     const Moments* partials, double* coords,
+                                                                     gpart_bisect_task* next, int* next_count,
+                                                                     int* status) {
+    __shared__ BatchShared sh;
+    const int t = blockIdx.x;
+    const gpart_bisect_task task = dt.task[t];
+    const int count = task.count;
+    int* ids = order + task.begin;
+    const int first = dt.first_block[t];
+    const Norm norm = finish_block(partials + first, dt.first_block[t + 1] - first, sh.shared8, &sh.norm);
     // (x, id) sort: bitonic over the next power of two, padding keys sort last
     int size = 1;
     while (size < count) size <<= 1;
     for (int i = threadIdx.x; i < size; i += kBatchThreads) {
         if (i < count) {
             const int v = ids[i];
-            sh.keys[i] = ordered_bits(x[v]);
+            const T xv = normalized(x, v, norm);
+            sh.keys[i] = ordered_bits(xv);
             sh.ids[i] = v;
-            if (coords) coords[v] = static_cast<double>(x[v]);
+            if (coords) coords[v] = static_cast<double>(xv);
         } else {
             sh.keys[i] = ~0ULL;
             sh.ids[i] = 0x7fffffff;
@@ -594,32 +606,54 @@ int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* label
         *next_count = 0;
         *status = 0;
         if (task_count <= 0) return;
+        if (task_count > kMaxDepthTasks) throw Error(GPART_ERR_INVALID_ARGUMENT, "too many tasks in one depth");
+        DepthTasks dt{};
+        dt.count = task_count;
+        dt.first_block[0] = 0;
+        for (int t = 0; t < task_count; ++t) {
+            if (tasks[t].count < 1 || tasks[t].count > kMaxBatchedTask)
+                throw Error(GPART_ERR_INVALID_ARGUMENT, "batched bisection task size out of range");
+            dt.task[t] = tasks[t];
+            dt.first_block[t + 1] = dt.first_block[t] + (tasks[t].count + 7) / 8;
+        }
+        const int blocks = dt.first_block[task_count];
         const std::int64_t n = g->n;
         Scratch<double> degree(ctx, n), xa(ctx, n), xb(ctx, n);
-        Scratch<Moments> vm(ctx, n);
+        Scratch<Moments> pa(ctx, blocks), pb(ctx, blocks);
         Scratch<int> counters(ctx, 2);
         fill_bytes(ctx, counters.get(), 0, 8);
         profile_begin(ctx, GPART_PROFILE_SPECTRAL);
-        auto launch = [&](auto o_tag, auto w_tag) {
+        auto launch = [&](auto o_tag, auto w_tag, auto t_tag) {
             using O = decltype(o_tag);
             using W = decltype(w_tag);
+            using T = decltype(t_tag);
             const auto* off = static_cast<const O*>(g->offsets);
             const auto* w = static_cast<const W*>(g->edge_weights);
-            if (precision == GPART_SPECTRAL_FP32)
-                hipLaunchKernelGGL((spectral_depth_kernel<O, W, float>), dim3(task_count), dim3(kBatchThreads), 0,
-                                   ctx->stream, tasks, task_count, off, g->neighbors, w, labels, order,
-                                   vertex_weights, seed, capacity, degree.get(), reinterpret_cast<float*>(xa.get()),
-                                   reinterpret_cast<float*>(xb.get()), vm.get(), coords, next, counters.get(),
-                                   counters.get() + 1);
-            else
-                hipLaunchKernelGGL((spectral_depth_kernel<O, W, double>), dim3(task_count), dim3(kBatchThreads), 0,
-                                   ctx->stream, tasks, task_count, off, g->neighbors, w, labels, order,
-                                   vertex_weights, seed, capacity, degree.get(), xa.get(), xb.get(), vm.get(), coords,
-                                   next, counters.get(), counters.get() + 1);
+            T* x = reinterpret_cast<T*>(xa.get());
+            T* y = reinterpret_cast<T*>(xb.get());
+            Moments* p = pa.get();
+            Moments* q = pb.get();
+            hipLaunchKernelGGL((depth_start_kernel<O, W, T>), dim3(blocks), dim3(256), 0, ctx->stream, dt, order,
+                               labels, off, g->neighbors, w, seed, degree.get(), x, p);
+            for (int step = 0; step < 64; ++step) {
+                hipLaunchKernelGGL((depth_step_kernel<O, W, T>), dim3(blocks), dim3(256), 0, ctx->stream, dt, order,
+                                   labels, off, g->neighbors, w, degree.get(), x, y, p, q);
+                std::swap(x, y);
+                std::swap(p, q);
+            }
+            hipLaunchKernelGGL((depth_finish_kernel<T>), dim3(task_count), dim3(kBatchThreads), 0, ctx->stream, dt,
+                               labels, order, vertex_weights, capacity, x, p, coords, next, counters.get(),
+                               counters.get() + 1);
             GPART_LAUNCH_CHECK();
         };
-        if (g->width == GPART_WIDTH_STANDARD) launch(std::int32_t{}, std::uint32_t{});
-        else launch(std::int64_t{}, std::uint64_t{});
+        const bool fp32 = precision == GPART_SPECTRAL_FP32;
+        if (g->width == GPART_WIDTH_STANDARD) {
+            if (fp32) launch(std::int32_t{}, std::uint32_t{}, float{});
+            else launch(std::int32_t{}, std::uint32_t{}, double{});
+        } else {
+            if (fp32) launch(std::int64_t{}, std::uint64_t{}, float{});
+            else launch(std::int64_t{}, std::uint64_t{}, double{});
+        }
         profile_end(ctx, GPART_PROFILE_SPECTRAL, 0.0);
         int h[2];
         read_back(ctx, h, counters.get(), sizeof(h));

@@ -80,11 +80,11 @@ __global__ void project_kernel(std::int64_t n, const std::int32_t* map, const st
 
 // ---------------------------------------------------------------- cache
 
-template <typename O, typename W, int G>
+template <typename O, typename W, int G, typename C>
 __global__ void __launch_bounds__(kBlock) build_cache_kernel(std::int64_t count, const std::int32_t* list,
                                                              const O* offsets, const std::int32_t* neighbors,
                                                              const W* weights, const std::int32_t* labels, int k,
-                                                             unsigned long long* cache) {
+                                                             C* cache) {
     constexpr int kGroups = kBlock / G;
     __shared__ unsigned long long rows[kGroups][32];
     const int local_group = threadIdx.x / G;
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) build_cache_kernel(std::int64_t count,
     }
     __syncthreads();
     if (group < count)
-        for (int p = sub; p < k; p += G) cache[static_cast<std::int64_t>(v) * k + p] = rows[local_group][p];
+        for (int p = sub; p < k; p += G) cache[static_cast<std::int64_t>(v) * k + p] = static_cast<C>(rows[local_group][p]);
 }
 
 // Rows of the n x k cache are read by groups of G lanes, lane j owning the
@@ -112,8 +112,8 @@ constexpr int kRowSpan = 8;
 
 // incremental_lp_proposals (:27-46): argmax over p != source of cache > 0,
 // lowest index on ties; accepted if best >= current or within the filter.
-template <int G>
-__global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, const unsigned long long* cache,
+template <int G, typename C>
+__global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, const C* cache,
                                                               const std::int32_t* labels, int k, double filter,
                                                               std::int32_t* locks, std::int32_t* targets,
                                                               long long* gains) {
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, co
     const int j = threadIdx.x & (G - 1);
     if (v >= n) return;  // uniform per group
     const std::int32_t source = labels[v];
-    const unsigned long long* row = cache + v * k;
+    const C* row = cache + v * k;
     unsigned long long best = 0, current = 0;
     int best_p = 0x7fffffff;
 #pragma unroll
@@ -225,13 +225,13 @@ constexpr int kHeavyDegree = 64;
 constexpr int kRoundBlocks = 1024;  // cap of grid-stride launches whose count lives on the device
 
 // incremental_update_cache_and_cut (:69-95) over a mover list whose length is on the device.
-template <typename O, typename W, int G>
+template <typename O, typename W, int G, typename C>
 __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* count_ptr, const std::int32_t* movers,
                                                               const O* offsets, const std::int32_t* neighbors,
                                                               const W* weights, const std::int32_t* labels,
                                                               const std::int32_t* targets,
                                                               const std::int32_t* flags, int k,
-                                                              unsigned long long* cache, long long* delta) {
+                                                              C* cache, long long* delta) {
     __shared__ long long block_sum[kBlock / kWave];
     const long long count = *count_ptr;
     const int sub = threadIdx.x & (G - 1);
@@ -246,8 +246,9 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* c
             const std::int32_t u = neighbors[e];
             if (u == v) continue;
             const auto w = static_cast<unsigned long long>(weights[e]);
-            atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, 0ULL - w);
-            atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, w);
+            // modular arithmetic: entries fit C at every point of the round
+            atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
+            atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
             const bool fu = flags[u] != 0;
             if (!fu || u > v) {
                 const std::int32_t lu = labels[u];
@@ -526,9 +527,9 @@ struct CandidateStore {
 
 // incremental_rebalance_proposals (:111-157) + balance_source_histogram (:174-186).
 // One group of G lanes per candidate, lane j owning parts [8j, 8j + 8).
-template <typename W, int G>
+template <typename W, int G, typename C>
 __global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundState* st, const std::int32_t* cand,
-                                                                   const unsigned long long* cache,
+                                                                   const C* cache,
                                                                    const std::int32_t* labels, const W* vertex_weights,
                                                                    int k, const unsigned long long* part_weights,
                                                                    std::uint64_t capacity, PowerTable powers,
@@ -548,7 +549,7 @@ __global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundSt
         const std::int64_t v = cand[index];
         const std::int32_t source = labels[v];
         const std::uint64_t weight = static_cast<std::uint64_t>(vertex_weights[v]);
-        const unsigned long long* row = cache + v * k;
+        const C* row = cache + v * k;
         unsigned long long slack[kRowSpan];
         unsigned long long conn = 0, best_slack = 0, current = 0;
         int bp = 0x7fffffff;
@@ -920,7 +921,7 @@ PowerTable make_power_table() {
     return t;
 }
 
-template <typename O, typename W>
+template <typename O, typename W, typename C = unsigned long long>
 struct Ops {
     static void cut(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_t* labels, std::uint64_t* out) {
         Scratch<unsigned long long> d(ctx, 1);
@@ -943,24 +944,24 @@ struct Ops {
     }
 
     static void build_cache(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
-                            const std::int32_t* labels, int k, unsigned long long* cache) {
+                            const std::int32_t* labels, int k, C* cache) {
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
         profile_begin(ctx, GPART_PROFILE_REFINE_CACHE);
         if (c->low_count)
-            hipLaunchKernelGGL((build_cache_kernel<O, W, 8>), dim3(grid_for(c->low_count * 8)), dim3(kBlock), 0,
+            hipLaunchKernelGGL((build_cache_kernel<O, W, 8, C>), dim3(grid_for(c->low_count * 8)), dim3(kBlock), 0,
                                ctx->stream, c->low_count, c->low, off, g->neighbors, w, labels, k, cache);
         if (c->medium_count)
-            hipLaunchKernelGGL((build_cache_kernel<O, W, 64>), dim3(grid_for(c->medium_count * 64)), dim3(kBlock), 0,
+            hipLaunchKernelGGL((build_cache_kernel<O, W, 64, C>), dim3(grid_for(c->medium_count * 64)), dim3(kBlock), 0,
                                ctx->stream, c->medium_count, c->medium, off, g->neighbors, w, labels, k, cache);
         if (c->high_count)
-            hipLaunchKernelGGL((build_cache_kernel<O, W, kBlock>), dim3(static_cast<unsigned>(c->high_count)),
+            hipLaunchKernelGGL((build_cache_kernel<O, W, kBlock, C>), dim3(static_cast<unsigned>(c->high_count)),
                                dim3(kBlock), 0, ctx->stream, c->high_count, c->high, off, g->neighbors, w, labels, k,
                                cache);
         GPART_LAUNCH_CHECK();
         const double n = static_cast<double>(g->n), m = static_cast<double>(g->m);
         profile_end(ctx, GPART_PROFILE_REFINE_CACHE,
-                    (n + 1) * sizeof(O) + m * (4 + sizeof(W)) + 4 * m + 8.0 * n * k);
+                    (n + 1) * sizeof(O) + m * (4 + sizeof(W)) + 4 * m + static_cast<double>(sizeof(C)) * n * k);
     }
 
     static void filter_movers(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
@@ -988,17 +989,17 @@ struct Ops {
     // incremental_update_cache_and_cut + incremental_commit over the round's movers.
     static void apply(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c, RoundState* st,
                       const std::int32_t* light, const std::int32_t* heavy, std::int32_t* labels,
-                      const std::int32_t* targets, const std::int32_t* flags, int k, unsigned long long* cache,
+                      const std::int32_t* targets, const std::int32_t* flags, int k, C* cache,
                       unsigned long long* part_weights, std::int32_t* locks, bool lp) {
         const std::int64_t n = g->n;
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
         const auto light_blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * 8), 4 * kRoundBlocks));
-        hipLaunchKernelGGL((update_cache_kernel<O, W, 8>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL((update_cache_kernel<O, W, 8, C>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
                            &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta);
         const std::int64_t heavy_bound = c->medium_count + c->high_count;
         if (heavy_bound > 0)
-            hipLaunchKernelGGL((update_cache_kernel<O, W, kBlock>),
+            hipLaunchKernelGGL((update_cache_kernel<O, W, kBlock, C>),
                                dim3(static_cast<unsigned>(std::min<std::int64_t>(heavy_bound, kRoundBlocks))),
                                dim3(kBlock), 0, ctx->stream, &st->heavy, heavy, off, g->neighbors, w, labels, targets,
                                flags, k, cache, &st->delta);
@@ -1024,7 +1025,7 @@ struct Ops {
 
     // One LP round: proposals, priority filter, movers, apply; one synchronisation.
     static void lp_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
-                         unsigned long long* cache, std::int32_t* labels, int k, double filter,
+                         C* cache, std::int32_t* labels, int k, double filter,
                          std::int32_t* locks, std::int32_t* targets, long long* gains, std::int32_t* flags,
                          unsigned long long* part_weights, gpart_refine_status* out) {
         const std::int64_t n = g->n;
@@ -1035,12 +1036,12 @@ struct Ops {
         profile_begin(ctx, GPART_PROFILE_REFINE_PROPOSALS);
         const unsigned blocks = grid_for(n * G);
         switch (G) {
-            case 1: hipLaunchKernelGGL(lp_proposals_kernel<1>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            case 2: hipLaunchKernelGGL(lp_proposals_kernel<2>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            default: hipLaunchKernelGGL(lp_proposals_kernel<4>, dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
+            case 1: hipLaunchKernelGGL((lp_proposals_kernel<1, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
+            case 2: hipLaunchKernelGGL((lp_proposals_kernel<2, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
+            default: hipLaunchKernelGGL((lp_proposals_kernel<4, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
         }
         GPART_LAUNCH_CHECK();
-        profile_end(ctx, GPART_PROFILE_REFINE_PROPOSALS, static_cast<double>(n) * (20.0 + 8.0 * k));
+        profile_end(ctx, GPART_PROFILE_REFINE_PROPOSALS, static_cast<double>(n) * (20.0 + sizeof(C) * k));
         filter_movers(ctx, g, c, labels, targets, gains, flags);
         Scratch<std::int32_t> light(ctx, n), heavy(ctx, n);
         device_scan<MoverPair, MoverPairOp>(ctx, MoverLoad<O>{flags, off}, MoverStore{light.get(), heavy.get(), n,
@@ -1054,7 +1055,7 @@ struct Ops {
     // One rebalance iteration: proposals, source and destination bucket quotas,
     // apply; one synchronisation.
     static void rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
-                                unsigned long long* cache, std::int32_t* labels, int k,
+                                C* cache, std::int32_t* labels, int k,
                                 unsigned long long* part_weights, std::uint64_t capacity, std::int32_t* targets,
                                 std::int32_t* flags, gpart_refine_status* out) {
         const std::int64_t n = g->n;
@@ -1087,7 +1088,7 @@ struct Ops {
         const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), 2 * kRoundBlocks));
         profile_begin(ctx, GPART_PROFILE_REBALANCE);
 #define GPART_BALANCE_PROPOSALS(GG)                                                                              \
-    hipLaunchKernelGGL((balance_proposals_kernel<W, GG>), dim3(blocks), dim3(kBlock), 0, ctx->stream, st.get(),   \
+    hipLaunchKernelGGL((balance_proposals_kernel<W, GG, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, st.get(),   \
                        cand.get(), cache, labels, vw, k, part_weights, capacity, powers, targets, cbucket.get(), \
                        src_replicas)
         switch (G) {
@@ -1124,7 +1125,7 @@ struct Ops {
         finish(ctx, st.get(), part_weights, k, out);
         // per candidate: id, label, weight, cache row, target and bucket writes
         profile_add_bytes(ctx, GPART_PROFILE_REBALANCE,
-                          static_cast<double>(out->candidates) * (16.0 + sizeof(W) + 8.0 * k));
+                          static_cast<double>(out->candidates) * (16.0 + sizeof(W) + sizeof(C) * k));
     }
 };
 
@@ -1142,6 +1143,29 @@ using namespace gpart_hip;
             using OpsT = Ops<std::int64_t, std::uint64_t>;           \
             call;                                                    \
         }                                                            \
+    } while (0)
+
+// Width and cache entry type (4 or 8 bytes) of the refinement entry points.
+#define GPART_CACHE_DISPATCH(width, bytes, call)                                                      \
+    do {                                                                                              \
+        if ((bytes) != 4 && (bytes) != 8) throw Error(GPART_ERR_INVALID_ARGUMENT, "cache entries are 4 or 8 bytes"); \
+        if ((width) == GPART_WIDTH_STANDARD && (bytes) == 4) {                                        \
+            using OpsT = Ops<std::int32_t, std::uint32_t, unsigned>;                                  \
+            using CacheT = unsigned;                                                                  \
+            call;                                                                                     \
+        } else if ((width) == GPART_WIDTH_STANDARD) {                                                 \
+            using OpsT = Ops<std::int32_t, std::uint32_t, unsigned long long>;                        \
+            using CacheT = unsigned long long;                                                        \
+            call;                                                                                     \
+        } else if ((bytes) == 4) {                                                                    \
+            using OpsT = Ops<std::int64_t, std::uint64_t, unsigned>;                                  \
+            using CacheT = unsigned;                                                                  \
+            call;                                                                                     \
+        } else {                                                                                      \
+            using OpsT = Ops<std::int64_t, std::uint64_t, unsigned long long>;                        \
+            using CacheT = unsigned long long;                                                        \
+            call;                                                                                     \
+        }                                                                                             \
     } while (0)
 
 extern "C" {
@@ -1178,36 +1202,37 @@ int gpart_project(gpart_ctx* ctx, int64_t n, const int32_t* map, const int32_t* 
 }
 
 int gpart_refine_build_cache(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes,
-                             const int32_t* labels, int k, uint64_t* cache) {
+                             const int32_t* labels, int k, void* cache, int cache_bytes) {
     return guarded([&] {
-        GPART_WIDTH_DISPATCH(g->width, OpsT::build_cache(ctx, g, classes, labels, k,
-                                                         reinterpret_cast<unsigned long long*>(cache)));
+        GPART_CACHE_DISPATCH(g->width, cache_bytes,
+                             OpsT::build_cache(ctx, g, classes, labels, k, static_cast<CacheT*>(cache)));
     });
 }
 
 int gpart_refine_lp_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes,
-                          uint64_t* cache, int32_t* labels, int k, double filter, int32_t* locks, int32_t* targets,
-                          int64_t* gains, int32_t* flags, uint64_t* part_weights, gpart_refine_status* status) {
+                          void* cache, int cache_bytes, int32_t* labels, int k, double filter, int32_t* locks,
+                          int32_t* targets, int64_t* gains, int32_t* flags, uint64_t* part_weights,
+                          gpart_refine_status* status) {
     return guarded([&] {
         if (k < 1 || k > 32) throw Error(GPART_ERR_INVALID_ARGUMENT, "part count must be in [1, 32]");
         if (g->n <= 0) throw Error(GPART_ERR_INVALID_ARGUMENT, "refinement round needs vertices");
-        GPART_WIDTH_DISPATCH(g->width,
-                             OpsT::lp_round(ctx, g, classes, reinterpret_cast<unsigned long long*>(cache), labels, k,
-                                            filter, locks, targets, reinterpret_cast<long long*>(gains), flags,
+        GPART_CACHE_DISPATCH(g->width, cache_bytes,
+                             OpsT::lp_round(ctx, g, classes, static_cast<CacheT*>(cache), labels, k, filter, locks,
+                                            targets, reinterpret_cast<long long*>(gains), flags,
                                             reinterpret_cast<unsigned long long*>(part_weights), status));
     });
 }
 
 int gpart_refine_rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes,
-                                 uint64_t* cache, int32_t* labels, int k, uint64_t* part_weights, uint64_t capacity,
-                                 int32_t* targets, int32_t* flags, gpart_refine_status* status) {
+                                 void* cache, int cache_bytes, int32_t* labels, int k, uint64_t* part_weights,
+                                 uint64_t capacity, int32_t* targets, int32_t* flags, gpart_refine_status* status) {
     return guarded([&] {
         if (k < 1 || k > 32) throw Error(GPART_ERR_INVALID_ARGUMENT, "part count must be in [1, 32]");
         if (g->n <= 0) throw Error(GPART_ERR_INVALID_ARGUMENT, "refinement round needs vertices");
-        GPART_WIDTH_DISPATCH(g->width,
-                             OpsT::rebalance_round(ctx, g, classes, reinterpret_cast<unsigned long long*>(cache),
-                                                   labels, k, reinterpret_cast<unsigned long long*>(part_weights),
-                                                   capacity, targets, flags, status));
+        GPART_CACHE_DISPATCH(g->width, cache_bytes,
+                             OpsT::rebalance_round(ctx, g, classes, static_cast<CacheT*>(cache), labels, k,
+                                                   reinterpret_cast<unsigned long long*>(part_weights), capacity,
+                                                   targets, flags, status));
     });
 }
 

@@ -112,14 +112,9 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
         }
     } else {
         const int precision = mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64;
-        // A batched task runs on one workgroup, so its time grows with its size;
-        // the plain form spreads one task over the whole GPU at ~200 launches per
-        // task. Large tasks therefore take the plain form (measured crossover on
-        // MI355X: about a thousand vertices).
-        constexpr std::int64_t kPlainAbove = 1024;
-        const auto max_task = std::min<std::int64_t>(gpart_spectral_depth_max_task(), kPlainAbove);
+        const auto max_task = gpart_spectral_depth_max_task();
         std::vector<gpart_bisect_task> depth{{0, n, 0, parts, total}};
-        DeviceVector<gpart_bisect_task> device_tasks(32), device_next(64);
+        DeviceVector<gpart_bisect_task> device_next(64);
         while (!depth.empty()) {
             std::vector<gpart_bisect_task> batched, next;
             for (const auto& t : depth) {
@@ -134,11 +129,9 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
                 }
             }
             if (!batched.empty()) {
-                GPART_CHECK(gpart_memcpy_htod(ctx, device_tasks.data(), batched.data(),
-                                              batched.size() * sizeof(gpart_bisect_task)));
                 std::int32_t produced = 0, status = 0;
                 GPART_CHECK(gpart_spectral_depth(ctx, &view, labels.data(), order.data(), vertex_weights.data(),
-                                                 device_tasks.data(), static_cast<std::int32_t>(batched.size()), seed,
+                                                 batched.data(), static_cast<std::int32_t>(batched.size()), seed,
                                                  capacity, precision, x.data(), device_next.data(), &produced,
                                                  &status));
                 if (status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");

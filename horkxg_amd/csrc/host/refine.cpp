@@ -85,10 +85,23 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     DeviceVector<std::int32_t> low(N), medium(N), high(N);
     gpart_degree_classes classes{low.data(), medium.data(), high.data(), 0, 0, 0, 0, 0};
     GPART_CHECK(gpart_degree_classes_build(ctx, &view, &classes));
-    DeviceVector<std::uint64_t> cache(N * static_cast<std::size_t>(k));
+    // Cache entries are 4 bytes when the level's total edge weight, a bound on
+    // every row sum, fits 32 bits; otherwise 8.
+    std::uint64_t maximum_edge = 0;
+    GPART_CHECK(gpart_max_edge_weight(ctx, &view, &maximum_edge));
+    int cache_bytes = 8;
+    const auto m = static_cast<std::uint64_t>(level.graph.edges());
+    if (m == 0 || maximum_edge <= 0xffffffffULL / m) {
+        cache_bytes = 4;
+    } else {
+        std::uint64_t checked = 0, limited = 0;
+        GPART_CHECK(gpart_edge_weight_sum(ctx, &view, &checked, &limited));
+        if (checked <= 0xffffffffULL) cache_bytes = 4;
+    }
+    DeviceVector<std::uint8_t> cache(N * static_cast<std::size_t>(k) * static_cast<std::size_t>(cache_bytes));
     DeviceVector<std::int32_t> targets(N), locks(N, 0), flags(N, 0);
     DeviceVector<std::int64_t> gains(N);
-    GPART_CHECK(gpart_refine_build_cache(ctx, &view, &classes, level.partition.data(), k, cache.data()));
+    GPART_CHECK(gpart_refine_build_cache(ctx, &view, &classes, level.partition.data(), k, cache.data(), cache_bytes));
 
     RefineStats stats;
     stats.initial_cut = level.cut();
@@ -112,8 +125,9 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
         level.host_part_weights.assign(status.part_weights, status.part_weights + k);
         if (options.strict_verify) {
             if (level.cut() != current_cut) throw std::runtime_error("incremental cut differs from CSR scan");
-            DeviceVector<std::uint64_t> check(N * static_cast<std::size_t>(k));
-            GPART_CHECK(gpart_refine_build_cache(ctx, &view, &classes, level.partition.data(), k, check.data()));
+            DeviceVector<std::uint8_t> check(cache.size());
+            GPART_CHECK(gpart_refine_build_cache(ctx, &view, &classes, level.partition.data(), k, check.data(),
+                                                 cache_bytes));
             if (check.to_host() != cache.to_host())
                 throw std::runtime_error("incremental affinity cache differs from CSR rebuild");
         }
@@ -122,7 +136,8 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
         for (int iteration = 0; iteration < 2 * k + 4 && level.maximum() > level.capacity; ++iteration) {
             ++stats.admission_iterations;
             gpart_refine_status status{};
-            GPART_CHECK(gpart_refine_rebalance_round(ctx, &view, &classes, cache.data(), level.partition.data(), k,
+            GPART_CHECK(gpart_refine_rebalance_round(ctx, &view, &classes, cache.data(), cache_bytes,
+                                                     level.partition.data(), k,
                                                      level.part_weights.data(), level.capacity, targets.data(),
                                                      flags.data(), &status));
             if (!status.movers) return false;
@@ -130,14 +145,13 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
         }
         return level.maximum() <= level.capacity;
     };
-    std::uint64_t maximum_edge = 0;
-    GPART_CHECK(gpart_max_edge_weight(ctx, &view, &maximum_edge));
     const double filter = maximum_edge <= 1 ? 0.25 : 0.75;
     int stale = 0, empty = 0;
     for (int phase = 0; phase < rounds && stale < options.stale_rounds; ++phase) {
         stats.rounds = phase + 1;
         gpart_refine_status status{};
-        GPART_CHECK(gpart_refine_lp_round(ctx, &view, &classes, cache.data(), level.partition.data(), k, filter,
+        GPART_CHECK(gpart_refine_lp_round(ctx, &view, &classes, cache.data(), cache_bytes, level.partition.data(),
+                                          k, filter,
                                           locks.data(), targets.data(), gains.data(), flags.data(),
                                           level.part_weights.data(), &status));
         stats.proposals += static_cast<std::uint64_t>(status.movers);

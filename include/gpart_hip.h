@@ -240,12 +240,15 @@ int gpart_spectral_task(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* 
 int gpart_spectral_assign(gpart_ctx* ctx, const int32_t* order, const gpart_bisect_task* task,
                           int32_t boundary, int32_t left_parts, int32_t* labels);
 
-/* Batched form: every task of one recursion depth in one launch, one workgroup
- * per task, vectors resident in LDS/L2; diffusion, (x, id) sort, weight prefix,
- * feasible split and label assignment all on the device. tasks/next are device
- * arrays. precision: GPART_SPECTRAL_FP64 (bit-identical to the plain form) or
- * GPART_SPECTRAL_FP32. host out: *next_count, *status (0 ok, 1 infeasible split,
- * 2 leaf over capacity). */
+/* Batched form: every task of one recursion depth shares each launch (the grid is
+ * the tasks' 8-vertex blocks): start, 64 diffusion steps that also finish the
+ * previous moments and normalise on read, then one workgroup per task for the
+ * (x, id) sort, weight prefix, feasible split and label assignment, all on the
+ * device. tasks: host array (at most 32 tasks, each at most
+ * gpart_spectral_depth_max_task() vertices); next: device array receiving the
+ * child tasks. precision: GPART_SPECTRAL_FP64 (bit-identical to the plain form)
+ * or GPART_SPECTRAL_FP32. host out: *next_count, *status (0 ok, 1 infeasible
+ * split, 2 leaf over capacity). */
 int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* labels,
                          int32_t* order, const uint64_t* vertex_weights,
                          const gpart_bisect_task* tasks, int32_t task_count, uint32_t seed,
@@ -266,10 +269,12 @@ int gpart_part_weights(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* l
 /* project_partition_kernel (src/refine.cu:89-100). */
 int gpart_project(gpart_ctx* ctx, int64_t n, const int32_t* map, const int32_t* coarse_labels,
                   int32_t* fine_labels);
-/* incremental_build_affinity (:7-25): cache[v*k + p]. */
+/* incremental_build_affinity (:7-25): cache[v*k + p], entries of cache_bytes
+ * (8, or 4 when every row's edge-weight sum is below 2^32: the entries and every
+ * value they pass through during a round then fit 32 bits). */
 int gpart_refine_build_cache(gpart_ctx* ctx, const gpart_csr_view* g,
                              const gpart_degree_classes* classes, const int32_t* labels, int k,
-                             uint64_t* cache);
+                             void* cache, int cache_bytes);
 /* Outcome of one refinement round, read back at its end. */
 typedef struct gpart_refine_status {
     int64_t movers;              /* vertices moved (0: nothing was applied) */
@@ -286,9 +291,10 @@ typedef struct gpart_refine_status {
  * gains, flags and part_weights are device arrays the round updates in place;
  * one host synchronisation. host out: *status. */
 int gpart_refine_lp_round(gpart_ctx* ctx, const gpart_csr_view* g,
-                          const gpart_degree_classes* classes, uint64_t* cache, int32_t* labels,
-                          int k, double filter, int32_t* locks, int32_t* targets, int64_t* gains,
-                          int32_t* flags, uint64_t* part_weights, gpart_refine_status* status);
+                          const gpart_degree_classes* classes, void* cache, int cache_bytes,
+                          int32_t* labels, int k, double filter, int32_t* locks, int32_t* targets,
+                          int64_t* gains, int32_t* flags, uint64_t* part_weights,
+                          gpart_refine_status* status);
 /* One iteration of the rebalance loop (src/refine_incremental.cuh:303-331):
  * incremental_rebalance_proposals (:111-157), the source and destination bucket
  * quotas (:167-232) and the same apply without locks. Members of a bucket are
@@ -296,7 +302,7 @@ int gpart_refine_lp_round(gpart_ctx* ctx, const gpart_csr_view* g,
  * reference's atomicAdd ranking (:185,216). One host synchronisation.
  * host out: *status (movers == 0: nothing could move). */
 int gpart_refine_rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g,
-                                 const gpart_degree_classes* classes, uint64_t* cache,
+                                 const gpart_degree_classes* classes, void* cache, int cache_bytes,
                                  int32_t* labels, int k, uint64_t* part_weights, uint64_t capacity,
                                  int32_t* targets, int32_t* flags, gpart_refine_status* status);
 /* Largest edge weight (src/refine_incremental.cuh:332). host out. */
