@@ -32,6 +32,7 @@ from ._native import (  # noqa: F401
     record_layout,
     cache_stats,
     set_log,
+    set_refine_rebuild,
     trim,
 )
 from . import device  # noqa: F401

@@ -89,6 +89,7 @@ def hip_lib():
         H.gpart_ctx_synchronize.argtypes = [C.c_void_p]
         H.gpart_cache_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        H.gpart_refine_set_rebuild.argtypes = [C.c_void_p, C.c_int64, C.c_int64]
         _hip = H
     return _hip
 
@@ -190,6 +191,13 @@ def cache_stats() -> dict:
                                        C.byref(released), secs))
     return {"cached_blocks": blocks.value, "cached_bytes": cached.value, "live_bytes": live.value,
             "released_blocks": released.value, "release_seconds": list(secs)}
+
+
+def set_refine_rebuild(min_entries: int = 1 << 23, share: int = 16) -> None:
+    """The calling thread's refinement tuning: rounds on levels of >= min_entries
+    entries rebuild the affinity cache when their movers hold more than
+    1/share of the entries (share <= 0: never). Results do not depend on it."""
+    _check(hip_lib().gpart_refine_set_rebuild(_ctx(), int(min_entries), int(share)))
 
 
 def set_log(enabled: bool) -> None:

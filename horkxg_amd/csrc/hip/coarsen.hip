@@ -14,6 +14,7 @@
 #include "internal.hpp"
 #include "reduce.hpp"
 #include "scan.hpp"
+#include "hubs.hpp"
 
 #include <algorithm>
 
@@ -23,8 +24,8 @@ namespace {
 constexpr int kLowDegree = 8;
 constexpr int kMediumDegree = 256;
 constexpr int kMediumSlots = 512;
-constexpr int kHighLdsDegree = 2048;
-constexpr int kHighSlots = 4096;
+constexpr int kHighLdsDegree = 1024;
+constexpr int kHighSlots = 2 * kHighLdsDegree;
 constexpr int kBlock = 256;
 
 struct RoundParams {
@@ -264,88 +265,10 @@ __global__ void __launch_bounds__(kBlock) sclp_medium_kernel(std::int64_t count,
     if (active && lane == 0) decode(v, source, current, best, best_tie, r, proposals, gains, ties);
 }
 
-// Workgroup per hub. Tables: LDS when deg <= kHighLdsDegree, else the hub's
-// region of a global table (table_offset[i] slots, power of two).
-template <typename O, typename W, bool Singleton>
-__global__ void __launch_bounds__(kBlock) sclp_high_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
-                                                           const std::int32_t* neighbors, const W* weights,
-                                                           const std::int32_t* clusters, RoundParams r,
-                                                           const std::int64_t* table_offset,
-                                                           std::int32_t* global_keys,
-                                                           unsigned long long* global_values,
-                                                           std::int32_t* proposals, unsigned long long* gains,
-                                                           std::uint32_t* ties) {
-    __shared__ std::int32_t lds_keys[Singleton ? 1 : kHighSlots];
-    __shared__ unsigned long long lds_values[Singleton ? 1 : kHighSlots];
+// Folds each wave's (current, best, tie) into thread 0 of the workgroup.
+__device__ __forceinline__ void block_best(unsigned long long& current, unsigned long long& best,
+                                           unsigned long long& best_tie) {
     __shared__ unsigned long long red_current[kBlock / kWave], red_best[kBlock / kWave], red_tie[kBlock / kWave];
-    const std::int64_t index = blockIdx.x;
-    if (index >= count) return;  // uniform per block
-    const std::int32_t v = list[index];
-    const std::int64_t begin = offsets[v];
-    const std::int64_t end = offsets[v + 1];
-    const std::int32_t source = clusters[v];
-    unsigned long long current = 0, best = 0, best_tie = 0;
-    if constexpr (Singleton) {
-        for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
-            const std::int32_t u = neighbors[e];
-            if (u == v) continue;
-            offer(v, u, source, static_cast<unsigned long long>(weights[e]), r, current, best, best_tie);
-        }
-    } else {
-        const bool in_lds = end - begin <= kHighLdsDegree;
-        std::int64_t slots = kHighSlots;
-        std::int32_t* gkeys = nullptr;
-        unsigned long long* gvalues = nullptr;
-        if (!in_lds) {
-            const std::int64_t off = table_offset[index];
-            slots = table_offset[index + 1] - off;
-            gkeys = global_keys + off;
-            gvalues = global_values + off;
-        }
-        const std::int64_t slot_mask = slots - 1;
-        if (in_lds) {
-            for (int s = threadIdx.x; s < kHighSlots; s += kBlock) {
-                lds_keys[s] = -1;
-                lds_values[s] = 0;
-            }
-        }
-        __syncthreads();
-        for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
-            const std::int32_t u = neighbors[e];
-            if (u == v) continue;
-            const std::int32_t t = clusters[u];
-            std::int64_t slot = static_cast<std::int64_t>(mix32(static_cast<std::uint32_t>(t))) & slot_mask;
-            const auto w = static_cast<unsigned long long>(weights[e]);
-            if (in_lds) {
-                while (true) {
-                    const std::int32_t found = atomicCAS(lds_keys + slot, -1, t);
-                    if (found == -1 || found == t) {
-                        atomicAdd(lds_values + slot, w);
-                        break;
-                    }
-                    slot = (slot + 1) & slot_mask;
-                }
-            } else {
-                while (true) {
-                    const std::int32_t found = atomicCAS(gkeys + slot, -1, t);
-                    if (found == -1 || found == t) {
-                        atomicAdd(gvalues + slot, w);
-                        break;
-                    }
-                    slot = (slot + 1) & slot_mask;
-                }
-            }
-        }
-        if (!in_lds) __threadfence();
-        __syncthreads();
-        for (std::int64_t s = threadIdx.x; s < slots; s += kBlock) {
-            const std::int32_t t = in_lds ? lds_keys[s] : __hip_atomic_load(gkeys + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t < 0) continue;
-            const unsigned long long c = in_lds ? lds_values[s]
-                                                : __hip_atomic_load(gvalues + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            offer(v, t, source, c, r, current, best, best_tie);
-        }
-    }
     current = group_max_u64<kWave>(current);
     group_best<kWave>(best, best_tie);
     const int wave = threadIdx.x / kWave;
@@ -363,21 +286,241 @@ __global__ void __launch_bounds__(kBlock) sclp_high_kernel(std::int64_t count, c
                 best_tie = red_tie[w];
             }
         }
-        decode(v, source, current, best, best_tie, r, proposals, gains, ties);
     }
+    __syncthreads();
+}
+
+// Workgroup per high vertex of degree <= kHighLdsDegree, with an LDS table of
+// the smallest power of two >= 2 * degree slots; larger rows are hubs, done in
+// chunks by the kernels below.
+template <typename O, typename W, bool Singleton>
+__global__ void __launch_bounds__(kBlock) sclp_high_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
+                                                           const std::int32_t* neighbors, const W* weights,
+                                                           const std::int32_t* clusters, RoundParams r,
+                                                           std::int32_t* proposals, unsigned long long* gains,
+                                                           std::uint32_t* ties) {
+    __shared__ std::int32_t lds_keys[Singleton ? 1 : kHighSlots];
+    __shared__ unsigned long long lds_values[Singleton ? 1 : kHighSlots];
+    const std::int64_t index = blockIdx.x;
+    if (index >= count) return;  // uniform per block
+    const std::int32_t v = list[index];
+    const std::int64_t begin = offsets[v];
+    const std::int64_t end = offsets[v + 1];
+    if (end - begin > kHighLdsDegree) return;  // a hub
+    const std::int32_t source = clusters[v];
+    unsigned long long current = 0, best = 0, best_tie = 0;
+    if constexpr (Singleton) {
+        for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
+            const std::int32_t u = neighbors[e];
+            if (u == v) continue;
+            offer(v, u, source, static_cast<unsigned long long>(weights[e]), r, current, best, best_tie);
+        }
+    } else {
+        int slots = 1;
+        while (slots < 2 * (end - begin)) slots <<= 1;
+        const int slot_mask = slots - 1;
+        for (int s = threadIdx.x; s < slots; s += kBlock) {
+            lds_keys[s] = -1;
+            lds_values[s] = 0;
+        }
+        __syncthreads();
+        for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
+            const std::int32_t u = neighbors[e];
+            if (u == v) continue;
+            const std::int32_t t = clusters[u];
+            int slot = static_cast<int>(mix32(static_cast<std::uint32_t>(t))) & slot_mask;
+            while (true) {
+                const std::int32_t found = atomicCAS(lds_keys + slot, -1, t);
+                if (found == -1 || found == t) {
+                    atomicAdd(lds_values + slot, static_cast<unsigned long long>(weights[e]));
+                    break;
+                }
+                slot = (slot + 1) & slot_mask;
+            }
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < slots; s += kBlock) {
+            const std::int32_t t = lds_keys[s];
+            if (t < 0) continue;
+            offer(v, t, source, lds_values[s], r, current, best, best_tie);
+        }
+    }
+    block_best(current, best, best_tie);
+    if (threadIdx.x == 0) decode(v, source, current, best, best_tie, r, proposals, gains, ties);
+}
+
+// Inserts (t, w) into an open-addressing table whose keys never change once set:
+// a key already present costs a load and an add, no compare-and-swap.
+template <typename Index>
+__device__ __forceinline__ void table_add(std::int32_t* keys, unsigned long long* values, Index slot, Index mask,
+                                          std::int32_t t, unsigned long long w) {
+    while (true) {
+        std::int32_t found = __hip_atomic_load(keys + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (found == -1) found = atomicCAS(keys + slot, -1, t);
+        if (found == -1 || found == t) {
+            atomicAdd(values + slot, w);
+            return;
+        }
+        slot = (slot + 1) & mask;
+    }
+}
+
+// Hubs (degree > kHighLdsDegree), one chunk of kHubChunk entries per workgroup
+// pass. Singleton rounds offer every entry directly and leave a partial
+// (current, best, tie) per chunk. Otherwise each half chunk is summed per
+// cluster in an LDS table first, whose entries then go into the hub's global
+// table (table_offset[i] .. table_offset[i + 1], a power of two >= 2 * degree):
+// a cluster that holds many of the hub's neighbours costs one global add per
+// half chunk instead of one per entry.
+constexpr int kHubStage = kHubChunk / 2;
+constexpr int kHubStageSlots = 2 * kHubStage;
+
+template <typename O, typename W, bool Singleton>
+__global__ void __launch_bounds__(kBlock) sclp_hub_insert_kernel(const std::int64_t* chunk_total,
+                                                                 const std::int64_t* chunk_start,
+                                                                 const std::int32_t* chunk_hub,
+                                                                 const std::int32_t* list, const O* offsets,
+                                                                 const std::int32_t* neighbors, const W* weights,
+                                                                 const std::int32_t* clusters, RoundParams r,
+                                                                 const std::int64_t* table_offset,
+                                                                 std::int32_t* global_keys,
+                                                                 unsigned long long* global_values,
+                                                                 unsigned long long* partial_current,
+                                                                 unsigned long long* partial_best,
+                                                                 unsigned long long* partial_tie) {
+    __shared__ std::int32_t stage_keys[Singleton ? 1 : kHubStageSlots];
+    __shared__ unsigned long long stage_values[Singleton ? 1 : kHubStageSlots];
+    const std::int64_t chunks = *chunk_total;
+    for (std::int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        std::int32_t i, v;
+        std::int64_t begin, end;
+        hub_chunk_range(offsets, list, chunk_start, chunk_hub, c, &i, &v, &begin, &end);
+        if constexpr (Singleton) {
+            const std::int32_t source = clusters[v];
+            unsigned long long current = 0, best = 0, best_tie = 0;
+            for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
+                const std::int32_t u = neighbors[e];
+                if (u == v) continue;
+                offer(v, u, source, static_cast<unsigned long long>(weights[e]), r, current, best, best_tie);
+            }
+            block_best(current, best, best_tie);
+            if (threadIdx.x == 0) {
+                partial_current[c] = current;
+                partial_best[c] = best;
+                partial_tie[c] = best_tie;
+            }
+        } else {
+            const std::int64_t off = table_offset[i];
+            const std::int64_t global_mask = table_offset[i + 1] - off - 1;
+            std::int32_t* keys = global_keys + off;
+            unsigned long long* values = global_values + off;
+            for (std::int64_t stage = begin; stage < end; stage += kHubStage) {
+                const std::int64_t stop = stage + kHubStage < end ? stage + kHubStage : end;
+                for (int s = threadIdx.x; s < kHubStageSlots; s += kBlock) {
+                    stage_keys[s] = -1;
+                    stage_values[s] = 0;
+                }
+                __syncthreads();
+                for (std::int64_t e = stage + threadIdx.x; e < stop; e += kBlock) {
+                    const std::int32_t u = neighbors[e];
+                    if (u == v) continue;
+                    const std::int32_t t = clusters[u];
+                    int slot = static_cast<int>(mix32(static_cast<std::uint32_t>(t))) & (kHubStageSlots - 1);
+                    while (true) {
+                        const std::int32_t found = atomicCAS(stage_keys + slot, -1, t);
+                        if (found == -1 || found == t) {
+                            atomicAdd(stage_values + slot, static_cast<unsigned long long>(weights[e]));
+                            break;
+                        }
+                        slot = (slot + 1) & (kHubStageSlots - 1);
+                    }
+                }
+                __syncthreads();
+                for (int s = threadIdx.x; s < kHubStageSlots; s += kBlock) {
+                    const std::int32_t t = stage_keys[s];
+                    if (t < 0) continue;
+                    const std::int64_t slot =
+                        static_cast<std::int64_t>(mix32(static_cast<std::uint32_t>(t))) & global_mask;
+                    table_add<std::int64_t>(keys, values, slot, global_mask, t, stage_values[s]);
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// Partial (current, best, tie) of each hub chunk over a strided share of the
+// hub's table: chunk j of a hub with h chunks reads slots j*256 + t + q*256*h.
+template <typename O>
+__global__ void __launch_bounds__(kBlock) sclp_hub_scan_kernel(const std::int64_t* chunk_total,
+                                                               const std::int64_t* chunk_start,
+                                                               const std::int32_t* chunk_hub,
+                                                               const std::int32_t* list, const std::int32_t* clusters,
+                                                               RoundParams r, const std::int64_t* table_offset,
+                                                               const std::int32_t* global_keys,
+                                                               const unsigned long long* global_values,
+                                                               unsigned long long* partial_current,
+                                                               unsigned long long* partial_best,
+                                                               unsigned long long* partial_tie) {
+    const std::int64_t chunks = *chunk_total;
+    for (std::int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const std::int32_t i = chunk_hub[c];
+        const std::int32_t v = list[i];
+        const std::int32_t source = clusters[v];
+        const std::int64_t first = chunk_start[i];
+        const std::int64_t stride = (chunk_start[i + 1] - first) * kBlock;
+        const std::int64_t off = table_offset[i];
+        const std::int64_t slots = table_offset[i + 1] - off;
+        unsigned long long current = 0, best = 0, best_tie = 0;
+        for (std::int64_t s = (c - first) * kBlock + threadIdx.x; s < slots; s += stride) {
+            const std::int32_t t = global_keys[off + s];
+            if (t < 0) continue;
+            offer(v, t, source, global_values[off + s], r, current, best, best_tie);
+        }
+        block_best(current, best, best_tie);
+        if (threadIdx.x == 0) {
+            partial_current[c] = current;
+            partial_best[c] = best;
+            partial_tie[c] = best_tie;
+        }
+    }
+}
+
+// One thread per high vertex: folds the chunk partials of a hub and decodes.
+__global__ void sclp_hub_decode_kernel(std::int64_t count, const std::int64_t* chunk_start, const std::int32_t* list,
+                                       const std::int32_t* clusters, RoundParams r,
+                                       const unsigned long long* partial_current,
+                                       const unsigned long long* partial_best, const unsigned long long* partial_tie,
+                                       std::int32_t* proposals, unsigned long long* gains, std::uint32_t* ties) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const std::int64_t first = chunk_start[i], last = chunk_start[i + 1];
+    if (first == last) return;
+    unsigned long long current = 0, best = 0, best_tie = 0;
+    for (std::int64_t c = first; c < last; ++c) {
+        current = partial_current[c] > current ? partial_current[c] : current;
+        if (better(partial_best[c], partial_tie[c], best, best_tie)) {
+            best = partial_best[c];
+            best_tie = partial_tie[c];
+        }
+    }
+    const std::int32_t v = list[i];
+    decode(v, clusters[v], current, best, best_tie, r, proposals, gains, ties);
 }
 
 template <typename O>
 __global__ void hub_table_size_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
                                       std::int64_t* sizes) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const std::int32_t v = list[i];
-    const std::int64_t deg = offsets[v + 1] - offsets[v];
+    if (i > count) return;
     std::int64_t size = 0;
-    if (deg > kHighLdsDegree) {
-        size = 1;
-        while (size < 2 * deg) size <<= 1;
+    if (i < count) {
+        const std::int32_t v = list[i];
+        const std::int64_t deg = offsets[v + 1] - offsets[v];
+        if (deg > kHighLdsDegree) {
+            size = 1;
+            while (size < 2 * deg) size <<= 1;
+        }
     }
     sizes[i] = size;
 }
@@ -643,12 +786,26 @@ struct U64Load {
 };
 
 // contract_record_bounds (src/coarsen.cu:72-85): saturating add through the map.
+// Wrapping adds first: a wrap sets *wrapped (the add that crosses 2^64 always
+// sees it) and the host then redoes the level with the saturating kernel below.
+// Zero bounds (isolated vertices, often packed into a few clusters) are skipped.
+__global__ void contract_bounds_add_kernel(std::int64_t n, const std::int32_t* map, const unsigned long long* fine,
+                                           unsigned long long* coarse, unsigned long long* wrapped) {
+    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const unsigned long long amount = fine[v];
+    if (amount == 0) return;
+    const unsigned long long previous = atomicAdd(coarse + map[v], amount);
+    if (previous > ~0ULL - amount) *wrapped = 1;
+}
+
 __global__ void contract_bounds_kernel(std::int64_t n, const std::int32_t* map, const unsigned long long* fine,
                                        unsigned long long* coarse) {
     const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= n) return;
     unsigned long long* target = coarse + map[v];
     const unsigned long long amount = fine[v];
+    if (amount == 0) return;
     unsigned long long previous = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (true) {
         const unsigned long long sum = ~0ULL - previous < amount ? ~0ULL : previous + amount;
@@ -712,14 +869,21 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
     const auto* w = static_cast<const W*>(g->edge_weights);
     RoundParams r{round->role_salt, round->tie_salt, round->mover_threshold, round->filter_roles != 0};
     const bool single = round->singleton != 0;
-    // per-hub global tables for hubs beyond the LDS table
+    // hubs (degree > kHighLdsDegree) in chunks; global tables outside singleton rounds
+    const bool hubs = c->high_count > 0 && c->max_degree > kHighLdsDegree;
+    HubChunks chunks;
     Scratch<std::int64_t> table_offset;
     Scratch<std::int32_t> gkeys;
-    Scratch<unsigned long long> gvalues;
-    if (!single && c->high_count > 0 && c->max_degree > kHighLdsDegree) {
+    Scratch<unsigned long long> gvalues, pcurrent, pbest, ptie;
+    if (hubs) {
+        build_hub_chunks(ctx, off, c->high, c->high_count, c->high_edges, kHighLdsDegree, chunks);
+        pcurrent.reset(ctx, chunks.bound);
+        pbest.reset(ctx, chunks.bound);
+        ptie.reset(ctx, chunks.bound);
+    }
+    if (hubs && !single) {
         table_offset.reset(ctx, c->high_count + 1);
-        fill_bytes(ctx, table_offset.get(), 0, static_cast<std::size_t>(c->high_count + 1) * 8);
-        hipLaunchKernelGGL(hub_table_size_kernel<O>, dim3(grid_for(c->high_count)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(hub_table_size_kernel<O>, dim3(grid_for(c->high_count + 1)), dim3(kBlock), 0, ctx->stream,
                            c->high_count, c->high, off, table_offset.get());
         GPART_LAUNCH_CHECK();
         exclusive_scan<std::int64_t>(ctx, table_offset.get(), table_offset.get(), c->high_count + 1);
@@ -755,12 +919,31 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
         const unsigned blocks = static_cast<unsigned>(c->high_count);
         if (single)
             hipLaunchKernelGGL((sclp_high_kernel<O, W, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
-                               c->high_count, c->high, off, g->neighbors, w, clusters, r, table_offset.get(),
-                               gkeys.get(), gvalues.get(), proposals, gains, ties);
+                               c->high_count, c->high, off, g->neighbors, w, clusters, r, proposals, gains, ties);
         else
             hipLaunchKernelGGL((sclp_high_kernel<O, W, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
-                               c->high_count, c->high, off, g->neighbors, w, clusters, r, table_offset.get(),
-                               gkeys.get(), gvalues.get(), proposals, gains, ties);
+                               c->high_count, c->high, off, g->neighbors, w, clusters, r, proposals, gains, ties);
+        GPART_LAUNCH_CHECK();
+    }
+    if (hubs) {
+        if (single) {
+            hipLaunchKernelGGL((sclp_hub_insert_kernel<O, W, true>), dim3(chunks.grid()), dim3(kBlock), 0,
+                               ctx->stream, chunks.total(), chunks.start.get(), chunks.hub.get(), c->high, off,
+                               g->neighbors, w, clusters, r, nullptr, nullptr, nullptr, pcurrent.get(), pbest.get(),
+                               ptie.get());
+        } else {
+            hipLaunchKernelGGL((sclp_hub_insert_kernel<O, W, false>), dim3(chunks.grid()), dim3(kBlock), 0,
+                               ctx->stream, chunks.total(), chunks.start.get(), chunks.hub.get(), c->high, off,
+                               g->neighbors, w, clusters, r, table_offset.get(), gkeys.get(), gvalues.get(), nullptr,
+                               nullptr, nullptr);
+            hipLaunchKernelGGL(sclp_hub_scan_kernel<O>, dim3(chunks.grid()), dim3(kBlock), 0, ctx->stream,
+                               chunks.total(), chunks.start.get(), chunks.hub.get(), c->high, clusters, r,
+                               table_offset.get(), gkeys.get(), gvalues.get(), pcurrent.get(), pbest.get(),
+                               ptie.get());
+        }
+        hipLaunchKernelGGL(sclp_hub_decode_kernel, dim3(grid_for(c->high_count)), dim3(kBlock), 0, ctx->stream,
+                           c->high_count, chunks.start.get(), c->high, clusters, r, pcurrent.get(), pbest.get(),
+                           ptie.get(), proposals, gains, ties);
         GPART_LAUNCH_CHECK();
     }
     const double n = static_cast<double>(g->n), m = static_cast<double>(g->m);
@@ -848,15 +1031,27 @@ int gpart_contract_record_bounds(gpart_ctx* ctx, int64_t n, const int32_t* map, 
     return guarded([&] {
         fill_bytes(ctx, coarse_bounds, 0, static_cast<std::size_t>(n_coarse) * 8);
         auto* cb = reinterpret_cast<unsigned long long*>(coarse_bounds);
+        const auto* fb = reinterpret_cast<const unsigned long long*>(fine_bounds);
+        Scratch<unsigned long long> meta(ctx, 2);  // maximum, wrapped
+        fill_bytes(ctx, meta.get(), 0, 16);
         if (n > 0) {
-            hipLaunchKernelGGL(contract_bounds_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, map,
-                               reinterpret_cast<const unsigned long long*>(fine_bounds), cb);
+            hipLaunchKernelGGL(contract_bounds_add_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, map,
+                               fb, cb, meta.get() + 1);
             GPART_LAUNCH_CHECK();
         }
-        Scratch<unsigned long long> mx(ctx, 1);
-        fill_bytes(ctx, mx.get(), 0, 8);
-        if (n_coarse > 0) device_reduce<unsigned long long>(ctx, U64Load{cb}, n_coarse, MaxU64{}, mx.get());
-        *maximum = read_scalar(ctx, mx.get());
+        if (n_coarse > 0) device_reduce<unsigned long long>(ctx, U64Load{cb}, n_coarse, MaxU64{}, meta.get());
+        unsigned long long h[2];
+        read_back(ctx, h, meta.get(), sizeof(h));
+        if (h[1]) {
+            fill_bytes(ctx, coarse_bounds, 0, static_cast<std::size_t>(n_coarse) * 8);
+            fill_bytes(ctx, meta.get(), 0, 8);
+            hipLaunchKernelGGL(contract_bounds_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, map, fb,
+                               cb);
+            GPART_LAUNCH_CHECK();
+            if (n_coarse > 0) device_reduce<unsigned long long>(ctx, U64Load{cb}, n_coarse, MaxU64{}, meta.get());
+            h[0] = read_scalar(ctx, meta.get());
+        }
+        *maximum = h[0];
     });
 }
 

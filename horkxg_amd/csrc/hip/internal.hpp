@@ -47,6 +47,11 @@ struct gpart_ctx {
         std::int64_t launches = 0;
     } profile[GPART_PROFILE_FAMILIES];
     std::vector<hipEvent_t> spare_events;
+    // Refinement rounds rebuild the affinity cache instead of updating it when the
+    // level has at least rebuild_min_entries entries and the round's movers hold
+    // more than 1/rebuild_share of them (share <= 0: never).
+    std::int64_t rebuild_min_entries = std::int64_t{1} << 23;
+    std::int64_t rebuild_share = 16;
 };
 
 namespace gpart_hip {

@@ -17,6 +17,8 @@
 #include "internal.hpp"
 #include "reduce.hpp"
 #include "scan.hpp"
+#include "tiles.hpp"
+#include "hubs.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -33,28 +35,50 @@ inline unsigned grid_for(std::int64_t threads) { return static_cast<unsigned>((t
 
 // ---------------------------------------------------------------- cut / weights
 
-template <typename O, typename W, int G>
-__global__ void __launch_bounds__(kBlock) cut_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
-                                                     const std::int32_t* neighbors, const W* weights,
-                                                     const std::int32_t* labels, unsigned long long* directed) {
-    // grid-stride over vertex groups; one partial per workgroup in directed[blockIdx.x]
-    __shared__ unsigned long long waves[kBlock / kWave];
-    const int sub = threadIdx.x & (G - 1);
-    const std::int64_t groups_per_grid = static_cast<std::int64_t>(gridDim.x) * (blockDim.x / G);
+// Directed cut weight over edge tiles (grid-stride over tiles); one partial per
+// workgroup in directed[blockIdx.x]. A lane loads its run's neighbours and
+// weights up front, then gathers their labels, so 16 loads are in flight per lane.
+template <typename O, typename W>
+__global__ void __launch_bounds__(kTileLanes) cut_kernel(std::int64_t n, std::int64_t m, std::int64_t tiles,
+                                                         const O* offsets, const std::int32_t* tile_row,
+                                                         const std::int32_t* neighbors, const W* weights,
+                                                         const std::int32_t* labels, unsigned long long* directed) {
+    __shared__ unsigned long long waves[kTileLanes / kWave];
     unsigned long long local = 0;
-    for (std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; group < count;
-         group += groups_per_grid) {
-        const std::int32_t v = list ? list[group] : static_cast<std::int32_t>(group);
-        const std::int32_t p = labels[v];
-        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G)
-            if (labels[neighbors[e]] != p) local += static_cast<unsigned long long>(weights[e]);
+    for (std::int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        std::int64_t e, stop;
+        std::int32_t r;
+        if (!tile_run(offsets, tile_row, tiles, n, m, t, &e, &stop, &r)) continue;
+        std::int32_t other[kTileRun];
+        W w[kTileRun];
+#pragma unroll
+        for (int j = 0; j < kTileRun; ++j) {
+            const bool in = e + j < stop;
+            other[j] = in ? neighbors[e + j] : 0;
+            w[j] = in ? weights[e + j] : W{0};
+        }
+#pragma unroll
+        for (int j = 0; j < kTileRun; ++j) other[j] = labels[other[j]];
+        std::int64_t next = static_cast<std::int64_t>(offsets[r + 1]);
+        std::int32_t p = labels[r];
+#pragma unroll
+        for (int j = 0; j < kTileRun; ++j) {
+            if (e + j < stop) {
+                if (e + j >= next) {
+                    do next = static_cast<std::int64_t>(offsets[++r + 1]);
+                    while (e + j >= next);
+                    p = labels[r];
+                }
+                if (other[j] != p) local += static_cast<unsigned long long>(w[j]);
+            }
+        }
     }
     local = group_sum_u64<kWave>(local);
     if (lane_id() == 0) waves[threadIdx.x / kWave] = local;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long total = 0;
-        for (int w = 0; w < kBlock / kWave; ++w) total += waves[w];
+        for (int w = 0; w < kTileLanes / kWave; ++w) total += waves[w];
         directed[blockIdx.x] = total;
     }
 }
@@ -84,25 +108,77 @@ template <typename O, typename W, int G, typename C>
 __global__ void __launch_bounds__(kBlock) build_cache_kernel(std::int64_t count, const std::int32_t* list,
                                                              const O* offsets, const std::int32_t* neighbors,
                                                              const W* weights, const std::int32_t* labels, int k,
-                                                             C* cache) {
+                                                             C* cache, const long long* gate, long long above) {
+    // gate: a rebuild inside a round runs only when the round's mover entries exceed
+    // `above`; grid-stride over groups so that a gated-off launch is a small grid
+    if (gate && *gate <= above) return;
     constexpr int kGroups = kBlock / G;
     __shared__ unsigned long long rows[kGroups][32];
     const int local_group = threadIdx.x / G;
     const int sub = threadIdx.x & (G - 1);
-    const std::int64_t group = static_cast<std::int64_t>(blockIdx.x) * kGroups + local_group;
-    for (int p = sub; p < 32; p += G) rows[local_group][p] = 0;
-    __syncthreads();
-    std::int32_t v = 0;
-    if (group < count) {
-        v = list ? list[group] : static_cast<std::int32_t>(group);
-        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G) {
-            const std::int32_t u = neighbors[e];
-            if (u != v) atomicAdd(&rows[local_group][labels[u]], static_cast<unsigned long long>(weights[e]));
+    for (std::int64_t first = static_cast<std::int64_t>(blockIdx.x) * kGroups; first < count;
+         first += static_cast<std::int64_t>(gridDim.x) * kGroups) {
+        const std::int64_t group = first + local_group;
+        for (int p = sub; p < 32; p += G) rows[local_group][p] = 0;
+        __syncthreads();
+        std::int32_t v = 0;
+        bool own = group < count;
+        if (own) {
+            v = list ? list[group] : static_cast<std::int32_t>(group);
+            // rows above kHubChunk entries are hubs, built by build_cache_hub_kernel
+            own = G < kBlock || offsets[v + 1] - offsets[v] <= kHubChunk;
+            if (own)
+                for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G) {
+                    const std::int32_t u = neighbors[e];
+                    if (u != v) atomicAdd(&rows[local_group][labels[u]], static_cast<unsigned long long>(weights[e]));
+                }
         }
+        __syncthreads();
+        if (own)
+            for (int p = sub; p < k; p += G)
+                cache[static_cast<std::int64_t>(v) * k + p] = static_cast<C>(rows[local_group][p]);
+        __syncthreads();
     }
-    __syncthreads();
-    if (group < count)
-        for (int p = sub; p < k; p += G) cache[static_cast<std::int64_t>(v) * k + p] = static_cast<C>(rows[local_group][p]);
+}
+
+template <typename C>
+__global__ void hub_rows_zero_kernel(std::int64_t count, const std::int64_t* chunk_start, const std::int32_t* list,
+                                     int k, C* cache, const long long* gate, long long above) {
+    if (gate && *gate <= above) return;
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count || chunk_start[i] == chunk_start[i + 1]) return;
+    C* row = cache + static_cast<std::int64_t>(list[i]) * k;
+    for (int p = 0; p < k; ++p) row[p] = 0;
+}
+
+// Hub rows (degree > kHubChunk), one chunk per workgroup pass: the chunk's part
+// sums gather in LDS and reach the zeroed cache row as k atomic adds.
+template <typename O, typename W, typename C>
+__global__ void __launch_bounds__(kBlock) build_cache_hub_kernel(const std::int64_t* chunk_total,
+                                                                 const std::int64_t* chunk_start,
+                                                                 const std::int32_t* chunk_hub,
+                                                                 const std::int32_t* list, const O* offsets,
+                                                                 const std::int32_t* neighbors, const W* weights,
+                                                                 const std::int32_t* labels, int k, C* cache,
+                                                                 const long long* gate, long long above) {
+    __shared__ unsigned long long row[32];
+    if (gate && *gate <= above) return;
+    const std::int64_t chunks = *chunk_total;
+    for (std::int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        if (threadIdx.x < 32) row[threadIdx.x] = 0;
+        __syncthreads();
+        std::int32_t i, v;
+        std::int64_t begin, end;
+        hub_chunk_range(offsets, list, chunk_start, chunk_hub, c, &i, &v, &begin, &end);
+        for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
+            const std::int32_t u = neighbors[e];
+            if (u != v) atomicAdd(&row[labels[u]], static_cast<unsigned long long>(weights[e]));
+        }
+        __syncthreads();
+        if (threadIdx.x < k && row[threadIdx.x])
+            atomicAdd(cache + static_cast<std::int64_t>(v) * k + threadIdx.x, static_cast<C>(row[threadIdx.x]));
+        __syncthreads();
+    }
 }
 
 // Rows of the n x k cache are read by groups of G lanes, lane j owning the
@@ -164,7 +240,30 @@ __global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, co
     gains[v] = gain;
 }
 
-// incremental_priority_filter (:48-67)
+// incremental_priority_filter (:48-67). The correction of a proposing vertex v
+// summed over its entries [e, end) with stride `step`: neighbours that propose
+// too and rank before v (higher gain, ties to the lower id).
+template <typename W>
+__device__ __forceinline__ long long filter_correction(std::int32_t v, std::int32_t target, std::int32_t source,
+                                                       long long gain, std::int64_t e, std::int64_t end, int step,
+                                                       const std::int32_t* neighbors, const W* weights,
+                                                       const std::int32_t* labels, const std::int32_t* targets,
+                                                       const long long* gains) {
+    long long correction = 0;
+    for (; e < end; e += step) {
+        const std::int32_t u = neighbors[e];
+        if (u == v) continue;
+        const std::int32_t tu = targets[u];
+        if (tu < 0) continue;
+        const long long gu = gains[u];
+        if (!(gu > gain || (gu == gain && u < v))) continue;
+        const std::int32_t lu = labels[u];
+        const long long w = static_cast<long long>(weights[e]);
+        correction += w * ((tu == target) - (tu == source) - (lu == target) + (lu == source));
+    }
+    return correction;
+}
+
 template <typename O, typename W, int G>
 __global__ void __launch_bounds__(kBlock) priority_filter_kernel(std::int64_t count, const std::int32_t* list,
                                                                  const O* offsets, const std::int32_t* neighbors,
@@ -177,22 +276,13 @@ __global__ void __launch_bounds__(kBlock) priority_filter_kernel(std::int64_t co
     const bool active = group < count;
     const std::int32_t v = active ? (list ? list[group] : static_cast<std::int32_t>(group)) : 0;
     const std::int32_t target = active ? targets[v] : -1;
+    // rows above kHubChunk entries are hubs, done in chunks by priority_filter_hub_kernel
+    if (G == kBlock && active && offsets[v + 1] - offsets[v] > kHubChunk) return;  // uniform per block
     long long correction = 0;
-    if (active && target >= 0) {
-        const std::int32_t source = labels[v];
-        const long long gain = gains[v];
-        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G) {
-            const std::int32_t u = neighbors[e];
-            if (u == v) continue;
-            const std::int32_t tu = targets[u];
-            if (tu < 0) continue;
-            const long long gu = gains[u];
-            if (!(gu > gain || (gu == gain && u < v))) continue;
-            const std::int32_t lu = labels[u];
-            const long long w = static_cast<long long>(weights[e]);
-            correction += w * ((tu == target) - (tu == source) - (lu == target) + (lu == source));
-        }
-    }
+    if (active && target >= 0)
+        correction = filter_correction(v, target, labels[v], gains[v], static_cast<std::int64_t>(offsets[v]) + sub,
+                                       static_cast<std::int64_t>(offsets[v + 1]), G, neighbors, weights, labels,
+                                       targets, gains);
     if constexpr (G <= kWave) {
         correction = group_sum_i64<G>(correction);
         if (active && sub == 0) flags[v] = target >= 0 && gains[v] + correction >= 0;
@@ -208,6 +298,50 @@ __global__ void __launch_bounds__(kBlock) priority_filter_kernel(std::int64_t co
     }
 }
 
+// Hub rows (degree > kHubChunk): each chunk adds its share of the correction to
+// sums[i]; priority_filter_hub_finish_kernel then sets the hub's flag.
+template <typename O, typename W>
+__global__ void __launch_bounds__(kBlock) priority_filter_hub_kernel(const std::int64_t* chunk_total,
+                                                                     const std::int64_t* chunk_start,
+                                                                     const std::int32_t* chunk_hub,
+                                                                     const std::int32_t* list, const O* offsets,
+                                                                     const std::int32_t* neighbors, const W* weights,
+                                                                     const std::int32_t* labels,
+                                                                     const std::int32_t* targets,
+                                                                     const long long* gains, long long* sums) {
+    __shared__ long long block_sum[kBlock / kWave];
+    const std::int64_t chunks = *chunk_total;
+    for (std::int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        std::int32_t i, v;
+        std::int64_t begin, end;
+        hub_chunk_range(offsets, list, chunk_start, chunk_hub, c, &i, &v, &begin, &end);
+        const std::int32_t target = targets[v];
+        long long correction = 0;
+        if (target >= 0)
+            correction = filter_correction(v, target, labels[v], gains[v], begin + threadIdx.x, end, kBlock,
+                                           neighbors, weights, labels, targets, gains);
+        correction = group_sum_i64<kWave>(correction);
+        if (lane_id() == 0) block_sum[threadIdx.x / kWave] = correction;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long total = 0;
+            for (int w = 0; w < kBlock / kWave; ++w) total += block_sum[w];
+            if (total) atomicAdd(reinterpret_cast<unsigned long long*>(sums + i), static_cast<unsigned long long>(total));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void priority_filter_hub_finish_kernel(std::int64_t count, const std::int64_t* chunk_start,
+                                                  const std::int32_t* list, const std::int32_t* targets,
+                                                  const long long* gains, long long* sums, std::int32_t* flags) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count || chunk_start[i] == chunk_start[i + 1]) return;
+    const std::int32_t v = list[i];
+    flags[v] = targets[v] >= 0 && gains[v] + sums[i] >= 0;
+    sums[i] = 0;
+}
+
 // ---------------------------------------------------------------- round state
 
 // Device record of one round, read back once at its end.
@@ -217,12 +351,14 @@ struct RoundState {
     long long delta;       // exact cut decrease
     long long candidates;  // rebalance: vertices of overloaded parts
     unsigned long long max_weight;
-    unsigned long long pad[3];
+    long long edges;       // CSR entries of all movers (light + heavy)
+    unsigned long long pad[2];
     unsigned long long pw[32];
 };
 
 constexpr int kHeavyDegree = 64;
 constexpr int kRoundBlocks = 1024;  // cap of grid-stride launches whose count lives on the device
+constexpr int kCacheBlocks = 4096;  // cap of the cache build launches (grid-stride)
 
 // incremental_update_cache_and_cut (:69-95) over a mover list whose length is on the device.
 template <typename O, typename W, int G, typename C>
@@ -231,9 +367,12 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* c
                                                               const W* weights, const std::int32_t* labels,
                                                               const std::int32_t* targets,
                                                               const std::int32_t* flags, int k,
-                                                              C* cache, long long* delta) {
+                                                              C* cache, long long* delta,
+                                                              const long long* edges_ptr, long long rebuild_above) {
     __shared__ long long block_sum[kBlock / kWave];
     const long long count = *count_ptr;
+    // a rebuilt cache replaces the atomics; the cut delta is still summed here
+    const bool update = rebuild_above < 0 || *edges_ptr <= rebuild_above;
     const int sub = threadIdx.x & (G - 1);
     const long long groups = static_cast<long long>(gridDim.x) * (kBlock / G);
     long long gain = 0;
@@ -246,9 +385,11 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* c
             const std::int32_t u = neighbors[e];
             if (u == v) continue;
             const auto w = static_cast<unsigned long long>(weights[e]);
-            // modular arithmetic: entries fit C at every point of the round
-            atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
-            atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
+            if (update) {
+                // modular arithmetic: entries fit C at every point of the round
+                atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
+                atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
+            }
             const bool fu = flags[u] != 0;
             if (!fu || u > v) {
                 const std::int32_t lu = labels[u];
@@ -264,6 +405,87 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* c
         long long total = 0;
         for (int w = 0; w < kBlock / kWave; ++w) total += block_sum[w];
         if (total) atomicAdd(reinterpret_cast<unsigned long long*>(delta), static_cast<unsigned long long>(total));
+    }
+}
+
+// Heavy movers (degree > kHeavyDegree) as one flattened entry range: heavy_start
+// holds the exclusive prefix of their degrees (count + 1 entries, the last one
+// the total), tiled as in tiles.hpp with the movers as rows.
+template <typename O>
+struct HeavyDegreeLoad {
+    const RoundState* st;
+    const std::int32_t* heavy;
+    const O* offsets;
+    __device__ __forceinline__ long long operator()(std::int64_t i) const {
+        if (i >= st->heavy) return 0;
+        const std::int32_t v = heavy[i];
+        return static_cast<long long>(offsets[v + 1] - offsets[v]);
+    }
+};
+
+__global__ void heavy_tiles_kernel(const RoundState* st, const long long* heavy_start, std::int32_t* tile_mover) {
+    const long long count = st->heavy;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long lo = heavy_start[i], hi = heavy_start[i + 1];
+        for (long long t = (lo + kTileEdges - 1) / kTileEdges; t * kTileEdges < hi; ++t)
+            tile_mover[t] = static_cast<std::int32_t>(i);
+    }
+}
+
+// incremental_update_cache_and_cut over the heavy movers' entries, kTileRun per lane.
+template <typename O, typename W, typename C>
+__global__ void __launch_bounds__(kTileLanes) update_cache_tiles_kernel(
+    const RoundState* st, const long long* heavy_start, const std::int32_t* tile_mover, const std::int32_t* movers,
+    const O* offsets, const std::int32_t* neighbors, const W* weights, const std::int32_t* labels,
+    const std::int32_t* targets, const std::int32_t* flags, int k, C* cache, long long* delta,
+    long long rebuild_above) {
+    __shared__ long long block_sum[kTileLanes / kWave];
+    const long long count = st->heavy;
+    const long long total = heavy_start[count];
+    const long long tiles = tile_count(total);
+    const bool update = rebuild_above < 0 || st->edges <= rebuild_above;
+    long long gain = 0;
+    for (long long t = blockIdx.x; t < tiles; t += gridDim.x) {
+        std::int64_t x, stop;
+        std::int32_t i;
+        if (!tile_run(heavy_start, tile_mover, tiles, count, total, t, &x, &stop, &i)) continue;
+        std::int32_t v = movers[i];
+        std::int32_t source = labels[v], target = targets[v];
+        long long next = heavy_start[i + 1];
+        std::int64_t base = static_cast<std::int64_t>(offsets[v]) - heavy_start[i];
+        for (; x < stop; ++x) {
+            while (x >= next) {
+                ++i;
+                v = movers[i];
+                source = labels[v];
+                target = targets[v];
+                base = static_cast<std::int64_t>(offsets[v]) - heavy_start[i];
+                next = heavy_start[i + 1];
+            }
+            const std::int64_t e = base + x;
+            const std::int32_t u = neighbors[e];
+            if (u == v) continue;
+            const auto w = static_cast<unsigned long long>(weights[e]);
+            if (update) {
+                atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
+                atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
+            }
+            const bool fu = flags[u] != 0;
+            if (!fu || u > v) {
+                const std::int32_t lu = labels[u];
+                const std::int32_t nu = fu ? targets[u] : lu;
+                gain += static_cast<long long>(w) * ((source != lu) - (target != nu));
+            }
+        }
+    }
+    gain = group_sum_i64<kWave>(gain);
+    if (lane_id() == 0) block_sum[threadIdx.x / kWave] = gain;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long sum = 0;
+        for (int w = 0; w < kTileLanes / kWave; ++w) sum += block_sum[w];
+        if (sum) atomicAdd(reinterpret_cast<unsigned long long*>(delta), static_cast<unsigned long long>(sum));
     }
 }
 
@@ -309,12 +531,12 @@ __global__ void round_status_kernel(RoundState* st, const unsigned long long* pa
 
 // Movers are split by degree so the cache update can give heavy rows a workgroup.
 struct MoverPair {
-    long long light, heavy;
+    long long light, heavy, edges;
 };
 struct MoverPairOp {
-    __device__ __forceinline__ MoverPair identity() const { return MoverPair{0, 0}; }
+    __device__ __forceinline__ MoverPair identity() const { return MoverPair{0, 0, 0}; }
     __device__ __forceinline__ MoverPair operator()(const MoverPair& a, const MoverPair& b) const {
-        return MoverPair{a.light + b.light, a.heavy + b.heavy};
+        return MoverPair{a.light + b.light, a.heavy + b.heavy, a.edges + b.edges};
     }
 };
 template <typename O>
@@ -322,9 +544,10 @@ struct MoverLoad {
     const std::int32_t* flags;
     const O* offsets;
     __device__ __forceinline__ MoverPair operator()(std::int64_t v) const {
-        if (!flags[v]) return MoverPair{0, 0};
-        const bool heavy = offsets[v + 1] - offsets[v] > kHeavyDegree;
-        return MoverPair{heavy ? 0 : 1, heavy ? 1 : 0};
+        if (!flags[v]) return MoverPair{0, 0, 0};
+        const long long degree = static_cast<long long>(offsets[v + 1] - offsets[v]);
+        const bool heavy = degree > kHeavyDegree;
+        return MoverPair{heavy ? 0 : 1, heavy ? 1 : 0, degree};
     }
 };
 struct MoverStore {
@@ -338,6 +561,7 @@ struct MoverStore {
         if (v == n - 1) {
             st->light = prefix.light + item.light;
             st->heavy = prefix.heavy + item.heavy;
+            st->edges = prefix.edges + item.edges;
         }
     }
 };
@@ -473,7 +697,11 @@ __device__ void append_movers(bool take, std::int32_t v, const O* offsets, std::
                               std::int32_t* heavy, std::int32_t* flags, RoundState* st, unsigned* counts,
                               unsigned long long* base) {
     const int lane = lane_id(), wave = threadIdx.x / kWave;
-    const bool is_heavy = take && offsets[v + 1] - offsets[v] > kHeavyDegree;
+    const long long degree = take ? static_cast<long long>(offsets[v + 1] - offsets[v]) : 0;
+    const bool is_heavy = degree > kHeavyDegree;
+    const long long edges = group_sum_i64<kWave>(degree);
+    if (lane == 0 && edges) atomicAdd(reinterpret_cast<unsigned long long*>(&st->edges),
+                                      static_cast<unsigned long long>(edges));
     const bool is_light = take && !is_heavy;
     const unsigned long long bl = __ballot(is_light), bh = __ballot(is_heavy);
     if (lane == 0) {
@@ -928,12 +1156,16 @@ struct Ops {
         fill_bytes(ctx, d.get(), 0, 8);
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
-        if (g->n > 0) {
-            // 16 lanes per vertex, capped grid, partials folded by one workgroup
-            const int blocks = static_cast<int>(std::min<std::int64_t>(grid_for(g->n * 16), kReduceMaxBlocks));
+        if (g->n > 0 && g->m > 0) {
+            // edge tiles, capped grid, partials folded by one workgroup
+            const std::int64_t tiles = tile_count(g->m);
+            Scratch<std::int32_t> tile_row(ctx, tiles);
+            hipLaunchKernelGGL(tile_rows_kernel<O>, dim3(grid_for(g->n)), dim3(kBlock), 0, ctx->stream, g->n, off,
+                               tile_row.get());
+            const int blocks = static_cast<int>(std::min<std::int64_t>(tiles, kReduceMaxBlocks));
             Scratch<unsigned long long> partials(ctx, blocks);
-            hipLaunchKernelGGL((cut_kernel<O, W, 16>), dim3(blocks), dim3(kBlock), 0, ctx->stream, g->n, nullptr, off,
-                               g->neighbors, w, labels, partials.get());
+            hipLaunchKernelGGL((cut_kernel<O, W>), dim3(blocks), dim3(kTileLanes), 0, ctx->stream, g->n, g->m, tiles,
+                               off, tile_row.get(), g->neighbors, w, labels, partials.get());
             hipLaunchKernelGGL((reduce_finish_kernel<unsigned long long, SumU64>), dim3(1), dim3(kReduceThreads), 0,
                                ctx->stream, partials.get(), blocks, SumU64{}, d.get());
             GPART_LAUNCH_CHECK();
@@ -943,22 +1175,39 @@ struct Ops {
         *out = directed / 2;
     }
 
+    // Writes every row of the cache. With a gate, the kernels run only when
+    // *gate > above (decided on the device, inside a round).
     static void build_cache(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
-                            const std::int32_t* labels, int k, C* cache) {
+                            const std::int32_t* labels, int k, C* cache, const long long* gate = nullptr,
+                            long long above = 0) {
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
-        profile_begin(ctx, GPART_PROFILE_REFINE_CACHE);
+        if (!gate) profile_begin(ctx, GPART_PROFILE_REFINE_CACHE);
+        auto grid = [](std::int64_t threads) {
+            return static_cast<unsigned>(std::min<std::int64_t>(grid_for(threads), kCacheBlocks));
+        };
         if (c->low_count)
-            hipLaunchKernelGGL((build_cache_kernel<O, W, 8, C>), dim3(grid_for(c->low_count * 8)), dim3(kBlock), 0,
-                               ctx->stream, c->low_count, c->low, off, g->neighbors, w, labels, k, cache);
+            hipLaunchKernelGGL((build_cache_kernel<O, W, 8, C>), dim3(grid(c->low_count * 8)), dim3(kBlock), 0,
+                               ctx->stream, c->low_count, c->low, off, g->neighbors, w, labels, k, cache, gate, above);
         if (c->medium_count)
-            hipLaunchKernelGGL((build_cache_kernel<O, W, 64, C>), dim3(grid_for(c->medium_count * 64)), dim3(kBlock), 0,
-                               ctx->stream, c->medium_count, c->medium, off, g->neighbors, w, labels, k, cache);
+            hipLaunchKernelGGL((build_cache_kernel<O, W, 64, C>), dim3(grid(c->medium_count * 64)), dim3(kBlock), 0,
+                               ctx->stream, c->medium_count, c->medium, off, g->neighbors, w, labels, k, cache, gate,
+                               above);
         if (c->high_count)
-            hipLaunchKernelGGL((build_cache_kernel<O, W, kBlock, C>), dim3(static_cast<unsigned>(c->high_count)),
+            hipLaunchKernelGGL((build_cache_kernel<O, W, kBlock, C>), dim3(grid(c->high_count * kBlock)),
                                dim3(kBlock), 0, ctx->stream, c->high_count, c->high, off, g->neighbors, w, labels, k,
-                               cache);
+                               cache, gate, above);
+        if (c->max_degree > kHubChunk) {
+            HubChunks chunks;
+            build_hub_chunks(ctx, off, c->high, c->high_count, c->high_edges, kHubChunk, chunks);
+            hipLaunchKernelGGL(hub_rows_zero_kernel<C>, dim3(grid_for(c->high_count)), dim3(kBlock), 0, ctx->stream,
+                               c->high_count, chunks.start.get(), c->high, k, cache, gate, above);
+            hipLaunchKernelGGL((build_cache_hub_kernel<O, W, C>), dim3(chunks.grid()), dim3(kBlock), 0, ctx->stream,
+                               chunks.total(), chunks.start.get(), chunks.hub.get(), c->high, off, g->neighbors, w,
+                               labels, k, cache, gate, above);
+        }
         GPART_LAUNCH_CHECK();
+        if (gate) return;
         const double n = static_cast<double>(g->n), m = static_cast<double>(g->m);
         profile_end(ctx, GPART_PROFILE_REFINE_CACHE,
                     (n + 1) * sizeof(O) + m * (4 + sizeof(W)) + 4 * m + static_cast<double>(sizeof(C)) * n * k);
@@ -981,12 +1230,29 @@ struct Ops {
             hipLaunchKernelGGL((priority_filter_kernel<O, W, kBlock>), dim3(static_cast<unsigned>(c->high_count)),
                                dim3(kBlock), 0, ctx->stream, c->high_count, c->high, off, g->neighbors, w, labels,
                                targets, gains, flags);
+        if (c->max_degree > kHubChunk) {
+            HubChunks chunks;
+            build_hub_chunks(ctx, off, c->high, c->high_count, c->high_edges, kHubChunk, chunks);
+            Scratch<long long> sums(ctx, c->high_count);
+            fill_bytes(ctx, sums.get(), 0, static_cast<std::size_t>(c->high_count) * 8);
+            hipLaunchKernelGGL((priority_filter_hub_kernel<O, W>), dim3(chunks.grid()), dim3(kBlock), 0, ctx->stream,
+                               chunks.total(), chunks.start.get(), chunks.hub.get(), c->high, off, g->neighbors, w,
+                               labels, targets, gains, sums.get());
+            hipLaunchKernelGGL(priority_filter_hub_finish_kernel, dim3(grid_for(c->high_count)), dim3(kBlock), 0,
+                               ctx->stream, c->high_count, chunks.start.get(), c->high, targets, gains, sums.get(),
+                               flags);
+        }
         GPART_LAUNCH_CHECK();
         const double nd = static_cast<double>(g->n), m = static_cast<double>(g->m);
         profile_end(ctx, GPART_PROFILE_REFINE_FILTER, (nd + 1) * sizeof(O) + m * (4 + sizeof(W)) + 16 * m + 20 * nd);
     }
 
     // incremental_update_cache_and_cut + incremental_commit over the round's movers.
+    // On levels of at least ctx->rebuild_min_entries entries the device chooses,
+    // from the round's mover entries, between atomic updates (two per mover entry)
+    // and a full rebuild of the cache after the commit (one gather per entry, no
+    // atomics, same cache); the cut delta is summed from the movers' entries
+    // either way.
     static void apply(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c, RoundState* st,
                       const std::int32_t* light, const std::int32_t* heavy, std::int32_t* labels,
                       const std::int32_t* targets, const std::int32_t* flags, int k, C* cache,
@@ -994,23 +1260,42 @@ struct Ops {
         const std::int64_t n = g->n;
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
+        const long long rebuild_above = rebuild_threshold(ctx, g);
         const auto light_blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * 8), 4 * kRoundBlocks));
         hipLaunchKernelGGL((update_cache_kernel<O, W, 8, C>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
-                           &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta);
-        const std::int64_t heavy_bound = c->medium_count + c->high_count;
-        if (heavy_bound > 0)
-            hipLaunchKernelGGL((update_cache_kernel<O, W, kBlock, C>),
-                               dim3(static_cast<unsigned>(std::min<std::int64_t>(heavy_bound, kRoundBlocks))),
-                               dim3(kBlock), 0, ctx->stream, &st->heavy, heavy, off, g->neighbors, w, labels, targets,
-                               flags, k, cache, &st->delta);
+                           &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta,
+                           &st->edges, rebuild_above);
+        if (c->max_degree > kHeavyDegree) {
+            const std::int64_t heavy_bound = c->medium_count + c->high_count;
+            Scratch<long long> heavy_start(ctx, heavy_bound + 1);
+            Scratch<std::int32_t> tile_mover(ctx, tile_count(g->m) + 1);
+            device_scan<long long, SumOp<long long>>(ctx, HeavyDegreeLoad<O>{st, heavy, off},
+                                                     ExclusiveStore<long long>{heavy_start.get()}, heavy_bound + 1,
+                                                     SumOp<long long>{});
+            hipLaunchKernelGGL(heavy_tiles_kernel,
+                               dim3(static_cast<unsigned>(std::min<std::int64_t>(grid_for(heavy_bound), kRoundBlocks))),
+                               dim3(kBlock), 0, ctx->stream, st, heavy_start.get(), tile_mover.get());
+            const auto tiles = static_cast<unsigned>(std::max<std::int64_t>(
+                1, std::min<std::int64_t>(tile_count(g->m), 2 * kRoundBlocks)));
+            hipLaunchKernelGGL((update_cache_tiles_kernel<O, W, C>), dim3(tiles), dim3(kTileLanes), 0, ctx->stream,
+                               st, heavy_start.get(), tile_mover.get(), heavy, off, g->neighbors, w, labels, targets,
+                               flags, k, cache, &st->delta, rebuild_above);
+        }
         hipLaunchKernelGGL(commit_kernel<W>,
                            dim3(static_cast<unsigned>(std::min<std::int64_t>(grid_for(n), kRoundBlocks))),
                            dim3(kBlock), 0, ctx->stream, st, light, heavy, labels, targets,
                            static_cast<const W*>(g->vertex_weights), part_weights, locks, lp, k);
         GPART_LAUNCH_CHECK();
+        if (rebuild_above >= 0) build_cache(ctx, g, c, labels, k, cache, &st->edges, rebuild_above);
     }
 
-    static void finish(gpart_ctx* ctx, RoundState* st, const unsigned long long* part_weights, int k,
+    // Mover entries above which a round rebuilds the cache; -1: never.
+    static long long rebuild_threshold(const gpart_ctx* ctx, const gpart_csr_view* g) {
+        return ctx->rebuild_share > 0 && g->m >= ctx->rebuild_min_entries ? g->m / ctx->rebuild_share : -1;
+    }
+
+    static void finish(gpart_ctx* ctx, const gpart_csr_view* g, RoundState* st, const unsigned long long* part_weights,
+                       int k,
                        gpart_refine_status* out) {
         hipLaunchKernelGGL(round_status_kernel, dim3(1), dim3(kWave), 0, ctx->stream, st, part_weights, k);
         GPART_LAUNCH_CHECK();
@@ -1021,6 +1306,9 @@ struct Ops {
         out->candidates = h.candidates;
         out->max_part_weight = h.max_weight;
         for (int p = 0; p < 32; ++p) out->part_weights[p] = p < k ? h.pw[p] : 0;
+        out->mover_entries = h.edges;
+        const long long rebuild_above = rebuild_threshold(ctx, g);
+        out->rebuilt = rebuild_above >= 0 && h.edges > rebuild_above ? 1 : 0;
     }
 
     // One LP round: proposals, priority filter, movers, apply; one synchronisation.
@@ -1049,7 +1337,7 @@ struct Ops {
                                             n, MoverPairOp{});
         apply(ctx, g, c, st.get(), light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, locks,
               true);
-        finish(ctx, st.get(), part_weights, k, out);
+        finish(ctx, g, st.get(), part_weights, k, out);
     }
 
     // One rebalance iteration: proposals, source and destination bucket quotas,
@@ -1122,7 +1410,7 @@ struct Ops {
         GPART_LAUNCH_CHECK();
         apply(ctx, g, c, st.get(), light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, nullptr,
               false);
-        finish(ctx, st.get(), part_weights, k, out);
+        finish(ctx, g, st.get(), part_weights, k, out);
         // per candidate: id, label, weight, cache row, target and bucket writes
         profile_add_bytes(ctx, GPART_PROFILE_REBALANCE,
                           static_cast<double>(out->candidates) * (16.0 + sizeof(W) + sizeof(C) * k));
@@ -1169,6 +1457,14 @@ using namespace gpart_hip;
     } while (0)
 
 extern "C" {
+
+int gpart_refine_set_rebuild(gpart_ctx* ctx, int64_t min_entries, int64_t share) {
+    return guarded([&] {
+        if (!ctx) throw Error(GPART_ERR_INVALID_ARGUMENT, "null context");
+        ctx->rebuild_min_entries = min_entries < 0 ? 0 : min_entries;
+        ctx->rebuild_share = share;
+    });
+}
 
 int gpart_cut(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, uint64_t* cut) {
     return guarded([&] { GPART_WIDTH_DISPATCH(g->width, OpsT::cut(ctx, g, labels, cut)); });

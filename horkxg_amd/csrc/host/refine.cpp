@@ -122,6 +122,8 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
             current_cut += static_cast<std::uint64_t>(-delta);
         }
         stats.accepted += static_cast<std::uint64_t>(status.movers);
+        stats.mover_entries += static_cast<std::uint64_t>(status.mover_entries);
+        stats.cache_rebuilds += status.rebuilt;
         level.host_part_weights.assign(status.part_weights, status.part_weights + k);
         if (options.strict_verify) {
             if (level.cut() != current_cut) throw std::runtime_error("incremental cut differs from CSR scan");

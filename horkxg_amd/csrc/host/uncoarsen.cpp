@@ -49,7 +49,8 @@ DeviceUncoarsenResult<Types> uncoarsen_impl(H& hierarchy, P&& coarsest_partition
                   << " workspace_resize_seconds=" << t.workspace_resize_seconds
                   << " projection_gpu_seconds=" << t.projection_gpu_seconds
                   << " refinement_seconds=" << t.refinement_seconds
-                  << " verification_seconds=" << t.verification_seconds << '\n';
+                  << " verification_seconds=" << t.verification_seconds << " mover_entries=" << s.mover_entries
+                  << " cache_rebuilds=" << s.cache_rebuilds << '\n';
     }
     const auto total = sum_weights(result.final_part_weights);
     const auto maximum = *std::max_element(result.final_part_weights.begin(), result.final_part_weights.end());

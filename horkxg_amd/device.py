@@ -22,6 +22,7 @@ def _bind():
     H.gpart_sort_pairs_u64.argtypes = [vp, vp, vp, C.c_int, i64, C.c_int, C.c_int]
     H.gpart_exclusive_scan_u64.argtypes = [vp, vp, vp, i64]
     H.gpart_compact_flags.argtypes = [vp, vp, i64, vp, vp]
+    H.gpart_contract_record_bounds.argtypes = [vp, i64, vp, vp, i64, vp, vp]
 
 
 _bind()
@@ -133,6 +134,17 @@ def contract(g: DeviceGraph, mapping, n_coarse: int, bound: int):
         if p.value:
             H.gpart_device_free(_ctx(), p)
     return out_off.astype(np.int64), out_nbr, out_w.astype(np.uint64)
+
+
+def contract_record_bounds(mapping, fine_bounds, n_coarse: int):
+    """Saturating sums of fine row bounds per coarse vertex, and their maximum."""
+    mp = DeviceArray(mapping, np.int32)
+    fb = DeviceArray(fine_bounds, np.uint64)
+    out = DeviceArray(dtype=np.uint64, count=max(n_coarse, 1))
+    mx = C.c_uint64()
+    _check(hip_lib().gpart_contract_record_bounds(_ctx(), mp.count, C.c_void_p(mp.ptr), C.c_void_p(fb.ptr), n_coarse,
+                                                  C.c_void_p(out.ptr), C.byref(mx)))
+    return out.get()[:n_coarse], mx.value
 
 
 def sort_pairs(keys, values=None, begin_bit: int = 0, end_bit: int = 64):

@@ -63,6 +63,12 @@ int gpart_release_cache(gpart_ctx* ctx);
  * the stream and trimming the pool (release_seconds[3]). */
 int gpart_cache_stats(gpart_ctx* ctx, int64_t* cached_blocks, uint64_t* cached_bytes, uint64_t* live_bytes,
                       int64_t* released_blocks, double* release_seconds);
+/* Tuning of the refinement rounds (no counterpart in the reference, whose rounds
+ * always update the cache with atomics): a round rebuilds the n x k cache from the
+ * CSR instead when the level has >= min_entries entries and its movers hold more
+ * than m / share of them; share <= 0 never rebuilds. Defaults 2^23 and 16. Results
+ * are identical either way. */
+int gpart_refine_set_rebuild(gpart_ctx* ctx, int64_t min_entries, int64_t share);
 int gpart_memcpy_htod(gpart_ctx* ctx, void* dst, const void* src, size_t bytes); /* host out: sync */
 int gpart_memcpy_dtoh(gpart_ctx* ctx, void* dst, const void* src, size_t bytes); /* host out: sync */
 int gpart_memcpy_dtod(gpart_ctx* ctx, void* dst, const void* src, size_t bytes);
@@ -282,6 +288,8 @@ typedef struct gpart_refine_status {
     int64_t candidates;          /* rebalance: vertices of overloaded parts */
     uint64_t max_part_weight;    /* after the round */
     uint64_t part_weights[32];   /* after the round; first k valid */
+    int64_t mover_entries;       /* CSR entries of the movers */
+    int32_t rebuilt;             /* 1: the cache was rebuilt instead of updated */
 } gpart_refine_status;
 
 /* One LP round of run_incremental_refinement (src/refine_incremental.cuh:338-352):

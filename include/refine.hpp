@@ -28,6 +28,8 @@ struct RefineStats {
     double seconds = 0.0;
     int admission_iterations = 0;
     int admission_fallbacks = 0;
+    std::uint64_t mover_entries = 0;  // CSR entries of all applied movers
+    int cache_rebuilds = 0;           // rounds that rebuilt the affinity cache instead of updating it
 };
 
 struct RefineLevelTimings {

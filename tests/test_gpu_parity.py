@@ -20,8 +20,10 @@ def need_gpu(hx):
         pytest.skip("no GPU")
 
 
-def hub_graph(n=6000, hub_degree=3000, seed=1):
-    """Ring + one hub of degree > 2048 (global hash path) + a medium hub."""
+def hub_graph(n=6000, hub_degrees=(3000, 600), seed=1):
+    """Ring + hubs: vertex i gets hub_degrees[i] random neighbours. Degrees above
+    1024 take the chunked global-table SCLP path, above 4096 several chunks (and
+    chunked cache rows in refinement)."""
     rng = np.random.default_rng(seed)
     rows = [set() for _ in range(n)]
 
@@ -32,10 +34,9 @@ def hub_graph(n=6000, hub_degree=3000, seed=1):
 
     for v in range(n):
         add(v, (v + 1) % n)
-    for u in rng.choice(np.arange(1, n), hub_degree, replace=False):
-        add(0, int(u))
-    for u in rng.choice(np.arange(2, n), 600, replace=False):
-        add(1, int(u))
+    for h, degree in enumerate(hub_degrees):
+        for u in rng.choice(np.arange(h + 1, n), degree, replace=False):
+            add(h, int(u))
     indptr = [0]
     indices = []
     for r in rows:
@@ -49,6 +50,7 @@ GRAPHS = {
     "grid256": lambda hx: hx.generate_grid2d(256, 256),
     "rmat12": lambda hx: hx.generate_rmat(12, 16),
     "hub": lambda hx: hub_graph(),
+    "bighub": lambda hx: hub_graph(30000, (20000, 9000, 5000, 1500, 700), seed=3),
 }
 
 
@@ -124,6 +126,32 @@ def test_contract_matches_oracle(hx, oracle, name, width):
         assert np.array_equal(w, expect[2])
 
 
+@pytest.mark.parametrize("scale", ["small", "saturating"])
+def test_contract_record_bounds(hx, scale):
+    """Per-cluster saturating sums of row bounds (src/coarsen.cu:72-85), with zero
+    bounds packed into one large cluster; closed form on Python integers."""
+    say(f"case contract record bounds scale={scale}")
+    rng = np.random.default_rng(11)
+    n, coarse = 200_000, 5000
+    mapping = rng.integers(0, coarse, n).astype(np.int32)
+    mapping[: n // 4] = 0  # a packed cluster of isolated vertices
+    if scale == "small":
+        bounds = rng.integers(0, 1000, n).astype(np.uint64)
+    else:
+        bounds = rng.integers(2**58, 2**62, n).astype(np.uint64)
+        bounds[rng.random(n) < 0.5] = 0
+    bounds[: n // 4] = 0
+    got, mx = hx.device.contract_record_bounds(mapping, bounds, coarse)
+    sums = [0] * coarse
+    for c, b in zip(mapping.tolist(), bounds.tolist()):
+        sums[c] += b
+    expect = np.array([min(x, 2**64 - 1) for x in sums], dtype=np.uint64)
+    assert np.array_equal(got, expect)
+    assert mx == int(expect.max())
+    if scale == "saturating":
+        assert (expect == np.uint64(2**64 - 1)).any()
+
+
 def compare_hierarchy(gh, oh):
     assert len(gh) == len(oh)
     for level in range(len(gh)):
@@ -136,7 +164,7 @@ def compare_hierarchy(gh, oh):
     assert gh.stop_reason == oh.stop_reason
 
 
-@pytest.mark.parametrize("name", ["grid256", "rmat12", "hub"])
+@pytest.mark.parametrize("name", ["grid256", "rmat12", "hub", "bighub"])
 @pytest.mark.parametrize("production", [False, True])
 def test_hierarchy_bit_exact(hx, oracle, name, production):
     say(f"case hierarchy graph={name} production={production}")
@@ -194,7 +222,7 @@ def test_refine_matches_oracle(hx, oracle, k):
     assert np.array_equal(gl, ol)
 
 
-@pytest.mark.parametrize("name,k", [("grid256", 4), ("grid256", 32), ("rmat12", 8), ("hub", 4)])
+@pytest.mark.parametrize("name,k", [("grid256", 4), ("grid256", 32), ("rmat12", 8), ("hub", 4), ("bighub", 8)])
 def test_partition_matches_oracle(hx, oracle, name, k):
     say(f"case partition graph={name} k={k}")
     indptr, indices = GRAPHS[name](hx)
@@ -207,3 +235,28 @@ def test_partition_matches_oracle(hx, oracle, name, k):
     assert oracle.cut(og, labels) == rep["final_cut"]
     assert rep["final_cut"] == orep.final_cut
     assert np.array_equal(labels, ol)
+
+
+@pytest.mark.parametrize("name,k", [("rmat12", 8), ("bighub", 4), ("grid256", 32)])
+def test_refine_rebuild_rounds_match_oracle(hx, oracle, name, k):
+    """Rounds that rebuild the affinity cache instead of updating it (forced on
+    every round here; by default only on levels of 2^23+ entries) give the same
+    labels; strict verification compares the cache with a CSR rebuild each round."""
+    say(f"case refine rebuild graph={name} k={k}")
+    indptr, indices = GRAPHS[name](hx)
+    n = len(indptr) - 1
+    labels = (np.arange(n) * 7919 % k).astype(np.int32)
+    og = oracle.Graph(indptr, indices.astype(np.int32))
+    ol, orep = oracle.refine(og, labels, k, 1.03, 32)
+    g = hx.Graph(indptr, indices)
+    try:
+        hx.set_refine_rebuild(0, 1 << 40)
+        gl, grep = g.refine(labels, k, 1.03, 32, verify=True)
+        pl, prep = g.partition(k, 1.03, 0)
+    finally:
+        hx.set_refine_rebuild()
+    assert grep["final_cut"] == orep.final_cut
+    assert np.array_equal(gl, ol)
+    pol, porep = oracle.partition(og, k, 1.03, 0)
+    assert prep["final_cut"] == porep.final_cut
+    assert np.array_equal(pl, pol)
