@@ -265,10 +265,11 @@ __global__ void __launch_bounds__(kBlock) sclp_medium_kernel(std::int64_t count,
     if (active && lane == 0) decode(v, source, current, best, best_tie, r, proposals, gains, ties);
 }
 
-// Folds each wave's (current, best, tie) into thread 0 of the workgroup.
+// Folds each wave's (current, best, tie) into thread 0 of a Threads-wide workgroup.
+template <int Threads = kBlock>
 __device__ __forceinline__ void block_best(unsigned long long& current, unsigned long long& best,
                                            unsigned long long& best_tie) {
-    __shared__ unsigned long long red_current[kBlock / kWave], red_best[kBlock / kWave], red_tie[kBlock / kWave];
+    __shared__ unsigned long long red_current[Threads / kWave], red_best[Threads / kWave], red_tie[Threads / kWave];
     current = group_max_u64<kWave>(current);
     group_best<kWave>(best, best_tie);
     const int wave = threadIdx.x / kWave;
@@ -279,7 +280,7 @@ __device__ __forceinline__ void block_best(unsigned long long& current, unsigned
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kBlock / kWave; ++w) {
+        for (int w = 1; w < Threads / kWave; ++w) {
             current = red_current[w] > current ? red_current[w] : current;
             if (better(red_best[w], red_tie[w], best, best_tie)) {
                 best = red_best[w];
@@ -508,16 +509,129 @@ __global__ void sclp_hub_decode_kernel(std::int64_t count, const std::int64_t* c
     decode(v, clusters[v], current, best, best_tie, r, proposals, gains, ties);
 }
 
+// Rows of kHighLdsDegree < degree <= kPartMaxDegree outside singleton rounds:
+// work item (v, p) of P = 2^j >= degree / kPartKeys items sums, in an LDS table,
+// the clusters t of v's neighbours with (mix32(t) & (P - 1)) == p, and leaves a
+// partial (current, best, tie). mix32 is a bijection, so the clusters of a row
+// split evenly; a pass that still meets more than kPartCap distinct clusters is
+// redone as 2, 4, ... finer passes (a finer split always separates distinct keys).
+// No global table, fill or atomic: each item reads the row once per pass.
+constexpr int kPartThreads = 512;
+constexpr int kPartSlots = 8192;
+constexpr std::int64_t kPartKeys = 4096;
+constexpr std::int64_t kPartMaxDegree = 32768;
+constexpr int kPartCap = kPartSlots - kPartThreads - 1;  // keys held + one in flight per thread < slots
+constexpr int kPartLaunchCap = 1024;
+
+template <typename O>
+struct PartLoad {
+    const std::int32_t* high;
+    const O* offsets;
+    std::int64_t count;
+    __device__ __forceinline__ std::int64_t operator()(std::int64_t i) const {
+        if (i >= count) return 0;
+        const std::int32_t v = high[i];
+        const std::int64_t d = static_cast<std::int64_t>(offsets[v + 1] - offsets[v]);
+        if (d <= kHighLdsDegree || d > kPartMaxDegree) return 0;
+        std::int64_t parts = 1;
+        while (parts * kPartKeys < d) parts <<= 1;
+        return parts;
+    }
+};
+
+template <typename O, typename W, typename V>
+__global__ void __launch_bounds__(kPartThreads) sclp_part_kernel(const std::int64_t* item_total,
+                                                                 const std::int64_t* item_start,
+                                                                 const std::int32_t* item_vertex,
+                                                                 const std::int32_t* list, const O* offsets,
+                                                                 const std::int32_t* neighbors, const W* weights,
+                                                                 const std::int32_t* clusters, RoundParams r,
+                                                                 unsigned long long* partial_current,
+                                                                 unsigned long long* partial_best,
+                                                                 unsigned long long* partial_tie) {
+    __shared__ std::int32_t keys[kPartSlots];
+    __shared__ V values[kPartSlots];
+    __shared__ int held;
+    const std::int64_t items = *item_total;
+    for (std::int64_t w = blockIdx.x; w < items; w += gridDim.x) {
+        const std::int32_t i = item_vertex[w];
+        const std::int32_t v = list[i];
+        const std::int64_t first = item_start[i];
+        const auto parts = static_cast<std::uint32_t>(item_start[i + 1] - first);
+        const auto p = static_cast<std::uint32_t>(w - first);
+        const std::int64_t begin = offsets[v], end = offsets[v + 1];
+        const std::int32_t source = clusters[v];
+        int slots = kPartSlots;
+        if (parts == 1) {  // degree <= kPartKeys: 2 * degree slots hold every key
+            slots = 1;
+            while (slots < 2 * (end - begin)) slots <<= 1;
+        }
+        const int slot_mask = slots - 1;
+        unsigned long long current = 0, best = 0, best_tie = 0;
+        for (std::uint32_t split = 1;; split <<= 1) {
+            current = best = best_tie = 0;
+            bool overflow = false;
+            const std::uint32_t mask = parts * split - 1;
+            for (std::uint32_t q = 0; q < split; ++q) {
+                for (int s = threadIdx.x; s < slots; s += kPartThreads) {
+                    keys[s] = -1;
+                    values[s] = 0;
+                }
+                if (threadIdx.x == 0) held = 0;
+                __syncthreads();
+                const std::uint32_t select = p + parts * q;
+                for (std::int64_t e = begin + threadIdx.x; e < end; e += kPartThreads) {
+                    const std::int32_t u = neighbors[e];
+                    if (u == v) continue;
+                    const std::int32_t t = clusters[u];
+                    const std::uint32_t h = mix32(static_cast<std::uint32_t>(t));
+                    if ((h & mask) != select) continue;
+                    if (__hip_atomic_load(&held, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= kPartCap)
+                        continue;  // this pass is redone finer
+                    const auto wt = static_cast<V>(weights[e]);
+                    int slot = static_cast<int>(mix32(h ^ 0x9e3779b9U)) & slot_mask;
+                    while (true) {
+                        const std::int32_t found = atomicCAS(keys + slot, -1, t);
+                        if (found == -1 || found == t) {
+                            if (found == -1) atomicAdd(&held, 1);
+                            atomicAdd(values + slot, wt);
+                            break;
+                        }
+                        slot = (slot + 1) & slot_mask;
+                    }
+                }
+                __syncthreads();
+                overflow = held >= kPartCap;
+                if (!overflow)
+                    for (int s = threadIdx.x; s < slots; s += kPartThreads) {
+                        const std::int32_t t = keys[s];
+                        if (t < 0) continue;
+                        offer(v, t, source, static_cast<unsigned long long>(values[s]), r, current, best, best_tie);
+                    }
+                __syncthreads();
+                if (overflow) break;
+            }
+            if (!overflow) break;
+        }
+        block_best<kPartThreads>(current, best, best_tie);
+        if (threadIdx.x == 0) {
+            partial_current[w] = current;
+            partial_best[w] = best;
+            partial_tie[w] = best_tie;
+        }
+    }
+}
+
 template <typename O>
 __global__ void hub_table_size_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
-                                      std::int64_t* sizes) {
+                                      std::int64_t min_degree, std::int64_t* sizes) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i > count) return;
     std::int64_t size = 0;
     if (i < count) {
         const std::int32_t v = list[i];
         const std::int64_t deg = offsets[v + 1] - offsets[v];
-        if (deg > kHighLdsDegree) {
+        if (deg > min_degree) {
             size = 1;
             while (size < 2 * deg) size <<= 1;
         }
@@ -869,14 +983,26 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
     const auto* w = static_cast<const W*>(g->edge_weights);
     RoundParams r{round->role_salt, round->tie_salt, round->mover_threshold, round->filter_roles != 0};
     const bool single = round->singleton != 0;
-    // hubs (degree > kHighLdsDegree) in chunks; global tables outside singleton rounds
-    const bool hubs = c->high_count > 0 && c->max_degree > kHighLdsDegree;
-    HubChunks chunks;
+    // Rows above kHighLdsDegree: singleton rounds offer every entry, in chunks.
+    // Otherwise rows up to kPartMaxDegree go to the partitioned LDS tables and
+    // larger hubs, in chunks, to per-hub global tables.
+    const bool parted = !single && c->high_count > 0 && c->max_degree > kHighLdsDegree;
+    const std::int64_t hub_degree = single ? kHighLdsDegree : kPartMaxDegree;
+    const bool hubs = c->high_count > 0 && c->max_degree > hub_degree;
+    HubChunks chunks, items;
     Scratch<std::int64_t> table_offset;
     Scratch<std::int32_t> gkeys;
-    Scratch<unsigned long long> gvalues, pcurrent, pbest, ptie;
+    Scratch<unsigned long long> gvalues, pcurrent, pbest, ptie, icurrent, ibest, itie;
+    if (parted) {
+        // sum over rows of 2^ceil(log2(ceil(d / kPartKeys))) <= 2 * (entries / kPartKeys + rows)
+        const std::int64_t bound = 2 * (c->high_edges / kPartKeys + 1) + 2 * c->high_count;
+        build_work_items(ctx, PartLoad<O>{c->high, off, c->high_count}, c->high_count, bound, items);
+        icurrent.reset(ctx, items.bound);
+        ibest.reset(ctx, items.bound);
+        itie.reset(ctx, items.bound);
+    }
     if (hubs) {
-        build_hub_chunks(ctx, off, c->high, c->high_count, c->high_edges, kHighLdsDegree, chunks);
+        build_hub_chunks(ctx, off, c->high, c->high_count, c->high_edges, hub_degree, chunks);
         pcurrent.reset(ctx, chunks.bound);
         pbest.reset(ctx, chunks.bound);
         ptie.reset(ctx, chunks.bound);
@@ -884,7 +1010,7 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
     if (hubs && !single) {
         table_offset.reset(ctx, c->high_count + 1);
         hipLaunchKernelGGL(hub_table_size_kernel<O>, dim3(grid_for(c->high_count + 1)), dim3(kBlock), 0, ctx->stream,
-                           c->high_count, c->high, off, table_offset.get());
+                           c->high_count, c->high, off, hub_degree, table_offset.get());
         GPART_LAUNCH_CHECK();
         exclusive_scan<std::int64_t>(ctx, table_offset.get(), table_offset.get(), c->high_count + 1);
         // the last entry (size 0 appended by the scan of count+1 values) holds the total
@@ -923,6 +1049,22 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
         else
             hipLaunchKernelGGL((sclp_high_kernel<O, W, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream,
                                c->high_count, c->high, off, g->neighbors, w, clusters, r, proposals, gains, ties);
+        GPART_LAUNCH_CHECK();
+    }
+    if (parted) {
+        const unsigned grid = static_cast<unsigned>(std::min<std::int64_t>(items.bound, kPartLaunchCap));
+        // connection sums fit 32 bits when every row weight does
+        if (round->value_bound != 0 && round->value_bound <= 0xffffffffULL)
+            hipLaunchKernelGGL((sclp_part_kernel<O, W, unsigned>), dim3(grid), dim3(kPartThreads), 0, ctx->stream,
+                               items.total(), items.start.get(), items.hub.get(), c->high, off, g->neighbors, w,
+                               clusters, r, icurrent.get(), ibest.get(), itie.get());
+        else
+            hipLaunchKernelGGL((sclp_part_kernel<O, W, unsigned long long>), dim3(grid), dim3(kPartThreads), 0,
+                               ctx->stream, items.total(), items.start.get(), items.hub.get(), c->high, off,
+                               g->neighbors, w, clusters, r, icurrent.get(), ibest.get(), itie.get());
+        hipLaunchKernelGGL(sclp_hub_decode_kernel, dim3(grid_for(c->high_count)), dim3(kBlock), 0, ctx->stream,
+                           c->high_count, items.start.get(), c->high, clusters, r, icurrent.get(), ibest.get(),
+                           itie.get(), proposals, gains, ties);
         GPART_LAUNCH_CHECK();
     }
     if (hubs) {

@@ -53,22 +53,30 @@ struct HubChunks {
     }
 };
 
+// Work items of the listed vertices: load(i) items for entry i (load(count) = 0),
+// at most `bound` in total.
+template <typename Load>
+void build_work_items(gpart_ctx* ctx, Load load, std::int64_t count, std::int64_t bound, HubChunks& out) {
+    out.count = count;
+    out.bound = bound;
+    out.start.reset(ctx, count + 1);
+    out.hub.reset(ctx, bound);
+    device_scan<std::int64_t, SumOp<std::int64_t>>(ctx, load, ExclusiveStore<std::int64_t>{out.start.get()},
+                                                   count + 1, SumOp<std::int64_t>{});
+    if (count > 0)
+        hipLaunchKernelGGL(hub_chunk_map_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0,
+                           ctx->stream, count, out.start.get(), out.hub.get());
+    GPART_LAUNCH_CHECK();
+}
+
 // high_edges bounds the entries of the listed vertices (their degree sum).
 template <typename O>
 void build_hub_chunks(gpart_ctx* ctx, const O* offsets, const std::int32_t* high, std::int64_t high_count,
                       std::int64_t high_edges, std::int64_t min_degree, HubChunks& out) {
-    out.count = high_count;
     // sum of ceil(d / C) over d > min_degree: at most edges / C plus one per hub
-    out.bound = high_edges / kHubChunk + std::min<std::int64_t>(high_count, high_edges / (min_degree + 1)) + 1;
-    out.start.reset(ctx, high_count + 1);
-    out.hub.reset(ctx, out.bound);
-    device_scan<std::int64_t, SumOp<std::int64_t>>(ctx, HubChunkLoad<O>{high, offsets, high_count, min_degree},
-                                                   ExclusiveStore<std::int64_t>{out.start.get()}, high_count + 1,
-                                                   SumOp<std::int64_t>{});
-    if (high_count > 0)
-        hipLaunchKernelGGL(hub_chunk_map_kernel, dim3(static_cast<unsigned>((high_count + 255) / 256)), dim3(256), 0,
-                           ctx->stream, high_count, out.start.get(), out.hub.get());
-    GPART_LAUNCH_CHECK();
+    const std::int64_t bound =
+        high_edges / kHubChunk + std::min<std::int64_t>(high_count, high_edges / (min_degree + 1)) + 1;
+    build_work_items(ctx, HubChunkLoad<O>{high, offsets, high_count, min_degree}, high_count, bound, out);
 }
 
 // Entry range of chunk c: hub index *i, vertex *v, entries [*begin, *end).

@@ -104,18 +104,36 @@ __global__ void project_kernel(std::int64_t n, const std::int32_t* map, const st
 
 // ---------------------------------------------------------------- cache
 
+// Adds the workgroup's sum of v to *out with one atomic. Called by every thread
+// of a kBlock-thread workgroup; waves holds kBlock / kWave entries.
+__device__ __forceinline__ void block_add_u64(unsigned long long v, unsigned long long* waves,
+                                              unsigned long long* out) {
+    v = group_sum_u64<kWave>(v);
+    if (lane_id() == 0) waves[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long total = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) total += waves[w];
+        if (total) atomicAdd(out, total);
+    }
+}
+
 template <typename O, typename W, int G, typename C>
 __global__ void __launch_bounds__(kBlock) build_cache_kernel(std::int64_t count, const std::int32_t* list,
                                                              const O* offsets, const std::int32_t* neighbors,
                                                              const W* weights, const std::int32_t* labels, int k,
-                                                             C* cache, const long long* gate, long long above) {
+                                                             C* cache, const long long* gate, long long above,
+                                                             unsigned long long* cut) {
     // gate: a rebuild inside a round runs only when the round's mover entries exceed
-    // `above`; grid-stride over groups so that a gated-off launch is a small grid
+    // `above`; grid-stride over groups so that a gated-off launch is a small grid.
+    // cut (optional) receives the directed cut weight of `labels`.
     if (gate && *gate <= above) return;
     constexpr int kGroups = kBlock / G;
     __shared__ unsigned long long rows[kGroups][32];
+    __shared__ unsigned long long wave_cut[kBlock / kWave];
     const int local_group = threadIdx.x / G;
     const int sub = threadIdx.x & (G - 1);
+    unsigned long long cut_local = 0;
     for (std::int64_t first = static_cast<std::int64_t>(blockIdx.x) * kGroups; first < count;
          first += static_cast<std::int64_t>(gridDim.x) * kGroups) {
         const std::int64_t group = first + local_group;
@@ -127,11 +145,17 @@ __global__ void __launch_bounds__(kBlock) build_cache_kernel(std::int64_t count,
             v = list ? list[group] : static_cast<std::int32_t>(group);
             // rows above kHubChunk entries are hubs, built by build_cache_hub_kernel
             own = G < kBlock || offsets[v + 1] - offsets[v] <= kHubChunk;
-            if (own)
+            if (own) {
+                const std::int32_t lv = labels[v];
                 for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G) {
                     const std::int32_t u = neighbors[e];
-                    if (u != v) atomicAdd(&rows[local_group][labels[u]], static_cast<unsigned long long>(weights[e]));
+                    if (u == v) continue;
+                    const std::int32_t lu = labels[u];
+                    const auto w = static_cast<unsigned long long>(weights[e]);
+                    atomicAdd(&rows[local_group][lu], w);
+                    if (lu != lv) cut_local += w;
                 }
+            }
         }
         __syncthreads();
         if (own)
@@ -139,6 +163,7 @@ __global__ void __launch_bounds__(kBlock) build_cache_kernel(std::int64_t count,
                 cache[static_cast<std::int64_t>(v) * k + p] = static_cast<C>(rows[local_group][p]);
         __syncthreads();
     }
+    if (cut) block_add_u64(cut_local, wave_cut, cut);
 }
 
 template <typename C>
@@ -160,9 +185,12 @@ __global__ void __launch_bounds__(kBlock) build_cache_hub_kernel(const std::int6
                                                                  const std::int32_t* list, const O* offsets,
                                                                  const std::int32_t* neighbors, const W* weights,
                                                                  const std::int32_t* labels, int k, C* cache,
-                                                                 const long long* gate, long long above) {
+                                                                 const long long* gate, long long above,
+                                                                 unsigned long long* cut) {
     __shared__ unsigned long long row[32];
+    __shared__ unsigned long long wave_cut[kBlock / kWave];
     if (gate && *gate <= above) return;
+    unsigned long long cut_local = 0;
     const std::int64_t chunks = *chunk_total;
     for (std::int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
         if (threadIdx.x < 32) row[threadIdx.x] = 0;
@@ -170,15 +198,21 @@ __global__ void __launch_bounds__(kBlock) build_cache_hub_kernel(const std::int6
         std::int32_t i, v;
         std::int64_t begin, end;
         hub_chunk_range(offsets, list, chunk_start, chunk_hub, c, &i, &v, &begin, &end);
+        const std::int32_t lv = labels[v];
         for (std::int64_t e = begin + threadIdx.x; e < end; e += kBlock) {
             const std::int32_t u = neighbors[e];
-            if (u != v) atomicAdd(&row[labels[u]], static_cast<unsigned long long>(weights[e]));
+            if (u == v) continue;
+            const std::int32_t lu = labels[u];
+            const auto w = static_cast<unsigned long long>(weights[e]);
+            atomicAdd(&row[lu], w);
+            if (lu != lv) cut_local += w;
         }
         __syncthreads();
         if (threadIdx.x < k && row[threadIdx.x])
             atomicAdd(cache + static_cast<std::int64_t>(v) * k + threadIdx.x, static_cast<C>(row[threadIdx.x]));
         __syncthreads();
     }
+    if (cut) block_add_u64(cut_local, wave_cut, cut);
 }
 
 // Rows of the n x k cache are read by groups of G lanes, lane j owning the
@@ -352,7 +386,8 @@ struct RoundState {
     long long candidates;  // rebalance: vertices of overloaded parts
     unsigned long long max_weight;
     long long edges;       // CSR entries of all movers (light + heavy)
-    unsigned long long pad[2];
+    unsigned long long cut_directed;  // rebuild rounds: directed cut weight after the round
+    unsigned long long pad[1];
     unsigned long long pw[32];
 };
 
@@ -370,9 +405,9 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* c
                                                               C* cache, long long* delta,
                                                               const long long* edges_ptr, long long rebuild_above) {
     __shared__ long long block_sum[kBlock / kWave];
+    // a rebuild round recomputes cache and cut from the CSR after the commit
+    if (rebuild_above >= 0 && *edges_ptr > rebuild_above) return;
     const long long count = *count_ptr;
-    // a rebuilt cache replaces the atomics; the cut delta is still summed here
-    const bool update = rebuild_above < 0 || *edges_ptr <= rebuild_above;
     const int sub = threadIdx.x & (G - 1);
     const long long groups = static_cast<long long>(gridDim.x) * (kBlock / G);
     long long gain = 0;
@@ -385,11 +420,9 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* c
             const std::int32_t u = neighbors[e];
             if (u == v) continue;
             const auto w = static_cast<unsigned long long>(weights[e]);
-            if (update) {
-                // modular arithmetic: entries fit C at every point of the round
-                atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
-                atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
-            }
+            // modular arithmetic: entries fit C at every point of the round
+            atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
+            atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
             const bool fu = flags[u] != 0;
             if (!fu || u > v) {
                 const std::int32_t lu = labels[u];
@@ -441,10 +474,10 @@ __global__ void __launch_bounds__(kTileLanes) update_cache_tiles_kernel(
     const std::int32_t* targets, const std::int32_t* flags, int k, C* cache, long long* delta,
     long long rebuild_above) {
     __shared__ long long block_sum[kTileLanes / kWave];
+    if (rebuild_above >= 0 && st->edges > rebuild_above) return;  // a rebuild round
     const long long count = st->heavy;
     const long long total = heavy_start[count];
     const long long tiles = tile_count(total);
-    const bool update = rebuild_above < 0 || st->edges <= rebuild_above;
     long long gain = 0;
     for (long long t = blockIdx.x; t < tiles; t += gridDim.x) {
         std::int64_t x, stop;
@@ -467,10 +500,8 @@ __global__ void __launch_bounds__(kTileLanes) update_cache_tiles_kernel(
             const std::int32_t u = neighbors[e];
             if (u == v) continue;
             const auto w = static_cast<unsigned long long>(weights[e]);
-            if (update) {
-                atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
-                atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
-            }
+            atomicAdd(cache + static_cast<std::int64_t>(u) * k + source, static_cast<C>(0ULL - w));
+            atomicAdd(cache + static_cast<std::int64_t>(u) * k + target, static_cast<C>(w));
             const bool fu = flags[u] != 0;
             if (!fu || u > v) {
                 const std::int32_t lu = labels[u];
@@ -1176,10 +1207,11 @@ struct Ops {
     }
 
     // Writes every row of the cache. With a gate, the kernels run only when
-    // *gate > above (decided on the device, inside a round).
+    // *gate > above (decided on the device, inside a round); cut, when given,
+    // receives the directed cut weight of `labels`.
     static void build_cache(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c,
                             const std::int32_t* labels, int k, C* cache, const long long* gate = nullptr,
-                            long long above = 0) {
+                            long long above = 0, unsigned long long* cut = nullptr) {
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
         if (!gate) profile_begin(ctx, GPART_PROFILE_REFINE_CACHE);
@@ -1188,15 +1220,16 @@ struct Ops {
         };
         if (c->low_count)
             hipLaunchKernelGGL((build_cache_kernel<O, W, 8, C>), dim3(grid(c->low_count * 8)), dim3(kBlock), 0,
-                               ctx->stream, c->low_count, c->low, off, g->neighbors, w, labels, k, cache, gate, above);
+                               ctx->stream, c->low_count, c->low, off, g->neighbors, w, labels, k, cache, gate, above,
+                               cut);
         if (c->medium_count)
             hipLaunchKernelGGL((build_cache_kernel<O, W, 64, C>), dim3(grid(c->medium_count * 64)), dim3(kBlock), 0,
                                ctx->stream, c->medium_count, c->medium, off, g->neighbors, w, labels, k, cache, gate,
-                               above);
+                               above, cut);
         if (c->high_count)
             hipLaunchKernelGGL((build_cache_kernel<O, W, kBlock, C>), dim3(grid(c->high_count * kBlock)),
                                dim3(kBlock), 0, ctx->stream, c->high_count, c->high, off, g->neighbors, w, labels, k,
-                               cache, gate, above);
+                               cache, gate, above, cut);
         if (c->max_degree > kHubChunk) {
             HubChunks chunks;
             build_hub_chunks(ctx, off, c->high, c->high_count, c->high_edges, kHubChunk, chunks);
@@ -1204,7 +1237,7 @@ struct Ops {
                                c->high_count, chunks.start.get(), c->high, k, cache, gate, above);
             hipLaunchKernelGGL((build_cache_hub_kernel<O, W, C>), dim3(chunks.grid()), dim3(kBlock), 0, ctx->stream,
                                chunks.total(), chunks.start.get(), chunks.hub.get(), c->high, off, g->neighbors, w,
-                               labels, k, cache, gate, above);
+                               labels, k, cache, gate, above, cut);
         }
         GPART_LAUNCH_CHECK();
         if (gate) return;
@@ -1286,7 +1319,7 @@ struct Ops {
                            dim3(kBlock), 0, ctx->stream, st, light, heavy, labels, targets,
                            static_cast<const W*>(g->vertex_weights), part_weights, locks, lp, k);
         GPART_LAUNCH_CHECK();
-        if (rebuild_above >= 0) build_cache(ctx, g, c, labels, k, cache, &st->edges, rebuild_above);
+        if (rebuild_above >= 0) build_cache(ctx, g, c, labels, k, cache, &st->edges, rebuild_above, &st->cut_directed);
     }
 
     // Mover entries above which a round rebuilds the cache; -1: never.
@@ -1309,6 +1342,12 @@ struct Ops {
         out->mover_entries = h.edges;
         const long long rebuild_above = rebuild_threshold(ctx, g);
         out->rebuilt = rebuild_above >= 0 && h.edges > rebuild_above ? 1 : 0;
+        out->cut_after = 0;
+        if (out->rebuilt) {
+            if (h.cut_directed & 1ULL) throw Error(GPART_ERR_RUNTIME, "refinement cut is not symmetric");
+            out->cut_after = h.cut_directed / 2;
+            out->cut_delta = 0;
+        }
     }
 
     // One LP round: proposals, priority filter, movers, apply; one synchronisation.

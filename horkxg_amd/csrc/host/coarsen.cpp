@@ -73,7 +73,7 @@ using clock_type = std::chrono::steady_clock;
 double since(clock_type::time_point t) { return std::chrono::duration<double>(clock_type::now() - t).count(); }
 
 gpart_sclp_round make_round(std::uint32_t seed, int level, int round, bool filter, std::uint32_t threshold,
-                            bool singleton) {
+                            bool singleton, std::uint64_t value_bound) {
     gpart_sclp_round r;
     r.role_salt = seed ^ (static_cast<std::uint32_t>(level) * 0x9e3779b9U) ^
                   (static_cast<std::uint32_t>(round) * 0x85ebca6bU) ^ 0x27d4eb2dU;
@@ -82,6 +82,7 @@ gpart_sclp_round make_round(std::uint32_t seed, int level, int round, bool filte
     r.mover_threshold = threshold;
     r.filter_roles = filter ? 1 : 0;
     r.singleton = singleton ? 1 : 0;
+    r.value_bound = value_bound;
     return r;
 }
 
@@ -148,7 +149,8 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
             std::min(0.50, static_cast<double>(removable) / static_cast<double>(cluster_count));
         const auto threshold = static_cast<std::uint32_t>(mover_fraction * 4294967296.0);
         const auto affinity_start = clock_type::now();
-        const auto r = make_round(seed, level, round, true, threshold, round == 0 && ws.sorted_unique_rows);
+        const auto r =
+            make_round(seed, level, round, true, threshold, round == 0 && ws.sorted_unique_rows, gain_bound);
         GPART_CHECK(gpart_sclp_proposals(ctx, &view, clusters.data(), &classes, &r, proposals.data(), gains.data(),
                                          ties.data()));
         stats.affinity_seconds += since(affinity_start);
@@ -195,7 +197,7 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
     if (two_hop_triggered) {
         const auto two_hop_start = clock_type::now();
         const auto cut_before = diagnostics ? cluster_cut(graph, clusters) : 0;
-        const auto r = make_round(seed, level, 4, false, 0, false);
+        const auto r = make_round(seed, level, 4, false, 0, false, gain_bound);
         GPART_CHECK(gpart_sclp_proposals(ctx, &view, clusters.data(), &classes, &r, proposals.data(), gains.data(),
                                          ties.data()));
         std::int64_t candidates = 0, pairable = 0, merged = 0, nonempty = 0;

@@ -115,7 +115,9 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     // Book-keeping of an applied round (the reference's apply, :280-301).
     auto absorb = [&](const gpart_refine_status& status) {
         const std::int64_t delta = status.cut_delta;
-        if (delta >= 0) {
+        if (status.rebuilt) {
+            current_cut = status.cut_after;
+        } else if (delta >= 0) {
             if (static_cast<std::uint64_t>(delta) > current_cut) throw std::runtime_error("incremental cut underflow");
             current_cut -= static_cast<std::uint64_t>(delta);
         } else {

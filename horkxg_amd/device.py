@@ -70,7 +70,7 @@ class DegreeClasses(C.Structure):
 
 class SclpRound(C.Structure):
     _fields_ = [("role_salt", C.c_uint32), ("tie_salt", C.c_uint32), ("mover_threshold", C.c_uint32),
-                ("filter_roles", C.c_int32), ("singleton", C.c_int32)]
+                ("filter_roles", C.c_int32), ("singleton", C.c_int32), ("value_bound", C.c_uint64)]
 
 
 class DeviceGraph:
@@ -101,9 +101,10 @@ def salts(seed: int, level: int, rnd: int):
 
 
 def sclp_proposals(g: DeviceGraph, clusters, seed: int, level: int, rnd: int, filter_roles: bool,
-                   mover_threshold: int, singleton: bool = False):
+                   mover_threshold: int, singleton: bool = False, value_bound: int = 0):
+    """value_bound: a bound on every row weight (0: unknown)."""
     role, tie = salts(seed, level, rnd)
-    r = SclpRound(role, tie, mover_threshold, int(filter_roles), int(singleton))
+    r = SclpRound(role, tie, mover_threshold, int(filter_roles), int(singleton), value_bound)
     cl = DeviceArray(clusters, np.int32)
     prop = DeviceArray(dtype=np.int32, count=g.n)
     gains = DeviceArray(dtype=np.uint64, count=g.n)

@@ -158,6 +158,8 @@ typedef struct gpart_sclp_round {
     uint32_t mover_threshold;  /* :1311-1315 */
     int32_t filter_roles;
     int32_t singleton;         /* round 0 on sorted duplicate-free rows (:1161) */
+    uint64_t value_bound;      /* bound on every row weight (0: unknown); a bound below
+                                  2^32 lets the affinity tables keep 32-bit sums */
 } gpart_sclp_round;
 
 /* Affinity + decode of one LP round, fused: sclp_singleton_affinity_kernel,
@@ -289,7 +291,10 @@ typedef struct gpart_refine_status {
     uint64_t max_part_weight;    /* after the round */
     uint64_t part_weights[32];   /* after the round; first k valid */
     int64_t mover_entries;       /* CSR entries of the movers */
-    int32_t rebuilt;             /* 1: the cache was rebuilt instead of updated */
+    int32_t rebuilt;             /* 1: cache and cut were recomputed from the CSR after
+                                    the commit (gpart_refine_set_rebuild); cut_delta is
+                                    then 0 and cut_after holds the cut */
+    uint64_t cut_after;          /* rebuilt rounds: the cut after the round */
 } gpart_refine_status;
 
 /* One LP round of run_incremental_refinement (src/refine_incremental.cuh:338-352):

@@ -50,7 +50,7 @@ GRAPHS = {
     "grid256": lambda hx: hx.generate_grid2d(256, 256),
     "rmat12": lambda hx: hx.generate_rmat(12, 16),
     "hub": lambda hx: hub_graph(),
-    "bighub": lambda hx: hub_graph(30000, (20000, 9000, 5000, 1500, 700), seed=3),
+    "bighub": lambda hx: hub_graph(60000, (45000, 20000, 9000, 5000, 1500, 700), seed=3),
 }
 
 
@@ -93,6 +93,8 @@ def test_sclp_proposals_match_oracle(hx, oracle, name):
     ew = ((lo * 2654435761 + hi * 40503) % 5 + 1).astype(np.uint64)
     dg = hx.device.DeviceGraph(indptr, indices, ew)
     og = oracle.Graph(indptr, indices.astype(np.int32), ew)
+    sums = np.concatenate([[0], np.cumsum(ew.astype(np.uint64))]).astype(np.uint64)
+    row_bound = max(1, int((sums[indptr[1:]] - sums[indptr[:-1]]).max()))
     for clusters_kind in ("identity", "random", "blocks"):
         if clusters_kind == "identity":
             clusters = np.arange(n, dtype=np.int32)
@@ -102,11 +104,54 @@ def test_sclp_proposals_match_oracle(hx, oracle, name):
             clusters = (np.arange(n) // 5).astype(np.int32)
         for rnd, filt, thr in [(0, True, 2**31), (1, True, 1_500_000_000), (4, False, 0)]:
             single = clusters_kind == "identity" and rnd == 0
-            p, g, t = hx.device.sclp_proposals(dg, clusters, 3, 1, rnd, filt, thr, singleton=single)
             op, ogn, ot = oracle.sclp_proposals(og, clusters, 3, 1, rnd, filt, thr)
-            assert np.array_equal(p, op), (clusters_kind, rnd)
-            assert np.array_equal(g, ogn), (clusters_kind, rnd)
-            assert np.array_equal(t, ot), (clusters_kind, rnd)
+            # unknown bound: 64-bit table sums; the row-weight bound: 32-bit sums
+            for bound in (0, row_bound):
+                p, g, t = hx.device.sclp_proposals(dg, clusters, 3, 1, rnd, filt, thr, singleton=single,
+                                                   value_bound=bound)
+                assert np.array_equal(p, op), (clusters_kind, rnd, bound)
+                assert np.array_equal(g, ogn), (clusters_kind, rnd, bound)
+                assert np.array_equal(t, ot), (clusters_kind, rnd, bound)
+
+
+def mix32(x):
+    """The lowbias32 integer hash (a bijection on 32-bit words)."""
+    x = np.asarray(x, dtype=np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    return x ^ (x >> 16)
+
+
+@pytest.mark.parametrize("bound", [0, 1 << 20])
+def test_sclp_partition_overflow_splits(hx, oracle, bound):
+    """A row of 16385 entries (8 hash partitions) whose neighbours sit in 16385
+    distinct clusters that all hash into partition 0: the LDS pass overflows and
+    is redone 2 then 4 ways finer; proposals still equal the oracle's."""
+    say(f"case sclp partition overflow bound={bound}")
+    d = 16385
+    n = d + 1
+    candidates = np.arange(1, 400_000, dtype=np.int64)
+    ids = candidates[(mix32(candidates) & 7) == 0][:d]
+    assert len(ids) == d
+    cluster_count = int(ids.max()) + 1
+    indptr = np.zeros(n + 1, np.int64)
+    indptr[1] = d
+    indptr[2:] = d + np.arange(1, n)
+    indices = np.concatenate([np.arange(1, n), np.zeros(d, np.int64)])
+    src = np.repeat(np.arange(n), np.diff(indptr))
+    lo, hi = np.minimum(src, indices), np.maximum(src, indices)
+    ew = ((lo * 2654435761 + hi * 40503) % 7 + 1).astype(np.uint64)
+    clusters = np.zeros(n, np.int32)
+    clusters[1:] = ids
+    clusters[0] = cluster_count  # the hub alone
+    dg = hx.device.DeviceGraph(indptr, indices, ew)
+    og = oracle.Graph(indptr, indices.astype(np.int32), ew)
+    for rnd, filt, thr in [(1, True, 3_000_000_000), (4, False, 0)]:
+        op, ogn, ot = oracle.sclp_proposals(og, clusters, 3, 1, rnd, filt, thr)
+        p, g, t = hx.device.sclp_proposals(dg, clusters, 3, 1, rnd, filt, thr, value_bound=bound)
+        assert np.array_equal(p, op) and np.array_equal(g, ogn) and np.array_equal(t, ot), rnd
 
 
 @pytest.mark.parametrize("name", ["grid64", "rmat12", "hub"])
