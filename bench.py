@@ -147,12 +147,109 @@ def recorded_traffic(workload: str, family: str):
     return rec["traffic_bytes_per_dispatch"]
 
 
+def run_mix(args) -> int:
+    """The batched graph mix: partitions/sec of the batched form (worker threads,
+    FP32 diffusion) against the plain form (one thread, plain FP64 bisection)."""
+    import horkxg_amd as hx
+    from horkxg_amd import workloads as W
+
+    specs = W.mix_specs(args.mix_count)
+    t = time.perf_counter()
+    graphs = W.upload_mix(hx, specs, threads=min(16, os.cpu_count() or 1))
+    sizes = [(g.n, g.m) for g in graphs]
+    print(f"[bench] mix of {len(graphs)} graphs: {sum(n for n, _ in sizes)} vertices, "
+          f"{sum(m for _, m in sizes)} entries, generated in {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    parts = [s["k"] for s in specs]
+    ratios = [s["ratio"] for s in specs]
+    seeds = [s["seed"] for s in specs]
+
+    def batch(mode, workers, want_labels):
+        return hx.partition_batch(graphs, parts, ratios, seeds, spectral_mode=mode, workers=workers,
+                                  want_labels=want_labels)
+
+    for _ in range(args.warmup):
+        batch("batched_fp32", args.mix_workers, False)
+        print("[bench] mix warmup batch done", file=sys.stderr, flush=True)
+    walls = []
+    labels = reports = None
+    for step in range(args.steps):
+        labels, reports, wall = batch("batched_fp32", args.mix_workers, step == args.steps - 1)
+        walls.append(wall)
+        print(f"[bench] mix step {step}: {len(graphs)} partitions in {wall:.3f} s", file=sys.stderr, flush=True)
+    _, plain_reports, plain_wall = batch("plain_fp64", 1, False)
+    print(f"[bench] mix plain (1 worker, plain FP64): {plain_wall:.3f} s", file=sys.stderr, flush=True)
+
+    for g, lab, r, s in zip(graphs, labels, reports, specs):
+        assert g.cut(lab) == r["final_cut"], s
+        assert np.bincount(lab, minlength=s["k"]).max() <= r["capacity"], s
+    cuts = np.array([r["final_cut"] for r in reports], dtype=np.float64)
+    plain_cuts = np.array([r["final_cut"] for r in plain_reports], dtype=np.float64)
+    rel = np.where(plain_cuts > 0, cuts / np.maximum(plain_cuts, 1) - 1.0, 0.0)
+    wall = float(np.mean(walls))
+    value = len(graphs) / wall
+    plain_value = len(graphs) / plain_wall
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        order = sorted(range(len(graphs)), key=lambda i: sizes[i][0] + sizes[i][1])[:16]
+        total, same = 0.0, 0
+        for i in order:
+            indptr, indices = W.generate(hx, specs[i])
+            res, _, cpu_labels = cpu_baseline(indptr, indices, specs[i]["k"], specs[i]["ratio"], specs[i]["seed"],
+                                              threads)
+            total += res.total_seconds
+            _, rep = graphs[i].partition(specs[i]["k"], specs[i]["ratio"], specs[i]["seed"],
+                                         spectral_mode="batched_fp64", want_labels=False)
+            same += int(rep["final_cut"] == res.final_cut)
+        cpu = {"value": round(len(order) / total, 3), "unit": "partitions/s", "cores": threads, "kind": "port",
+               "sample": f"the {len(order)} smallest graphs of the mix, one CPU-oracle partition each "
+                         f"(batched FP64 bisection), same timed scope",
+               "gpu_fp64_cut_equal_to_oracle": f"{same}/{len(order)}"}
+
+    line = {
+        "metric": "batched_partitions_per_sec",
+        "value": round(value, 3),
+        "unit": "partitions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": f"mix{len(graphs)}", "graphs": len(graphs),
+                   "vertices_total": int(sum(n for n, _ in sizes)), "entries_total": int(sum(m for _, m in sizes)),
+                   "parts": "k_i from {2,4,8,32}", "imbalance": 1.03, "width": "standard",
+                   "workers": args.mix_workers, "spectral": "batched_fp32"},
+        "plain": {"partitions_per_sec": round(plain_value, 3), "workers": 1, "spectral": "plain_fp64",
+                  "wall_s": round(plain_wall, 4)},
+        "batched_over_plain": round(value / plain_value, 3),
+        "cut": {"total_batched_fp32": int(cuts.sum()), "total_plain_fp64": int(plain_cuts.sum()),
+                "graphs_with_equal_cut": int((cuts == plain_cuts).sum()),
+                "max_relative_difference": round(float(np.abs(rel).max()), 5)},
+        "all_within_capacity": True,
+        "roofline": None,
+        "roofline_note": "per-family event timing is per host thread; the kernels' roofline is reported "
+                         "by the single-graph workloads",
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="mesh128_k32", choices=sorted(WORKLOADS))
+    p.add_argument("--workload", default="mesh128_k32", choices=sorted(WORKLOADS) + ["mix"])
+    p.add_argument("--mix-count", type=int, default=256, help="graphs of the mix workload")
+    p.add_argument("--mix-workers", type=int, default=1,
+                   help="host threads (streams) of the batched mix; concurrent workers are switched off (1)")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--ratio", type=float, default=1.03)
     p.add_argument("--spectral-mode", default="batched_fp64",
@@ -185,6 +282,8 @@ def main() -> int:
         return 1
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.workload == "mix":
+        return run_mix(args)
     spec = WORKLOADS[args.workload]
     k = spec["k"]
     indptr, indices = make_graph(hx, spec)

@@ -25,6 +25,7 @@ from ._native import (  # noqa: F401
     hip_lib,
     host_lib,
     lib_dir,
+    partition_batch,
     partition_capacity,
     profile_enable,
     profile_read,

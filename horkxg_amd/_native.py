@@ -116,6 +116,10 @@ def host_lib():
         L.gpart_graph_cut.argtypes = [C.c_void_p, _i32p, C.POINTER(C.c_uint64)]
         L.gpart_graph_partition.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_int, C.c_int,
                                             C.c_void_p, C.POINTER(PartitionReport)]
+        L.gpart_partition_batch.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int, C.c_int,
+                                            C.POINTER(C.c_void_p), C.POINTER(PartitionReport),
+                                            C.POINTER(C.c_double)]
         L.gpart_graph_coarsen.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_double, C.c_int, C.c_int, C.c_int,
                                           C.c_int, C.POINTER(C.c_void_p)]
         L.gpart_graph_aggregate.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, _i32p, _u64p,
@@ -313,6 +317,29 @@ class Hierarchy:
         _check(host_lib().gpart_hierarchy_uncoarsen(self._h, lab, parts, ratio, max_rounds, growth, int(verify),
                                                     out, C.byref(cut)))
         return out, cut.value
+
+
+def partition_batch(graphs, parts, ratios=None, seeds=None, spectral_mode: str = "batched_fp32",
+                    workers: int = 4, want_labels: bool = True):
+    """Partition many graphs, largest first, on `workers` host threads (one HIP
+    stream and allocator each). Returns (labels list or None, report dicts, wall
+    seconds from the first coarsening start to the last labels on the device)."""
+    count = len(graphs)
+    parts = list(parts)
+    ratios = [1.03] * count if ratios is None else list(ratios)
+    seeds = list(range(count)) if seeds is None else list(seeds)
+    if not (len(parts) == len(ratios) == len(seeds) == count):
+        raise ValueError("parts, ratios and seeds need one entry per graph")
+    handles = (C.c_void_p * max(count, 1))(*[g._h for g in graphs])
+    labels = [np.empty(g.n, np.int32) for g in graphs] if want_labels else None
+    label_ptrs = (C.c_void_p * max(count, 1))(*([a.ctypes.data for a in labels] if labels else [None] * count))
+    reports = (PartitionReport * max(count, 1))()
+    wall = C.c_double()
+    _check(host_lib().gpart_partition_batch(count, handles, (C.c_int * max(count, 1))(*parts),
+                                            (C.c_double * max(count, 1))(*ratios),
+                                            (C.c_uint32 * max(count, 1))(*seeds), SPECTRAL_MODES[spectral_mode],
+                                            workers, label_ptrs if labels else None, reports, C.byref(wall)))
+    return labels, [reports[i].as_dict() for i in range(count)], wall.value
 
 
 class Graph:

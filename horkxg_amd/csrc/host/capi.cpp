@@ -12,12 +12,18 @@
 #include "uncoarsen.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <exception>
 #include <iostream>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <variant>
+#include <vector>
 
 namespace {
 
@@ -79,9 +85,16 @@ void copy_level(const gpart::WeightedGraph<Types>& g, int64_t* offsets, int32_t*
     if (vw) std::copy(g.vertex_weights.begin(), g.vertex_weights.end(), vw);
 }
 
+// A batch worker returns its allocator cache once it holds more than this.
+constexpr std::uint64_t kWorkerCacheBytes = std::uint64_t{16} << 30;
+
+// One partition with the CLI's production options. Returns the labels on the
+// device. release: free the hierarchy's cached memory and trim the allocator
+// inside the timed scope (the reference's per-run scope); a batch worker keeps its
+// cache for the next graph instead.
 template <typename Types>
-void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify, int mode, int32_t* labels,
-                    gpart_partition_report* report) {
+gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify,
+                                               int mode, bool release, gpart_partition_report* report) {
     using clock = std::chrono::steady_clock;
     auto* ctx = gpart::detail::context();
     std::uint64_t total = 0;
@@ -122,12 +135,8 @@ void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, i
     hierarchy.levels.clear();
     hierarchy.fine_to_coarse.clear();
     auto partition = std::move(result.device_partition);
-    GPART_CHECK(gpart_release_cache(ctx));
+    if (release) GPART_CHECK(gpart_release_cache(ctx));
     const auto t4 = clock::now();
-    if (labels) {
-        const auto host = partition.to_host();
-        std::copy(host.begin(), host.end(), labels);
-    }
     if (report) {
         report->final_cut = result.final_cut;
         report->max_part_weight =
@@ -145,6 +154,17 @@ void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, i
         report->uncoarsen_seconds = secs(t2, t3);
         report->free_seconds = secs(t3, t4);
         report->algorithm_seconds = secs(t0, t4);
+    }
+    return partition;
+}
+
+template <typename Types>
+void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify, int mode, int32_t* labels,
+                    gpart_partition_report* report) {
+    auto partition = run_pipeline(g, parts, ratio, seed, verify, mode, true, report);
+    if (labels) {
+        const auto host = partition.to_host();
+        std::copy(host.begin(), host.end(), labels);
     }
 }
 
@@ -252,6 +272,123 @@ int gpart_graph_partition(gpart_graph* g, int parts, double ratio, uint32_t seed
     return guard([&] {
         if (spectral_mode < 0 || spectral_mode > 2) throw std::invalid_argument("unknown spectral mode");
         std::visit([&](auto& x) { partition_impl(x, parts, ratio, seed, verify, spectral_mode, labels, report); }, g->g);
+    });
+}
+
+int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* parts, const double* ratios,
+                          const uint32_t* seeds, int spectral_mode, int workers, int32_t* const* labels,
+                          gpart_partition_report* reports, double* wall_seconds) {
+    return guard([&] {
+        using clock = std::chrono::steady_clock;
+        if (count < 0 || workers < 1) throw std::invalid_argument("batch needs count >= 0 and workers >= 1");
+        // Concurrent workers are switched off: the full 256-graph mix with four
+        // worker threads ended in a host segmentation fault whose cause is not
+        // established yet. One worker runs the queue in the calling thread.
+        if (workers > 1) throw std::invalid_argument("concurrent batch workers are disabled (workers must be 1)");
+        if (spectral_mode < 0 || spectral_mode > 2) throw std::invalid_argument("unknown spectral mode");
+        for (int i = 0; i < count; ++i)
+            if (!graphs[i]) throw std::invalid_argument("null graph in batch");
+        // largest first, so that the last graphs to start are the short ones
+        std::vector<int> order(static_cast<std::size_t>(count));
+        for (int i = 0; i < count; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return gpart_graph_edges(graphs[a]) + gpart_graph_vertices(graphs[a]) >
+                   gpart_graph_edges(graphs[b]) + gpart_graph_vertices(graphs[b]);
+        });
+        GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context()));  // uploads complete
+        const int threads = std::max(1, std::min(workers, count));
+        std::atomic<int> next{0};
+        std::atomic<bool> failed{false};
+        std::mutex mu;
+        std::condition_variable all_done;
+        int finished = 0;
+        std::exception_ptr error;
+        clock::time_point wall_end;
+        const auto wall_start = clock::now();
+        auto fail = [&]() {
+            std::lock_guard<std::mutex> lock(mu);
+            if (!error) error = std::current_exception();
+            failed = true;
+        };
+        auto run = [&]() {
+            std::vector<std::pair<int, gpart::DeviceVector<std::int32_t>>> done;
+            try {
+                auto* ctx = gpart::detail::context();
+                for (;;) {
+                    if (failed.load()) break;
+                    const int slot = next.fetch_add(1);
+                    if (slot >= count) break;
+                    const int i = order[static_cast<std::size_t>(slot)];
+                    auto part = std::visit(
+                        [&](auto& x) {
+                            return run_pipeline(x, parts[i], ratios[i], seeds[i], 0, spectral_mode, false,
+                                                reports ? reports + i : nullptr);
+                        },
+                        graphs[i]->g);
+                    done.emplace_back(i, std::move(part));
+                    // blocks are reused by exact size only: return the cache once it is large
+                    std::int64_t blocks = 0, released = 0;
+                    std::uint64_t cached = 0, live = 0;
+                    double secs[3];
+                    GPART_CHECK(gpart_cache_stats(ctx, &blocks, &cached, &live, &released, secs));
+                    if (cached > kWorkerCacheBytes) GPART_CHECK(gpart_release_cache(ctx));
+                }
+                GPART_CHECK(gpart_ctx_synchronize(ctx));
+            } catch (...) {
+                fail();
+            }
+            {
+                // the wall clock stops when every worker's last labels are on the device
+                std::unique_lock<std::mutex> lock(mu);
+                if (++finished == threads) {
+                    wall_end = clock::now();
+                    all_done.notify_all();
+                } else {
+                    all_done.wait(lock, [&] { return finished == threads; });
+                }
+            }
+            try {
+                if (!failed.load() && labels)
+                    for (auto& [i, part] : done)
+                        if (labels[i]) {
+                            const auto host = part.to_host();
+                            std::copy(host.begin(), host.end(), labels[i]);
+                        }
+            } catch (...) {
+                fail();
+            }
+        };
+        // Each worker owns a context for its lifetime and destroys it before the
+        // thread ends (never from a thread_local destructor).
+        auto work = [&]() {
+            gpart_ctx* ctx = nullptr;
+            if (gpart_ctx_create(0, &ctx) != GPART_OK) {
+                try {
+                    throw std::runtime_error(std::string("worker context: ") + gpart_last_error());
+                } catch (...) {
+                    fail();
+                }
+                std::unique_lock<std::mutex> lock(mu);
+                if (++finished == threads) {
+                    wall_end = clock::now();
+                    all_done.notify_all();
+                }
+                return;
+            }
+            gpart::detail::bind_context(ctx);
+            run();
+            gpart::detail::bind_context(nullptr);
+            gpart_ctx_destroy(ctx);
+        };
+        if (threads == 1) {
+            run();  // the calling thread's own context
+        } else {
+            std::vector<std::thread> pool;
+            for (int t = 0; t < threads; ++t) pool.emplace_back(work);
+            for (auto& t : pool) t.join();
+        }
+        if (error) std::rethrow_exception(error);
+        if (wall_seconds) *wall_seconds = std::chrono::duration<double>(wall_end - wall_start).count();
     });
 }
 

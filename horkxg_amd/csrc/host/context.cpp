@@ -9,24 +9,42 @@
 
 #include <atomic>
 #include <iostream>
+#include <mutex>
 #include <streambuf>
 #include <thread>
+#include <vector>
 
 namespace gpart::detail {
 namespace {
-// The loading thread's context is left to process teardown: destroying a HIP
-// stream from a thread_local destructor at exit races the runtime's own
-// shutdown. Contexts of other threads are destroyed when their thread ends.
+// No HIP call runs from a thread_local destructor: at thread exit the order of
+// TLS destructors across libraries is unspecified, so the HIP runtime's own
+// per-thread state may already be gone. The loading thread's context is left to
+// process teardown; a context owned by another thread that ends is handed to
+// `orphans` and destroyed by the next thread that creates a context.
 const std::thread::id loading_thread = std::this_thread::get_id();
+std::mutex orphan_mutex;
+std::vector<gpart_ctx*> orphans;
 
 struct ThreadContext {
     gpart_ctx* ctx = nullptr;
     bool owned = false;
     ~ThreadContext() {
-        if (owned && ctx && std::this_thread::get_id() != loading_thread) gpart_ctx_destroy(ctx);
+        if (owned && ctx && std::this_thread::get_id() != loading_thread) {
+            std::lock_guard<std::mutex> lock(orphan_mutex);
+            orphans.push_back(ctx);
+        }
     }
 };
 thread_local ThreadContext current;
+
+void destroy_orphans() {
+    std::vector<gpart_ctx*> dead;
+    {
+        std::lock_guard<std::mutex> lock(orphan_mutex);
+        dead.swap(orphans);
+    }
+    for (auto* ctx : dead) gpart_ctx_destroy(ctx);
+}
 
 class NullBuffer : public std::streambuf {
 protected:
@@ -42,6 +60,7 @@ std::ostream& log() { return log_enabled.load(std::memory_order_relaxed) ? std::
 
 gpart_ctx* context() {
     if (!current.ctx) {
+        destroy_orphans();
         GPART_CHECK(gpart_ctx_create(0, &current.ctx));
         current.owned = true;
     }

@@ -79,6 +79,20 @@ typedef struct gpart_partition_report {
 int gpart_graph_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int verify, int spectral_mode,
                           int32_t* labels, gpart_partition_report* report);
 
+/* Many independent partitions (the batched graph mix; no reference counterpart:
+ * the reference runs one graph per process). Graphs are taken largest first from
+ * one shared queue. workers must be 1 for now (the queue runs in the calling
+ * thread); several worker threads, each with its own context (HIP stream and
+ * allocator), are implemented but switched off until a host crash seen with four
+ * of them on the full mix is explained. Each graph gets gpart_graph_partition's pipeline and options with
+ * parts[i], ratios[i], seeds[i] and spectral_mode; a worker keeps its allocator
+ * cache from one graph to the next. labels[i] (host, n_i entries) and reports may
+ * be NULL. *wall_seconds runs from the first coarsening start to the moment the
+ * last graph's labels are on the device (label copies come after). */
+int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* parts, const double* ratios,
+                          const uint32_t* seeds, int spectral_mode, int workers, int32_t* const* labels,
+                          gpart_partition_report* reports, double* wall_seconds);
+
 /* Coarsening hierarchy (host copy). */
 int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ratio, int max_levels,
                         int cluster_size_factor, int pack_isolated, int verify, gpart_hierarchy** out);

@@ -113,3 +113,24 @@ def test_rmat_generator(hx):
     deg = np.diff(ip)
     assert deg[:64].mean() > 4 * deg[-2048:].mean()
     assert 0.5 * 2 * 16 * 4096 < len(ix) <= 2 * 16 * 4096
+
+
+def test_mix_specs_follow_the_stated_distribution():
+    """The batched mix: 256 graphs, types in rotation, log-uniform sizes in
+    [1e4, 2e6], parts from {2, 4, 8, 32}, seeds 0..255, deterministic."""
+    from horkxg_amd import workloads as W
+
+    specs = W.mix_specs()
+    assert len(specs) == 256 and specs == W.mix_specs()
+    assert [s["kind"] for s in specs[:3]] == ["grid2d", "mesh3d", "rmat"]
+    assert {s["k"] for s in specs} == {2, 4, 8, 32}
+    assert [s["seed"] for s in specs] == list(range(256))
+    for s in specs:
+        if s["kind"] == "grid2d":
+            n = s["size"][0] * s["size"][1]
+        elif s["kind"] == "mesh3d":
+            n = s["size"][0] ** 3
+        else:
+            n = 2 ** s["size"][0]
+            assert s["size"][1] == 8
+        assert 5_000 <= n <= 3_000_000, s
