@@ -39,11 +39,13 @@ FAMILY_KERNELS = {
     "spectral": "spectral_depth_kernel",
 }
 
+# spectral: the bisection form each configuration is quoted with (FP32 diffusion
+# for the R-MAT graphs, FP64 for the mesh and the grid).
 WORKLOADS = {
-    "mesh128_k32": dict(kind="mesh3d", size=(128, 128, 128), k=32),
-    "grid256_k4": dict(kind="grid2d", size=(256, 256), k=4),
-    "rmat22_k8": dict(kind="rmat", size=(22, 16), k=8),
-    "rmat18_k8": dict(kind="rmat", size=(18, 16), k=8),
+    "mesh128_k32": dict(kind="mesh3d", size=(128, 128, 128), k=32, spectral="batched_fp64"),
+    "grid256_k4": dict(kind="grid2d", size=(256, 256), k=4, spectral="batched_fp64"),
+    "rmat22_k8": dict(kind="rmat", size=(22, 16), k=8, spectral="batched_fp32"),
+    "rmat18_k8": dict(kind="rmat", size=(18, 16), k=8, spectral="batched_fp32"),
 }
 
 
@@ -252,8 +254,8 @@ def main() -> int:
                    help="host threads (streams) of the batched mix; concurrent workers are switched off (1)")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--ratio", type=float, default=1.03)
-    p.add_argument("--spectral-mode", default="batched_fp64",
-                   choices=["plain_fp64", "batched_fp64", "batched_fp32"])
+    p.add_argument("--spectral-mode", default=None, choices=["plain_fp64", "batched_fp64", "batched_fp32"],
+                   help="default: the workload's own form")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
     p.add_argument("--no-profile", action="store_true", help="skip the live per-family event timing")
@@ -286,6 +288,8 @@ def main() -> int:
         return run_mix(args)
     spec = WORKLOADS[args.workload]
     k = spec["k"]
+    if args.spectral_mode is None:
+        args.spectral_mode = spec["spectral"]
     indptr, indices = make_graph(hx, spec)
     n, m = len(indptr) - 1, len(indices)
     print(f"[bench] workload={args.workload} vertices={n} entries={m} k={k}", file=sys.stderr, flush=True)
@@ -342,6 +346,25 @@ def main() -> int:
                 else None}
             for f, v in fam.items()}
 
+    precision = None
+    if args.spectral_mode == "batched_fp32":
+        # FP32 against FP64 diffusion on the coarsest level of the production
+        # hierarchy, and the final cut of one FP64 partition (both untimed)
+        csf = max(64, (512 + k - 1) // k)
+        h = graph.coarsen(k, seed=args.seed, cluster_size_factor=csf, pack_isolated=True)
+        off, adj, ew, vw = h.level(len(h) - 1)
+        cmp = hx.Graph(off, adj, edge_weights=ew, vertex_weights=vw).spectral_compare(k, args.ratio, args.seed)
+        _, rep64 = graph.partition(k, args.ratio, args.seed, spectral_mode="batched_fp64", want_labels=False)
+        precision = {
+            "coarsest_vertices": len(off) - 1,
+            "max_abs_error_per_depth": [d["max_abs_error"] for d in cmp["depths"]],
+            "rms_error_per_depth": [d["rms_error"] for d in cmp["depths"]],
+            "splits_differ_per_depth": [d["splits_differ"] for d in cmp["depths"]],
+            "labels_differ_per_depth": [d["labels_differ"] for d in cmp["depths"]],
+            "initial_cut_fp64": cmp["cut_fp64"], "initial_cut_fp32": cmp["cut_fp32"],
+            "final_cut_fp64": rep64["final_cut"], "final_cut_fp32": int(cut),
+        }
+
     cpu = None
     reference_cut = None
     if not args.no_cpu_baseline:
@@ -381,6 +404,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if precision:
+        line["fp32_vs_fp64"] = precision
     print(json.dumps(line), flush=True)
     return 0
 

@@ -47,6 +47,25 @@ class PartitionReport(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class SpectralLevelError(C.Structure):
+    _fields_ = [("depth", C.c_int32), ("tasks", C.c_int32), ("vertices", C.c_int64),
+                ("max_abs_error", C.c_double), ("rms_error", C.c_double), ("splits_differ", C.c_int32),
+                ("pad", C.c_int32), ("labels_differ", C.c_int64)]
+
+
+class SpectralComparison(C.Structure):
+    _fields_ = [("depths", C.c_int32), ("pad", C.c_int32), ("level", SpectralLevelError * 16),
+                ("cut_fp64", C.c_uint64), ("cut_fp32", C.c_uint64), ("max_load_fp64", C.c_uint64),
+                ("max_load_fp32", C.c_uint64), ("labels_differ", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        levels = [{name: getattr(self.level[i], name) for name, _ in SpectralLevelError._fields_ if name != "pad"}
+                  for i in range(self.depths)]
+        out = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("pad", "level", "depths")}
+        out["depths"] = levels
+        return out
+
+
 class RefineReport(C.Structure):
     _fields_ = [("rounds", C.c_int32), ("admission_iterations", C.c_int32),
                 ("admission_fallbacks", C.c_int32), ("pad", C.c_int32), ("proposals", C.c_uint64),
@@ -116,6 +135,8 @@ def host_lib():
         L.gpart_graph_cut.argtypes = [C.c_void_p, _i32p, C.POINTER(C.c_uint64)]
         L.gpart_graph_partition.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_int, C.c_int,
                                             C.c_void_p, C.POINTER(PartitionReport)]
+        L.gpart_graph_spectral_compare.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32,
+                                                   C.POINTER(SpectralComparison)]
         L.gpart_partition_batch.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
                                             C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int, C.c_int,
                                             C.POINTER(C.c_void_p), C.POINTER(PartitionReport),
@@ -403,6 +424,13 @@ class Graph:
                                                 labels.ctypes.data if labels is not None else None,
                                                 C.byref(rep)))
         return labels, rep.as_dict()
+
+    def spectral_compare(self, parts: int, ratio: float = 1.03, seed: int = 0) -> dict:
+        """FP32 against FP64 diffusion of the batched bisection on this graph, per
+        recursion depth, and the cuts of both complete recursions."""
+        out = SpectralComparison()
+        _check(host_lib().gpart_graph_spectral_compare(self._h, parts, ratio, seed, C.byref(out)))
+        return out.as_dict()
 
     def coarsen(self, parts: int, seed: int = 0, stop_ratio: float = 0.90, max_levels: int = 24,
                 cluster_size_factor: int = 256, pack_isolated: bool = False, verify: bool = False) -> Hierarchy:

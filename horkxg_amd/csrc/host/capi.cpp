@@ -541,6 +541,32 @@ int gpart_graph_initial_partition(gpart_graph* g, int parts, double ratio, uint3
     });
 }
 
+int gpart_graph_spectral_compare(gpart_graph* g, int parts, double ratio, uint32_t seed,
+                                 gpart_spectral_comparison* out) {
+    return guard([&] {
+        std::visit(
+            [&](auto& x) {
+                using Types = std::conditional_t<std::is_same_v<std::decay_t<decltype(x)>, GraphOf<gpart::StandardTypes>>,
+                                                 gpart::StandardTypes, gpart::BigTypes>;
+                const auto c = gpart::compare_spectral_precision<Types>(x.device, parts, ratio, seed);
+                if (c.depths.size() > 16) throw std::runtime_error("more than 16 recursion depths");
+                *out = gpart_spectral_comparison{};
+                out->depths = static_cast<int32_t>(c.depths.size());
+                for (std::size_t i = 0; i < c.depths.size(); ++i) {
+                    const auto& e = c.depths[i];
+                    out->level[i] = gpart_spectral_level_error{e.depth, e.tasks, e.vertices, e.max_abs_error,
+                                                               e.rms_error, e.splits_differ, 0, e.labels_differ};
+                }
+                out->cut_fp64 = c.cut_fp64;
+                out->cut_fp32 = c.cut_fp32;
+                out->max_load_fp64 = c.max_load_fp64;
+                out->max_load_fp32 = c.max_load_fp32;
+                out->labels_differ = c.labels_differ;
+            },
+            g->g);
+    });
+}
+
 int gpart_graph_refine(gpart_graph* g, int32_t* labels, int parts, double ratio, int max_rounds, int stale_rounds,
                        int verify, gpart_refine_report* report) {
     return guard([&] {

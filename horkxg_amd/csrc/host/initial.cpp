@@ -165,6 +165,112 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
     return labels;
 }
 
+template <typename Types>
+SpectralComparison compare_spectral_precision(const DeviceWeightedGraph<Types>& graph, int parts, double imbalance,
+                                              std::uint32_t seed) {
+    auto* ctx = detail::context();
+    const auto n64 = graph.vertices();
+    if (n64 <= 0 || n64 > std::numeric_limits<int>::max() - 255 || parts < 2 || parts > n64 || parts > 32 ||
+        !std::isfinite(imbalance) || imbalance < 1.0)
+        throw std::invalid_argument("invalid GPU initial partition input");
+    const int n = static_cast<int>(n64);
+    const auto view = graph.view();
+    U64 total = 0, max_weight = 0, min_weight = 0;
+    GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &total, &max_weight, &min_weight));
+    const U64 capacity = partition_capacity(total, parts, imbalance);
+    DeviceVector<std::int32_t> labels(static_cast<std::size_t>(n), 0), order(static_cast<std::size_t>(n));
+    GPART_CHECK(gpart_sequence_i32(ctx, order.data(), n));
+    DeviceVector<U64> vertex_weights(static_cast<std::size_t>(n));
+    GPART_CHECK(gpart_widen_weights(ctx, graph.vertex_weights.data(), Types::kWidth, n, vertex_weights.data()));
+    DeviceVector<double> x(static_cast<std::size_t>(n)), y(static_cast<std::size_t>(n)),
+        degree(static_cast<std::size_t>(n)), x32(static_cast<std::size_t>(n));
+    DeviceVector<U64> prefix(static_cast<std::size_t>(n));
+    DeviceVector<gpart_bisect_task> next64(64), next32(64);
+    const auto max_task = gpart_spectral_depth_max_task();
+
+    auto run_depth = [&](DeviceVector<std::int32_t>& lab, DeviceVector<std::int32_t>& ord, int precision,
+                         DeviceVector<double>& coords, DeviceVector<gpart_bisect_task>& next,
+                         const std::vector<gpart_bisect_task>& tasks) {
+        std::int32_t produced = 0, status = 0;
+        GPART_CHECK(gpart_spectral_depth(ctx, &view, lab.data(), ord.data(), vertex_weights.data(), tasks.data(),
+                                         static_cast<std::int32_t>(tasks.size()), seed, capacity, precision,
+                                         coords.data(), next.data(), &produced, &status));
+        if (status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");
+        if (status == 2) throw std::runtime_error("GPU initializer infeasible balance");
+        std::vector<gpart_bisect_task> out(static_cast<std::size_t>(produced));
+        if (produced)
+            GPART_CHECK(gpart_memcpy_dtoh(ctx, out.data(), next.data(), out.size() * sizeof(gpart_bisect_task)));
+        return out;
+    };
+
+    SpectralComparison result;
+    std::vector<gpart_bisect_task> depth{{0, n, 0, parts, total}};
+    for (int d = 0; !depth.empty(); ++d) {
+        std::vector<gpart_bisect_task> batched, next;
+        for (const auto& t : depth) {
+            if (t.parts == 1) continue;
+            if (t.count > max_task) {
+                std::vector<gpart_bisect_task> children;
+                plain_task(graph, labels, order, vertex_weights, t, seed, capacity, x, y, degree, prefix, children);
+                next.insert(next.end(), children.begin(), children.end());
+            } else {
+                batched.push_back(t);
+            }
+        }
+        if (!batched.empty()) {
+            DeviceVector<std::int32_t> labels32(labels), order32(order);
+            const auto children32 = run_depth(labels32, order32, GPART_SPECTRAL_FP32, x32, next32, batched);
+            const auto children64 = run_depth(labels, order, GPART_SPECTRAL_FP64, x, next64, batched);
+            const auto h64 = x.to_host(), h32 = x32.to_host();
+            const auto l64 = labels.to_host(), l32 = labels32.to_host();
+            const auto ord = order.to_host();
+            SpectralDepthError e;
+            e.depth = d;
+            e.tasks = static_cast<int>(batched.size());
+            double squares = 0.0;
+            for (const auto& t : batched) {
+                for (int i = t.begin; i < t.begin + t.count; ++i) {
+                    const int v = ord[static_cast<std::size_t>(i)];
+                    const double diff = std::fabs(h32[v] - h64[v]);
+                    e.max_abs_error = std::max(e.max_abs_error, diff);
+                    squares += diff * diff;
+                    e.labels_differ += l32[v] != l64[v];
+                }
+                e.vertices += t.count;
+                // the left child starts at the task's segment and keeps its first part
+                auto left = [&](const std::vector<gpart_bisect_task>& kids) {
+                    for (const auto& c : kids)
+                        if (c.begin == t.begin && c.first_part == t.first_part) return c.count;
+                    return -1;
+                };
+                e.splits_differ += left(children32) != left(children64);
+            }
+            e.rms_error = e.vertices ? std::sqrt(squares / static_cast<double>(e.vertices)) : 0.0;
+            result.depths.push_back(e);
+            next.insert(next.end(), children64.begin(), children64.end());
+        }
+        depth.swap(next);
+    }
+    auto measure = [&](SpectralMode mode, U64& cut, U64& max_load) {
+        auto lab = initial_partition_gpu<Types>(graph, parts, imbalance, seed, mode);
+        GPART_CHECK(gpart_cut(ctx, &view, lab.data(), &cut));
+        DeviceVector<U64> loads(static_cast<std::size_t>(parts));
+        GPART_CHECK(gpart_part_weights(ctx, &view, lab.data(), parts, loads.data()));
+        const auto h = loads.to_host();
+        max_load = *std::max_element(h.begin(), h.end());
+        return lab.to_host();
+    };
+    const auto final64 = measure(SpectralMode::BatchedFp64, result.cut_fp64, result.max_load_fp64);
+    const auto final32 = measure(SpectralMode::BatchedFp32, result.cut_fp32, result.max_load_fp32);
+    for (std::size_t v = 0; v < final64.size(); ++v) result.labels_differ += final64[v] != final32[v];
+    return result;
+}
+
+template SpectralComparison compare_spectral_precision<StandardTypes>(const DeviceWeightedGraph<StandardTypes>&, int,
+                                                                      double, std::uint32_t);
+template SpectralComparison compare_spectral_precision<BigTypes>(const DeviceWeightedGraph<BigTypes>&, int, double,
+                                                                 std::uint32_t);
+
 template DeviceVector<std::int32_t> initial_partition_gpu<StandardTypes>(const DeviceWeightedGraph<StandardTypes>&, int,
                                                                          double, std::uint32_t, SpectralMode,
                                                                          DeviceVector<double>*);

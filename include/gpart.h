@@ -118,6 +118,38 @@ int gpart_hierarchy_uncoarsen(const gpart_hierarchy* h, const int32_t* coarsest_
 int gpart_graph_initial_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int spectral_mode,
                                   int32_t* labels, double* coords);
 
+/* FP32 against FP64 diffusion of the batched bisection on one graph (normally a
+ * coarsest level). No reference counterpart: the reference diffuses in FP64 only.
+ * Every recursion depth runs both precisions on the same tasks from the FP64
+ * state of the depth before. Per depth: the largest and RMS difference of the
+ * normalised vectors, the tasks whose split boundary differs and the vertices that
+ * land on a different side. Then the cut and the largest part weight of complete
+ * FP64 and FP32 recursions, and the vertices whose final label differs. */
+typedef struct gpart_spectral_level_error {
+    int32_t depth;
+    int32_t tasks;
+    int64_t vertices;
+    double max_abs_error;
+    double rms_error;
+    int32_t splits_differ;
+    int32_t pad;
+    int64_t labels_differ;
+} gpart_spectral_level_error;
+
+typedef struct gpart_spectral_comparison {
+    int32_t depths; /* entries of level[] used (at most 16) */
+    int32_t pad;
+    gpart_spectral_level_error level[16];
+    uint64_t cut_fp64;
+    uint64_t cut_fp32;
+    uint64_t max_load_fp64;
+    uint64_t max_load_fp32;
+    int64_t labels_differ;
+} gpart_spectral_comparison;
+
+int gpart_graph_spectral_compare(gpart_graph* g, int parts, double ratio, uint32_t seed,
+                                 gpart_spectral_comparison* out);
+
 typedef struct gpart_refine_report {
     int32_t rounds;
     int32_t admission_iterations;

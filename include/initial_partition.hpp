@@ -4,6 +4,9 @@
 
 #include "graph.hpp"
 
+#include <cstdint>
+#include <vector>
+
 namespace gpart {
 
 // How the bisection tasks run: the reference's per-task launch sequence in FP64,
@@ -18,5 +21,28 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
                                                             double imbalance_ratio, std::uint32_t seed = 0,
                                                             SpectralMode mode = SpectralMode::BatchedFp64,
                                                             DeviceVector<double>* coords = nullptr);
+
+// FP32 against FP64 diffusion on one graph (no reference counterpart: the
+// reference has only FP64). Each recursion depth runs both precisions on the same
+// tasks, from the FP64 state of the depth before; the FP64 result carries on.
+struct SpectralDepthError {
+    int depth = 0;
+    int tasks = 0;                   // batched tasks compared at this depth
+    std::int64_t vertices = 0;       // their vertices
+    double max_abs_error = 0.0;      // max |x32 - x64| of the normalised vectors
+    double rms_error = 0.0;
+    int splits_differ = 0;           // tasks whose split boundary differs
+    std::int64_t labels_differ = 0;  // vertices on a different side after the depth
+};
+struct SpectralComparison {
+    std::vector<SpectralDepthError> depths;
+    std::uint64_t cut_fp64 = 0, cut_fp32 = 0;  // full batched recursions in each precision
+    std::uint64_t max_load_fp64 = 0, max_load_fp32 = 0;
+    std::int64_t labels_differ = 0;
+};
+
+template <typename Types>
+SpectralComparison compare_spectral_precision(const DeviceWeightedGraph<Types>& graph, int parts,
+                                              double imbalance_ratio, std::uint32_t seed = 0);
 
 }  // namespace gpart

@@ -305,3 +305,33 @@ def test_refine_rebuild_rounds_match_oracle(hx, oracle, name, k):
     pol, porep = oracle.partition(og, k, 1.03, 0)
     assert prep["final_cut"] == porep.final_cut
     assert np.array_equal(pl, pol)
+
+
+@pytest.mark.parametrize("name,k", [("grid256", 4), ("rmat12", 8), ("grid64", 32)])
+def test_spectral_fp32_against_fp64(hx, oracle, name, k):
+    """FP32 diffusion against FP64, per recursion depth, on the coarsest level of the
+    production hierarchy: the FP64 side reproduces the plain initial partition, the
+    FP32 side the batched FP32 one, and the per-depth vector error stays small."""
+    say(f"case spectral fp32 vs fp64 graph={name} k={k}")
+    indptr, indices = GRAPHS[name](hx)
+    g = hx.Graph(indptr, indices)
+    csf = max(64, (512 + k - 1) // k)
+    h = g.coarsen(k, seed=0, cluster_size_factor=csf, pack_isolated=True)
+    off, adj, ew, vw = h.level(len(h) - 1)
+    coarse = hx.Graph(off, adj, edge_weights=ew, vertex_weights=vw)
+    rep = coarse.spectral_compare(k, 1.03, 0)
+    depths = rep["depths"]
+    assert [d["depth"] for d in depths] == list(range(len(depths)))
+    assert len(depths) == (k - 1).bit_length()
+    assert [d["tasks"] for d in depths] == [2 ** i for i in range(len(depths))]
+    for d in depths:
+        assert d["vertices"] == len(off) - 1
+        assert 0.0 <= d["rms_error"] <= d["max_abs_error"] < 1e-3, d
+        assert d["labels_differ"] >= 0 and 0 <= d["splits_differ"] <= d["tasks"]
+    lab64 = coarse.initial_partition(k, 1.03, 0, "batched_fp64")
+    lab32 = coarse.initial_partition(k, 1.03, 0, "batched_fp32")
+    assert rep["cut_fp64"] == coarse.cut(lab64)
+    assert rep["cut_fp32"] == coarse.cut(lab32)
+    assert rep["labels_differ"] == int((lab64 != lab32).sum())
+    cap = hx.partition_capacity(int(vw.astype(object).sum()), k, 1.03)
+    assert max(rep["max_load_fp64"], rep["max_load_fp32"]) <= cap
