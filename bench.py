@@ -46,6 +46,9 @@ WORKLOADS = {
     "grid256_k4": dict(kind="grid2d", size=(256, 256), k=4, spectral="batched_fp64"),
     "rmat22_k8": dict(kind="rmat", size=(22, 16), k=8, spectral="batched_fp32"),
     "rmat18_k8": dict(kind="rmat", size=(18, 16), k=8, spectral="batched_fp32"),
+    # big width (64-bit offsets and weights), generated on the GPU
+    "rmat26_k32": dict(kind="rmat", size=(26, 16), k=32, spectral="batched_fp32", width="big", device=True,
+                       steps=2, warmup=1),
 }
 
 
@@ -246,8 +249,8 @@ def run_mix(args) -> int:
 def main() -> int:
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=None, help="default 5 (2 for rmat26_k32)")
+    p.add_argument("--warmup", type=int, default=None, help="default 1")
     p.add_argument("--workload", default="mesh128_k32", choices=sorted(WORKLOADS) + ["mix"])
     p.add_argument("--mix-count", type=int, default=256, help="graphs of the mix workload")
     p.add_argument("--mix-workers", type=int, default=1,
@@ -288,12 +291,26 @@ def main() -> int:
         return run_mix(args)
     spec = WORKLOADS[args.workload]
     k = spec["k"]
+    width = spec.get("width", "standard")
     if args.spectral_mode is None:
         args.spectral_mode = spec["spectral"]
-    indptr, indices = make_graph(hx, spec)
-    n, m = len(indptr) - 1, len(indices)
-    print(f"[bench] workload={args.workload} vertices={n} entries={m} k={k}", file=sys.stderr, flush=True)
-    graph = hx.Graph(indptr, indices)
+    if args.steps is None:
+        args.steps = spec.get("steps", 5)
+    if args.warmup is None:
+        args.warmup = spec.get("warmup", 1)
+    if spec.get("device"):
+        scale, ef = spec["size"]
+        t = time.perf_counter()
+        graph = hx.Graph.generate_rmat(scale, ef, 0.57, 0.19, 0.19, 0, width=width)
+        indptr = indices = None
+        n, m = graph.n, graph.m
+        print(f"[bench] generated on the GPU in {time.perf_counter() - t:.2f} s", file=sys.stderr, flush=True)
+    else:
+        indptr, indices = make_graph(hx, spec)
+        n, m = len(indptr) - 1, len(indices)
+        graph = hx.Graph(indptr, indices, width=width)
+    print(f"[bench] workload={args.workload} vertices={n} entries={m} k={k} width={width}", file=sys.stderr,
+          flush=True)
 
     families = list(hx.PROFILE_FAMILIES)
     for _ in range(args.warmup):
@@ -347,13 +364,16 @@ def main() -> int:
             for f, v in fam.items()}
 
     precision = None
-    if args.spectral_mode == "batched_fp32":
+    # (skipped for graphs generated on the GPU: their host hierarchy copy is the
+    # size of the input; the R-MAT 22 line carries the comparison)
+    if args.spectral_mode == "batched_fp32" and indptr is not None:
         # FP32 against FP64 diffusion on the coarsest level of the production
         # hierarchy, and the final cut of one FP64 partition (both untimed)
         csf = max(64, (512 + k - 1) // k)
         h = graph.coarsen(k, seed=args.seed, cluster_size_factor=csf, pack_isolated=True)
         off, adj, ew, vw = h.level(len(h) - 1)
-        cmp = hx.Graph(off, adj, edge_weights=ew, vertex_weights=vw).spectral_compare(k, args.ratio, args.seed)
+        cmp = hx.Graph(off, adj, width=width, edge_weights=ew, vertex_weights=vw).spectral_compare(
+            k, args.ratio, args.seed)
         _, rep64 = graph.partition(k, args.ratio, args.seed, spectral_mode="batched_fp64", want_labels=False)
         precision = {
             "coarsest_vertices": len(off) - 1,
@@ -367,7 +387,7 @@ def main() -> int:
 
     cpu = None
     reference_cut = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and indptr is not None:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         res, cpu_wall, cpu_labels = cpu_baseline(indptr, indices, k, args.ratio, args.seed, threads)
         reference_cut = int(res.final_cut)
@@ -390,7 +410,7 @@ def main() -> int:
         "dtype": "int64",
         "data": "synthetic",
         "config": {"workload": args.workload, "vertices": n, "directed_entries": m, "k": k,
-                   "imbalance": args.ratio, "seed": args.seed, "width": "standard",
+                   "imbalance": args.ratio, "seed": args.seed, "width": width,
                    "spectral": args.spectral_mode},
         "edge_cut": int(cut),
         "reference_cut": reference_cut,

@@ -135,6 +135,10 @@ def host_lib():
         L.gpart_graph_cut.argtypes = [C.c_void_p, _i32p, C.POINTER(C.c_uint64)]
         L.gpart_graph_partition.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_int, C.c_int,
                                             C.c_void_p, C.POINTER(PartitionReport)]
+        L.gpart_graph_generate_rmat.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                                C.c_uint64, C.POINTER(C.c_void_p)]
+        L.gpart_graph_generate_lattice.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]
+        L.gpart_graph_arrays.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.gpart_graph_spectral_compare.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32,
                                                    C.POINTER(SpectralComparison)]
         L.gpart_partition_batch.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
@@ -396,6 +400,45 @@ class Graph:
         g._keep = []
         g.width = width
         return g
+
+    @classmethod
+    def _adopt(cls, handle, width: str) -> "Graph":
+        g = cls.__new__(cls)
+        g._h = handle
+        g._keep = []
+        g.width = width
+        return g
+
+    @classmethod
+    def generate_rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+                      seed: int = 0, width: str = "standard") -> "Graph":
+        """The graph of generate_rmat(...), built on the GPU (unit weights)."""
+        h = C.c_void_p()
+        _check(host_lib().gpart_graph_generate_rmat(WIDTHS[width], scale, edge_factor, a, b, c, seed, C.byref(h)))
+        return cls._adopt(h.value, width)
+
+    @classmethod
+    def generate_grid2d(cls, rows: int, cols: int, width: str = "standard") -> "Graph":
+        h = C.c_void_p()
+        _check(host_lib().gpart_graph_generate_lattice(WIDTHS[width], cols, rows, 1, C.byref(h)))
+        return cls._adopt(h.value, width)
+
+    @classmethod
+    def generate_mesh3d(cls, nx: int, ny: int, nz: int, width: str = "standard") -> "Graph":
+        h = C.c_void_p()
+        _check(host_lib().gpart_graph_generate_lattice(WIDTHS[width], nx, ny, nz, C.byref(h)))
+        return cls._adopt(h.value, width)
+
+    def arrays(self):
+        """(offsets int64, neighbors int32, edge weights u64, vertex weights u64) on the host."""
+        n, m = self.n, self.m
+        off = np.empty(n + 1, np.int64)
+        nbr = np.empty(m, np.int32)
+        ew = np.empty(m, np.uint64)
+        vw = np.empty(n, np.uint64)
+        _check(host_lib().gpart_graph_arrays(self._h, off.ctypes.data, nbr.ctypes.data, ew.ctypes.data,
+                                             vw.ctypes.data))
+        return off, nbr, ew, vw
 
     def __del__(self):
         if getattr(self, "_h", None) and _host is not None:

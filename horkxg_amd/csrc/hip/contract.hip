@@ -284,9 +284,8 @@ void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degre
     nbr = static_cast<std::int32_t*>(ctx_alloc(ctx, sizeof(std::int32_t) * std::max<std::int64_t>(coarse_m, 1)));
     wts = static_cast<W*>(ctx_alloc(ctx, sizeof(W) * std::max<std::int64_t>(coarse_m, 1)));
     if (coarse_m > 0) {
-        GPART_HIP_CHECK(hipMemcpyAsync(nbr, tmp_nbr.get(), sizeof(std::int32_t) * coarse_m, hipMemcpyDeviceToDevice,
-                                       ctx->stream));
-        GPART_HIP_CHECK(hipMemcpyAsync(wts, tmp_w.get(), sizeof(W) * coarse_m, hipMemcpyDeviceToDevice, ctx->stream));
+        copy_device(ctx, nbr, tmp_nbr.get(), sizeof(std::int32_t) * coarse_m);
+        copy_device(ctx, wts, tmp_w.get(), sizeof(W) * coarse_m);
     }
     hipLaunchKernelGGL(row_offsets_kernel<O>, dim3(grid_for(coarse_m + 1)), dim3(kBlock), 0, ctx->stream, coarse_m,
                        sources.get(), n_coarse, offsets);

@@ -141,6 +141,8 @@ T read_scalar(gpart_ctx* ctx, const T* device) {
 }
 
 void fill_bytes(gpart_ctx* ctx, void* dst, int value, std::size_t bytes);
+// Stream-ordered device-to-device copy by kernel.
+void copy_device(gpart_ctx* ctx, void* dst, const void* src, std::size_t bytes);
 
 // Profiling hooks around a family's main kernel launches.
 void profile_begin(gpart_ctx* ctx, int family);

@@ -241,22 +241,22 @@ int gpart_sort_pairs_u64(gpart_ctx* ctx, uint64_t* keys, void* values, int value
         bool alt = false;
         if (!values) {
             alt = radix_sort_keys(ctx, keys, keys_alt.get(), n, begin_bit, end_bit);
-            if (alt) GPART_HIP_CHECK(hipMemcpyAsync(keys, keys_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            if (alt) copy_device(ctx, keys, keys_alt.get(), n * 8);
         } else if (value_bytes == 4) {
             Scratch<std::uint32_t> values_alt(ctx, n);
             alt = radix_sort_pairs(ctx, keys, keys_alt.get(), static_cast<std::uint32_t*>(values), values_alt.get(), n,
                                    begin_bit, end_bit);
             if (alt) {
-                GPART_HIP_CHECK(hipMemcpyAsync(keys, keys_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
-                GPART_HIP_CHECK(hipMemcpyAsync(values, values_alt.get(), n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+                copy_device(ctx, keys, keys_alt.get(), n * 8);
+                copy_device(ctx, values, values_alt.get(), n * 4);
             }
         } else if (value_bytes == 8) {
             Scratch<std::uint64_t> values_alt(ctx, n);
             alt = radix_sort_pairs(ctx, keys, keys_alt.get(), static_cast<std::uint64_t*>(values), values_alt.get(), n,
                                    begin_bit, end_bit);
             if (alt) {
-                GPART_HIP_CHECK(hipMemcpyAsync(keys, keys_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
-                GPART_HIP_CHECK(hipMemcpyAsync(values, values_alt.get(), n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                copy_device(ctx, keys, keys_alt.get(), n * 8);
+                copy_device(ctx, values, values_alt.get(), n * 8);
             }
         } else {
             throw Error(GPART_ERR_INVALID_ARGUMENT, "value_bytes must be 4 or 8");

@@ -31,7 +31,18 @@ template <typename Types>
 struct GraphOf {
     gpart::WeightedGraph<Types> host;
     gpart::DeviceWeightedGraph<Types> device;
+    bool host_ready = true;  // false for graphs generated on the device until a host copy is needed
 };
+
+// The host copy, made from the device copy on first use.
+template <typename Types>
+gpart::WeightedGraph<Types>& host_of(GraphOf<Types>& g) {
+    if (!g.host_ready) {
+        g.host = gpart::copy_device_weighted(g.device);
+        g.host_ready = true;
+    }
+    return g.host;
+}
 
 thread_local std::string capi_error;
 
@@ -97,8 +108,11 @@ gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, dou
                                                int mode, bool release, gpart_partition_report* report) {
     using clock = std::chrono::steady_clock;
     auto* ctx = gpart::detail::context();
-    std::uint64_t total = 0;
-    for (const auto w : g.host.vertex_weights) total += static_cast<std::uint64_t>(w);
+    std::uint64_t total = 0, heaviest = 0, lightest = 0;
+    {
+        const auto view = g.device.view();
+        GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &total, &heaviest, &lightest));
+    }
     gpart::partition_capacity(total, parts, ratio);
     gpart::CoarsenOptions co;
     co.parts = parts;
@@ -180,6 +194,43 @@ struct gpart_hierarchy {
     std::variant<gpart::Hierarchy<gpart::StandardTypes>, gpart::Hierarchy<gpart::BigTypes>> h;
 };
 
+namespace {
+// A device CSR from the kernel library's generators, adopted with unit weights.
+template <typename Types>
+GraphOf<Types> adopt_generated(void* offsets, std::int32_t* neighbors, std::int64_t n, std::int64_t m) {
+    using O = typename Types::OffsetT;
+    using W = typename Types::WeightT;
+    GraphOf<Types> holder;
+    holder.device.offsets = gpart::DeviceVector<O>::adopt(static_cast<O*>(offsets), static_cast<std::size_t>(n + 1));
+    holder.device.neighbors = gpart::DeviceVector<std::int32_t>::adopt(neighbors, static_cast<std::size_t>(m));
+    holder.device.edge_weights = gpart::DeviceVector<W>(static_cast<std::size_t>(m), W{1});
+    holder.device.vertex_weights = gpart::DeviceVector<W>(static_cast<std::size_t>(n), W{1});
+    holder.host_ready = false;
+    return holder;
+}
+
+template <typename F>
+int generate_into(int width, gpart_graph** out, F&& generate) {
+    return guard([&] {
+        void* offsets = nullptr;
+        std::int32_t* neighbors = nullptr;
+        std::int64_t n = 0, m = 0;
+        auto* ctx = gpart::detail::context();
+        GPART_CHECK(generate(ctx, &offsets, &neighbors, &n, &m));
+        auto* g = new gpart_graph{width, GraphOf<gpart::StandardTypes>{}};
+        try {
+            if (width == GPART_WIDTH_STANDARD) g->g = adopt_generated<gpart::StandardTypes>(offsets, neighbors, n, m);
+            else g->g = adopt_generated<gpart::BigTypes>(offsets, neighbors, n, m);
+            GPART_CHECK(gpart_ctx_synchronize(ctx));
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+}  // namespace
+
 extern "C" {
 
 int gpart_partition_capacity_c(uint64_t total, int parts, double ratio, uint64_t* out) {
@@ -221,6 +272,26 @@ int gpart_graph_upload(int width, int64_t n, const int64_t* indptr, int64_t m, c
     });
 }
 
+int gpart_graph_generate_rmat(int width, int scale, int64_t edge_factor, double a, double b, double c, uint64_t seed,
+                              gpart_graph** out) {
+    return generate_into(width, out, [&](gpart_ctx* ctx, void** o, int32_t** nb, int64_t* n, int64_t* m) {
+        return gpart_generate_rmat_device(ctx, width, scale, edge_factor, a, b, c, seed, o, nb, n, m);
+    });
+}
+
+int gpart_graph_arrays(gpart_graph* g, int64_t* offsets, int32_t* neighbors, uint64_t* edge_weights,
+                       uint64_t* vertex_weights) {
+    return guard([&] {
+        std::visit([&](auto& x) { copy_level(host_of(x), offsets, neighbors, edge_weights, vertex_weights); }, g->g);
+    });
+}
+
+int gpart_graph_generate_lattice(int width, int64_t nx, int64_t ny, int64_t nz, gpart_graph** out) {
+    return generate_into(width, out, [&](gpart_ctx* ctx, void** o, int32_t** nb, int64_t* n, int64_t* m) {
+        return gpart_generate_lattice_device(ctx, width, nx, ny, nz, o, nb, n, m);
+    });
+}
+
 int gpart_graph_load(int width, const char* indptr_path, const char* indices_path, int validate, gpart_graph** out) {
     return guard([&] {
         auto* g = new gpart_graph{width, GraphOf<gpart::StandardTypes>{}};
@@ -248,18 +319,18 @@ int gpart_graph_load(int width, const char* indptr_path, const char* indices_pat
 void gpart_graph_free(gpart_graph* g) { delete g; }
 
 int64_t gpart_graph_vertices(const gpart_graph* g) {
-    return std::visit([](const auto& x) { return x.host.vertices(); }, g->g);
+    return std::visit([](const auto& x) { return x.device.vertices(); }, g->g);
 }
 
 int64_t gpart_graph_edges(const gpart_graph* g) {
-    return std::visit([](const auto& x) { return x.host.edges(); }, g->g);
+    return std::visit([](const auto& x) { return x.device.edges(); }, g->g);
 }
 
 int gpart_graph_cut(gpart_graph* g, const int32_t* labels, uint64_t* cut) {
     return guard([&] {
         std::visit(
             [&](auto& x) {
-                gpart::DeviceVector<std::int32_t> d(std::vector<std::int32_t>(labels, labels + x.host.vertices()));
+                gpart::DeviceVector<std::int32_t> d(std::vector<std::int32_t>(labels, labels + x.device.vertices()));
                 const auto view = x.device.view();
                 GPART_CHECK(gpart_cut(gpart::detail::context(), &view, d.data(), cut));
             },
@@ -405,7 +476,7 @@ int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ra
         o.strict_verify = verify != 0;
         std::visit(
             [&](auto& x) {
-                auto h = gpart::coarsen(x.host, o);
+                auto h = gpart::coarsen(host_of(x), o);
                 *out = new gpart_hierarchy{g->width, std::move(h)};
             },
             g->g);
@@ -578,9 +649,9 @@ int gpart_graph_refine(gpart_graph* g, int32_t* labels, int parts, double ratio,
                 o.max_rounds = max_rounds;
                 o.stale_rounds = stale_rounds;
                 o.strict_verify = verify != 0;
-                std::vector<std::int32_t> l(labels, labels + x.host.vertices());
+                std::vector<std::int32_t> l(labels, labels + x.device.vertices());
                 gpart::RefineStats s;
-                gpart::refine_partition(x.host, l, o, &s);
+                gpart::refine_partition(host_of(x), l, o, &s);
                 std::copy(l.begin(), l.end(), labels);
                 if (report) {
                     report->rounds = s.rounds;

@@ -11,6 +11,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <utility>
 #include <vector>
 
@@ -143,6 +144,11 @@ void DeviceVector<T>::fill(const T& value) {
     for (std::size_t i = 1; i < sizeof(T); ++i) uniform_bytes = uniform_bytes && bytes[i] == bytes[0];
     if (uniform_bytes) {
         GPART_CHECK(gpart_memset(detail::context(), data_, bytes[0], size_ * sizeof(T)));
+    } else if (sizeof(T) == 4 || sizeof(T) == 8) {
+        std::uint64_t pattern = 0;
+        std::memcpy(&pattern, &value, sizeof(T));
+        GPART_CHECK(gpart_fill_pattern(detail::context(), data_, pattern, static_cast<int>(sizeof(T)),
+                                       static_cast<std::int64_t>(size_)));
     } else {
         std::vector<T> host(size_, value);
         GPART_CHECK(gpart_memcpy_htod(detail::context(), data_, host.data(), size_ * sizeof(T)));

@@ -51,6 +51,18 @@ int gpart_graph_upload(int width, int64_t n, const int64_t* indptr, int64_t m, c
 /* Load the reference's binary int64 files (native byte order). */
 int gpart_graph_load(int width, const char* indptr_path, const char* indices_path, int validate,
                      gpart_graph** out);
+/* Synthetic graphs built on the GPU, identical to the host generators below
+ * (gpart_generate_rmat, gpart_generate_grid2d with rows = ny and cols = nx, and
+ * gpart_generate_mesh3d; nz == 1 selects the grid), with unit weights. The host
+ * copy is made on first use by an entry point that needs it. No reference
+ * counterpart (graph_untils/ prepares graphs on the host). */
+int gpart_graph_generate_rmat(int width, int scale, int64_t edge_factor, double a, double b, double c, uint64_t seed,
+                              gpart_graph** out);
+int gpart_graph_generate_lattice(int width, int64_t nx, int64_t ny, int64_t nz, gpart_graph** out);
+/* The graph's CSR on the host (offsets n + 1, neighbors m, weights m and n); any
+ * pointer may be NULL. */
+int gpart_graph_arrays(gpart_graph* g, int64_t* offsets, int32_t* neighbors, uint64_t* edge_weights,
+                       uint64_t* vertex_weights);
 void gpart_graph_free(gpart_graph* g);
 int64_t gpart_graph_vertices(const gpart_graph* g);
 int64_t gpart_graph_edges(const gpart_graph* g);

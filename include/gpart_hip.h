@@ -74,6 +74,8 @@ int gpart_memcpy_dtoh(gpart_ctx* ctx, void* dst, const void* src, size_t bytes);
 int gpart_memcpy_dtod(gpart_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Fill with a byte value using a kernel (safe under stream capture). */
 int gpart_memset(gpart_ctx* ctx, void* dst, int value, size_t bytes);
+/* count copies of a 4- or 8-byte pattern (kernel; safe under stream capture). */
+int gpart_fill_pattern(gpart_ctx* ctx, void* dst, uint64_t pattern, int pattern_bytes, int64_t count);
 /* iota: out[i] = i */
 int gpart_sequence_i32(gpart_ctx* ctx, int32_t* out, int64_t n);
 /* Widen vertex weights of either width to uint64. */
@@ -139,6 +141,21 @@ typedef struct gpart_degree_classes {
 /* host out: counts. */
 int gpart_degree_classes_build(gpart_ctx* ctx, const gpart_csr_view* g,
                                gpart_degree_classes* classes);
+
+/* ---------------- synthetic graphs on the device ---------------- */
+
+/* The host generators' graphs (gpart_generate_rmat/grid2d/mesh3d of gpart.h),
+ * entry for entry, built on the device: unit weights, rows sorted ascending, no
+ * self loops or duplicates. offsets (n + 1 entries of the width's offset type) and
+ * neighbors (m int32) are context allocations for the caller (gpart_device_free).
+ * No reference counterpart (graph_untils/ prepares graphs on the host).
+ * host out: *n, *m. */
+int gpart_generate_rmat_device(gpart_ctx* ctx, int width, int scale, int64_t edge_factor, double a, double b,
+                               double c, uint64_t seed, void** offsets, int32_t** neighbors, int64_t* n,
+                               int64_t* m);
+/* nz == 1: the 2D grid of rows = ny, cols = nx; otherwise the nx x ny x nz mesh. */
+int gpart_generate_lattice_device(gpart_ctx* ctx, int width, int64_t nx, int64_t ny, int64_t nz, void** offsets,
+                                  int32_t** neighbors, int64_t* n, int64_t* m);
 
 /* ---------------- SCLP coarsening (src/coarsen.cu) ---------------- */
 
