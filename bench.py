@@ -313,15 +313,22 @@ def main() -> int:
           flush=True)
 
     families = list(hx.PROFILE_FAMILIES)
+    cold_seconds = None
     for _ in range(args.warmup):
-        graph.partition(k, args.ratio, args.seed, spectral_mode=args.spectral_mode, want_labels=False)
-        print("[bench] warmup step done", file=sys.stderr, flush=True)
+        _, rep = graph.partition(k, args.ratio, args.seed, spectral_mode=args.spectral_mode, want_labels=False)
+        if cold_seconds is None:
+            cold_seconds = rep["algorithm_seconds"]
+        print(f"[bench] warmup step done algorithm_seconds={rep['algorithm_seconds']:.4f}", file=sys.stderr,
+              flush=True)
     if not args.no_profile:
         hx.profile_reset()
         for f in families:
             hx.profile_enable(f, True)
     reports = []
-    hx.trim()
+    # The context keeps its device segments across runs (freshly recycled device
+    # memory is scrubbed by the driver on allocation); the first, cold step is
+    # reported beside the steady-state value.
+    alloc_before = hx.cache_info()
     t0 = time.perf_counter()
     for step in range(args.steps):
         labels, rep = graph.partition(k, args.ratio, args.seed, spectral_mode=args.spectral_mode,
@@ -330,6 +337,7 @@ def main() -> int:
         print(f"[bench] step {step} algorithm_seconds={rep['algorithm_seconds']:.4f} cut={rep['final_cut']}",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    alloc_after = hx.cache_info()
     fam = {}
     if not args.no_profile:
         for f in families:
@@ -421,6 +429,17 @@ def main() -> int:
         "stage_seconds": {s: round(float(np.mean([r[s] for r in reports])), 5)
                           for s in ("coarsen_seconds", "initial_seconds", "uncoarsen_seconds", "free_seconds")},
         "algorithm_seconds_min": round(float(alg.min()), 6),
+        "cold_step_seconds": (round(float(cold_seconds), 6) if cold_seconds is not None else None),
+        "device_memory": {
+            "peak_reserved_gib": round(alloc_after["peak_reserved_bytes"] / 2**30, 3),
+            "peak_live_gib": round(alloc_after["peak_live_bytes"] / 2**30, 3),
+            "allocations_in_timed_steps": alloc_after["segment_allocations"] - alloc_before["segment_allocations"],
+            "allocation_seconds_in_timed_steps": round(alloc_after["segment_allocation_seconds"]
+                                                       - alloc_before["segment_allocation_seconds"], 4),
+            "allocations_total": alloc_after["segment_allocations"],
+            "allocation_seconds_total": round(alloc_after["segment_allocation_seconds"], 4),
+            "out_of_memory_releases": alloc_after["out_of_memory_releases"],
+        },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -31,6 +31,7 @@ from ._native import (  # noqa: F401
     profile_read,
     profile_reset,
     record_layout,
+    cache_info,
     cache_stats,
     set_log,
     set_refine_rebuild,

@@ -109,6 +109,7 @@ def hip_lib():
         H.gpart_cache_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         H.gpart_refine_set_rebuild.argtypes = [C.c_void_p, C.c_int64, C.c_int64]
+        H.gpart_cache_info.argtypes = [C.c_void_p, C.c_void_p]
         _hip = H
     return _hip
 
@@ -211,9 +212,25 @@ def record_layout(vertices: int, bound: int):
     return vb.value, wb.value, mask.value
 
 
+class _CacheInfo(C.Structure):
+    _fields_ = [("reserved_bytes", C.c_uint64), ("live_bytes", C.c_uint64), ("free_bytes", C.c_uint64),
+                ("peak_reserved_bytes", C.c_uint64), ("peak_live_bytes", C.c_uint64), ("segments", C.c_int64),
+                ("live_blocks", C.c_int64), ("free_blocks", C.c_int64), ("segment_allocations", C.c_int64),
+                ("segment_allocation_seconds", C.c_double), ("segment_releases", C.c_int64),
+                ("out_of_memory_releases", C.c_int64)]
+
+
+def cache_info() -> dict:
+    """The calling thread's allocator counters: bytes reserved in segments, live and
+    free, their peaks, and the device allocations and releases made so far."""
+    info = _CacheInfo()
+    _check(hip_lib().gpart_cache_info(_ctx(), C.byref(info)))
+    return {name: getattr(info, name) for name, _ in _CacheInfo._fields_}
+
+
 def cache_stats() -> dict:
     """The calling thread's allocator: cached blocks/bytes, live bytes and the last
-    release's block count and seconds (returning blocks, stream wait, pool trim)."""
+    release's segment count and seconds (0, stream wait, freeing segments)."""
     blocks, cached, live, released = C.c_int64(), C.c_uint64(), C.c_uint64(), C.c_int64()
     secs = (C.c_double * 3)()
     _check(hip_lib().gpart_cache_stats(_ctx(), C.byref(blocks), C.byref(cached), C.byref(live),

@@ -2,12 +2,14 @@
 #pragma once
 
 #include "../../../include/gpart_hip.h"
+#include "block_cache.hpp"
 #include "common.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -16,25 +18,11 @@
 struct gpart_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // Caching sub-allocator. Blocks (rounded sizes) are carved from large slabs
-    // taken stream-ordered from the context's own HIP memory pool (release
-    // threshold UINT64_MAX); a freed block is cached by size. A release frees the
-    // slabs that hold no live block and trims the pool: a few HIP calls instead
-    // of one per block (each costs tens of microseconds of host time).
-    struct Slab {
-        char* base = nullptr;
-        std::size_t size = 0;
-        std::size_t used = 0;
-        std::int64_t live = 0;
-    };
-    hipMemPool_t pool = nullptr;
-    std::map<int, Slab> slabs;
-    int next_slab = 0;
-    std::multimap<std::size_t, std::pair<void*, int>> free_blocks;    // size -> (block, slab)
-    std::map<void*, std::pair<std::size_t, int>> live_blocks;         // block -> (size, slab)
-    std::size_t cached_bytes = 0;
-    std::size_t live_bytes = 0;
-    // Host seconds of the last release: returning blocks, synchronising, trimming.
+    // Best-fit caching of device memory in segments the context keeps until an
+    // explicit release (block_cache.hpp); all work of the context runs on its one
+    // stream, so a freed block may serve the next allocation without a wait.
+    std::unique_ptr<gpart_hip::BlockCache> cache;
+    // Host seconds of the last release: waiting for the stream, freeing segments.
     double release_seconds[3] = {0, 0, 0};
     std::int64_t released_blocks = 0;
     // Small pinned staging buffer for host-out scalars.

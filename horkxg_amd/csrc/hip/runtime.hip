@@ -75,97 +75,36 @@ void throw_hip(hipError_t error, const char* file, int line, const char* call) {
     throw Error(GPART_ERR_RUNTIME, s.str());
 }
 
-constexpr std::size_t kSlabBytes = std::size_t{256} << 20;
-
-// Frees every slab without live blocks (and its cached blocks) to the pool,
-// waits for the stream and trims the pool.
+// Returns every cached segment without live blocks to the device: waits for the
+// stream first, since freed blocks may still be read by queued work.
 void release_blocks(gpart_ctx* ctx) {
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
-    std::int64_t released = 0;
-    for (auto it = ctx->free_blocks.begin(); it != ctx->free_blocks.end();) {
-        if (ctx->slabs.at(it->second.second).live == 0) {
-            ctx->cached_bytes -= it->first;
-            it = ctx->free_blocks.erase(it);
-        } else {
-            ++it;
-        }
-    }
-    for (auto it = ctx->slabs.begin(); it != ctx->slabs.end();) {
-        if (it->second.live == 0) {
-            GPART_HIP_CHECK(hipFreeAsync(it->second.base, ctx->stream));
-            ++released;
-            it = ctx->slabs.erase(it);
-        } else {
-            ++it;
-        }
-    }
-    ctx->released_blocks = released;
-    const auto t1 = clock::now();
     GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto t1 = clock::now();
+    ctx->released_blocks = ctx->cache->release_free_segments();
     const auto t2 = clock::now();
-    GPART_HIP_CHECK(hipMemPoolTrimTo(ctx->pool, 0));
-    const auto t3 = clock::now();
-    ctx->release_seconds[0] = std::chrono::duration<double>(t1 - t0).count();
-    ctx->release_seconds[1] = std::chrono::duration<double>(t2 - t1).count();
-    ctx->release_seconds[2] = std::chrono::duration<double>(t3 - t2).count();
+    ctx->release_seconds[0] = 0.0;
+    ctx->release_seconds[1] = std::chrono::duration<double>(t1 - t0).count();
+    ctx->release_seconds[2] = std::chrono::duration<double>(t2 - t1).count();
 }
-
-namespace {
-// A block of `size` bytes from the first slab with room, or from a new slab.
-std::pair<void*, int> carve(gpart_ctx* ctx, std::size_t size) {
-    for (auto& [id, slab] : ctx->slabs)
-        if (slab.size - slab.used >= size) {
-            void* p = slab.base + slab.used;
-            slab.used += size;
-            return {p, id};
-        }
-    gpart_ctx::Slab slab;
-    slab.size = size > kSlabBytes ? size : kSlabBytes;
-    void* base = nullptr;
-    hipError_t e = hipMallocFromPoolAsync(&base, slab.size, ctx->pool, ctx->stream);
-    if (e != hipSuccess) {
-        // Return unused slabs to the pool, trim it and retry once at the exact size.
-        (void)hipGetLastError();
-        release_blocks(ctx);
-        slab.size = size;
-        GPART_HIP_CHECK(hipMallocFromPoolAsync(&base, slab.size, ctx->pool, ctx->stream));
-    }
-    slab.base = static_cast<char*>(base);
-    slab.used = size;
-    const int id = ctx->next_slab++;
-    ctx->slabs.emplace(id, slab);
-    return {base, id};
-}
-}  // namespace
 
 void* ctx_alloc(gpart_ctx* ctx, std::size_t bytes) {
-    const auto size = round_size(bytes);
-    auto it = ctx->free_blocks.find(size);
-    std::pair<void*, int> block;
-    if (it != ctx->free_blocks.end()) {
-        block = it->second;
-        ctx->free_blocks.erase(it);
-        ctx->cached_bytes -= size;
-    } else {
-        block = carve(ctx, size);
+    try {
+        return ctx->cache->allocate(bytes);
+    } catch (const std::runtime_error& e) {
+        if (dynamic_cast<const Error*>(&e)) throw;
+        throw Error(GPART_ERR_RUNTIME, e.what());
     }
-    ctx->slabs.at(block.second).live += 1;
-    ctx->live_blocks[block.first] = {size, block.second};
-    ctx->live_bytes += size;
-    return block.first;
 }
 
 void ctx_free(gpart_ctx* ctx, void* p) {
     if (!p) return;
-    auto it = ctx->live_blocks.find(p);
-    if (it == ctx->live_blocks.end()) throw Error(GPART_ERR_INVALID_ARGUMENT, "free of unknown device block");
-    const auto [size, slab] = it->second;
-    ctx->free_blocks.emplace(size, std::make_pair(p, slab));
-    ctx->slabs.at(slab).live -= 1;
-    ctx->cached_bytes += size;
-    ctx->live_bytes -= size;
-    ctx->live_blocks.erase(it);
+    try {
+        ctx->cache->free(p);
+    } catch (const std::invalid_argument& e) {
+        throw Error(GPART_ERR_INVALID_ARGUMENT, e.what());
+    }
 }
 
 void read_back(gpart_ctx* ctx, void* host, const void* device, std::size_t bytes) {
@@ -271,16 +210,21 @@ int gpart_ctx_create(int device, gpart_ctx** out) {
             // Default priority: no stream priorities anywhere in this library.
             GPART_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
             GPART_HIP_CHECK(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
-            hipMemPoolProps props{};
-            props.allocType = hipMemAllocationTypePinned;
-            props.handleTypes = hipMemHandleTypeNone;
-            props.location.type = hipMemLocationTypeDevice;
-            props.location.id = device;
-            GPART_HIP_CHECK(hipMemPoolCreate(&ctx->pool, &props));
-            std::uint64_t keep = ~std::uint64_t{0};
-            GPART_HIP_CHECK(hipMemPoolSetAttribute(ctx->pool, hipMemPoolAttrReleaseThreshold, &keep));
+            SegmentSource source;
+            source.allocate = [ctx](std::size_t bytes) -> void* {
+                void* p = nullptr;
+                const hipError_t e = hipMalloc(&p, bytes);
+                if (e == hipErrorOutOfMemory) {
+                    (void)hipGetLastError();
+                    return nullptr;
+                }
+                GPART_HIP_CHECK(e);
+                return p;
+            };
+            source.release = [](void* p) { GPART_HIP_CHECK(hipFree(p)); };
+            source.drain = [ctx] { GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream)); };
+            ctx->cache = std::make_unique<BlockCache>(std::move(source));
         } catch (...) {
-            if (ctx->pool) (void)hipMemPoolDestroy(ctx->pool);
             if (ctx->pinned) (void)hipHostFree(ctx->pinned);
             if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
             delete ctx;
@@ -294,9 +238,10 @@ int gpart_ctx_destroy(gpart_ctx* ctx) {
     if (!ctx) return GPART_OK;
     return guarded([&] {
         GPART_HIP_CHECK(hipSetDevice(ctx->device));
-        for (auto& kv : ctx->slabs) GPART_HIP_CHECK(hipFreeAsync(kv.second.base, ctx->stream));
         GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        GPART_HIP_CHECK(hipMemPoolDestroy(ctx->pool));
+        // Every segment goes, including those of blocks the caller never freed.
+        for (const auto& kv : ctx->cache->segments()) GPART_HIP_CHECK(hipFree(kv.first));
+        ctx->cache.reset();
         for (auto& p : ctx->profile)
             for (auto& e : p.events) {
                 GPART_HIP_CHECK(hipEventDestroy(e.first));
@@ -312,11 +257,31 @@ int gpart_ctx_destroy(gpart_ctx* ctx) {
 int gpart_cache_stats(gpart_ctx* ctx, int64_t* cached_blocks, uint64_t* cached_bytes, uint64_t* live_bytes,
                       int64_t* released_blocks, double* release_seconds) {
     return guarded([&] {
-        *cached_blocks = static_cast<int64_t>(ctx->free_blocks.size());
-        *cached_bytes = ctx->cached_bytes;
-        *live_bytes = ctx->live_bytes;
+        const auto st = ctx->cache->stats();
+        *cached_blocks = ctx->cache->free_blocks();
+        *cached_bytes = ctx->cache->free_bytes();
+        *live_bytes = st.live_bytes;
         *released_blocks = ctx->released_blocks;
         for (int i = 0; i < 3; ++i) release_seconds[i] = ctx->release_seconds[i];
+    });
+}
+
+int gpart_cache_info(gpart_ctx* ctx, gpart_cache_info_t* out) {
+    return guarded([&] {
+        if (!out) throw Error(GPART_ERR_INVALID_ARGUMENT, "null cache info");
+        const auto st = ctx->cache->stats();
+        out->reserved_bytes = st.reserved_bytes;
+        out->live_bytes = st.live_bytes;
+        out->free_bytes = ctx->cache->free_bytes();
+        out->peak_reserved_bytes = st.peak_reserved_bytes;
+        out->peak_live_bytes = st.peak_live_bytes;
+        out->segments = st.segments;
+        out->live_blocks = st.live_blocks;
+        out->free_blocks = ctx->cache->free_blocks();
+        out->segment_allocations = st.segment_allocations;
+        out->segment_allocation_seconds = st.segment_allocation_seconds;
+        out->segment_releases = st.segment_releases;
+        out->out_of_memory_releases = st.out_of_memory_releases;
     });
 }
 

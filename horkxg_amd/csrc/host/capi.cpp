@@ -96,16 +96,13 @@ void copy_level(const gpart::WeightedGraph<Types>& g, int64_t* offsets, int32_t*
     if (vw) std::copy(g.vertex_weights.begin(), g.vertex_weights.end(), vw);
 }
 
-// A batch worker returns its allocator cache once it holds more than this.
-constexpr std::uint64_t kWorkerCacheBytes = std::uint64_t{16} << 30;
-
 // One partition with the CLI's production options. Returns the labels on the
-// device. release: free the hierarchy's cached memory and trim the allocator
-// inside the timed scope (the reference's per-run scope); a batch worker keeps its
-// cache for the next graph instead.
+// device. The run ends, inside the timed scope, by returning every block of the
+// hierarchy to the context's cache (the reference's per-run scratch scope); the
+// cache keeps its device segments for the next run.
 template <typename Types>
 gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify,
-                                               int mode, bool release, gpart_partition_report* report) {
+                                               int mode, gpart_partition_report* report) {
     using clock = std::chrono::steady_clock;
     auto* ctx = gpart::detail::context();
     std::uint64_t total = 0, heaviest = 0, lightest = 0;
@@ -149,7 +146,6 @@ gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, dou
     hierarchy.levels.clear();
     hierarchy.fine_to_coarse.clear();
     auto partition = std::move(result.device_partition);
-    if (release) GPART_CHECK(gpart_release_cache(ctx));
     const auto t4 = clock::now();
     if (report) {
         report->final_cut = result.final_cut;
@@ -175,7 +171,7 @@ gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, dou
 template <typename Types>
 void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify, int mode, int32_t* labels,
                     gpart_partition_report* report) {
-    auto partition = run_pipeline(g, parts, ratio, seed, verify, mode, true, report);
+    auto partition = run_pipeline(g, parts, ratio, seed, verify, mode, report);
     if (labels) {
         const auto host = partition.to_host();
         std::copy(host.begin(), host.end(), labels);
@@ -392,17 +388,11 @@ int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* part
                     const int i = order[static_cast<std::size_t>(slot)];
                     auto part = std::visit(
                         [&](auto& x) {
-                            return run_pipeline(x, parts[i], ratios[i], seeds[i], 0, spectral_mode, false,
+                            return run_pipeline(x, parts[i], ratios[i], seeds[i], 0, spectral_mode,
                                                 reports ? reports + i : nullptr);
                         },
                         graphs[i]->g);
                     done.emplace_back(i, std::move(part));
-                    // blocks are reused by exact size only: return the cache once it is large
-                    std::int64_t blocks = 0, released = 0;
-                    std::uint64_t cached = 0, live = 0;
-                    double secs[3];
-                    GPART_CHECK(gpart_cache_stats(ctx, &blocks, &cached, &live, &released, secs));
-                    if (cached > kWorkerCacheBytes) GPART_CHECK(gpart_release_cache(ctx));
                 }
                 GPART_CHECK(gpart_ctx_synchronize(ctx));
             } catch (...) {

@@ -51,18 +51,38 @@ int gpart_ctx_destroy(gpart_ctx* ctx);
 int gpart_ctx_synchronize(gpart_ctx* ctx);
 void* gpart_ctx_stream(gpart_ctx* ctx);
 /* Stream-ordered caching allocator; replaces ScratchAllocator/ScratchPoolSession
- * (include/detail/scratch_pool.hpp:15-106). Freed blocks are reused by later
- * allocations on the same stream. */
+ * (include/detail/scratch_pool.hpp:15-106). Blocks are cut best-fit from device
+ * segments the context keeps; a freed block serves later allocations on the same
+ * stream. Segments go back to the device only through gpart_release_cache, context
+ * destruction, or an allocation the device cannot otherwise serve: ROCm scrubs
+ * recycled device memory inside the allocation call (about 30 ms per GiB on
+ * MI355X), so the reference's per-run pool trim would charge every run for it. */
 int gpart_device_alloc(gpart_ctx* ctx, size_t bytes, void** out);
 int gpart_device_free(gpart_ctx* ctx, void* ptr);
-/* Synchronise, then return every cached block to the device
+/* Synchronise, then return every segment without live blocks to the device
  * (ScratchPoolSession::finish, scratch_pool.hpp:42-49). */
 int gpart_release_cache(gpart_ctx* ctx);
 /* Allocator state: cached (free) blocks and bytes, live bytes, and for the last
- * release the number of slabs freed and host seconds spent returning them, waiting for
- * the stream and trimming the pool (release_seconds[3]). */
+ * release the number of segments freed and host seconds (0, waiting for the stream,
+ * freeing segments) in release_seconds[3]. */
 int gpart_cache_stats(gpart_ctx* ctx, int64_t* cached_blocks, uint64_t* cached_bytes, uint64_t* live_bytes,
                       int64_t* released_blocks, double* release_seconds);
+/* Full allocator counters (no counterpart in the reference). */
+typedef struct gpart_cache_info_t {
+    uint64_t reserved_bytes;        /* held in segments */
+    uint64_t live_bytes;            /* in live blocks */
+    uint64_t free_bytes;            /* in cached free blocks */
+    uint64_t peak_reserved_bytes;
+    uint64_t peak_live_bytes;
+    int64_t segments;
+    int64_t live_blocks;
+    int64_t free_blocks;
+    int64_t segment_allocations;    /* device allocations made, all time */
+    double segment_allocation_seconds; /* host seconds inside them */
+    int64_t segment_releases;       /* segments returned, all time */
+    int64_t out_of_memory_releases; /* releases forced by a failed device allocation */
+} gpart_cache_info_t;
+int gpart_cache_info(gpart_ctx* ctx, gpart_cache_info_t* out);
 /* Tuning of the refinement rounds (no counterpart in the reference, whose rounds
  * always update the cache with atomics): a round rebuilds the n x k cache from the
  * CSR instead when the level has >= min_entries entries and its movers hold more
