@@ -351,6 +351,16 @@ def main() -> int:
     independent_cut = graph.cut(labels)
     assert independent_cut == cut, (independent_cut, cut)
     assert loads.max() <= cap, (loads.max(), cap)
+    # the GPU verifier and quality metrics on the final labels (host labels uploaded)
+    t = time.perf_counter()
+    verified = graph.verify(labels, k, str(args.ratio))
+    verify_seconds = time.perf_counter() - t
+    t = time.perf_counter()
+    quality = graph.analyze(labels, k)
+    analyze_seconds = time.perf_counter() - t
+    # unit weights: the verifier's entry count is the partition's cut
+    assert verified["cut"] == cut and verified["cut"] * 2 == quality["edge_cut"], (verified["cut"], cut)
+    assert verified["capacity_ok"] and verified["max_load"] == loads.max()
 
     roofline = None
     if fam:
@@ -429,6 +439,13 @@ def main() -> int:
         "stage_seconds": {s: round(float(np.mean([r[s] for r in reports])), 5)
                           for s in ("coarsen_seconds", "initial_seconds", "uncoarsen_seconds", "free_seconds")},
         "algorithm_seconds_min": round(float(alg.min()), 6),
+        "verify": {"cut": verified["cut"], "capacity": verified["capacity"],
+                   "capacity_ok": verified["capacity_ok"], "seconds": round(verify_seconds, 4)},
+        "quality": {"edge_cut_ratio": round(quality["edge_cut_ratio"], 6),
+                    "vertex_imbalance": round(quality["vertex_imbalance"], 6),
+                    "edge_set_imbalance": round(quality["edge_set_imbalance"], 6),
+                    "max_local_cut": quality["max_local_cut"], "avg_local_cut": quality["avg_local_cut"],
+                    "seconds": round(analyze_seconds, 4)},
         "cold_step_seconds": (round(float(cold_seconds), 6) if cold_seconds is not None else None),
         "device_memory": {
             "peak_reserved_gib": round(alloc_after["peak_reserved_bytes"] / 2**30, 3),

@@ -32,6 +32,7 @@ from ._native import (  # noqa: F401
     profile_reset,
     record_layout,
     cache_info,
+    decimal_capacity,
     cache_stats,
     set_log,
     set_refine_rebuild,

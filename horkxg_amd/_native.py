@@ -47,6 +47,27 @@ class PartitionReport(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class VerifyReport(C.Structure):
+    _fields_ = [("vertices", C.c_int64), ("undirected_edges", C.c_int64), ("cut", C.c_int64),
+                ("cut_ratio", C.c_double), ("max_load", C.c_int64), ("max_load_ratio", C.c_double),
+                ("capacity", C.c_int64), ("capacity_ok", C.c_int32), ("parts", C.c_int32)]
+
+
+class QualityReport(C.Structure):
+    _fields_ = [("parts", C.c_int32), ("pad", C.c_int32), ("vertices", C.c_int64),
+                ("directed_edges", C.c_int64), ("edge_cut", C.c_uint64), ("weighted_edge_cut", C.c_uint64),
+                ("edge_cut_ratio", C.c_double), ("avg_local_cut", C.c_double), ("max_local_cut", C.c_uint64),
+                ("weighted_avg_local_cut", C.c_double), ("weighted_max_local_cut", C.c_uint64),
+                ("vertex_avg", C.c_double), ("min_vertices", C.c_int64), ("max_vertices", C.c_int64),
+                ("vertex_imbalance", C.c_double), ("vertex_max_deviation", C.c_double),
+                ("edge_set_avg", C.c_double), ("min_edge_set", C.c_int64), ("max_edge_set", C.c_int64),
+                ("edge_set_imbalance", C.c_double), ("edge_set_max_deviation", C.c_double)]
+
+
+PER_PART_COUNTERS = ("vertices", "internal", "outgoing", "incoming", "weighted_internal", "weighted_outgoing",
+                     "weighted_incoming")
+
+
 class SpectralLevelError(C.Structure):
     _fields_ = [("depth", C.c_int32), ("tasks", C.c_int32), ("vertices", C.c_int64),
                 ("max_abs_error", C.c_double), ("rms_error", C.c_double), ("splits_differ", C.c_int32),
@@ -134,6 +155,11 @@ def host_lib():
         L.gpart_graph_edges.restype = C.c_int64
         L.gpart_graph_edges.argtypes = [C.c_void_p]
         L.gpart_graph_cut.argtypes = [C.c_void_p, _i32p, C.POINTER(C.c_uint64)]
+        L.gpart_decimal_capacity.argtypes = [C.c_char_p, C.c_int64, C.c_int, C.POINTER(C.c_int64)]
+        L.gpart_graph_verify.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_char_p, C.POINTER(VerifyReport),
+                                         C.c_void_p]
+        L.gpart_graph_analyze.argtypes = [C.c_void_p, _i32p, C.c_int, C.POINTER(QualityReport), C.c_void_p,
+                                          C.c_int]
         L.gpart_graph_partition.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_int, C.c_int,
                                             C.c_void_p, C.POINTER(PartitionReport)]
         L.gpart_graph_generate_rmat.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
@@ -203,6 +229,13 @@ def device_count() -> int:
 def partition_capacity(total: int, parts: int, ratio: float) -> int:
     out = C.c_uint64()
     _check(host_lib().gpart_partition_capacity_c(total, parts, ratio, C.byref(out)))
+    return out.value
+
+
+def decimal_capacity(imbalance: str, n: int, parts: int) -> int:
+    """floor(imbalance * n / parts) for a decimal string, exactly (host only)."""
+    out = C.c_int64()
+    _check(host_lib().gpart_decimal_capacity(str(imbalance).encode(), int(n), int(parts), C.byref(out)))
     return out.value
 
 
@@ -474,6 +507,39 @@ class Graph:
         out = C.c_uint64()
         _check(host_lib().gpart_graph_cut(self._h, np.ascontiguousarray(labels, dtype=np.int32), C.byref(out)))
         return out.value
+
+    def verify(self, labels, parts: int, imbalance: str = "1.03") -> dict:
+        """tools/verify_partition.py:22-65 on the device: the same checks, error
+        messages (GpartError, kind "invalid_argument") and result dictionary."""
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        if labels.size != self.n:
+            raise GpartError(1, "partition label count does not match vertices")
+        rep = VerifyReport()
+        loads = np.zeros(max(int(parts), 1), np.int64)
+        _check(host_lib().gpart_graph_verify(self._h, labels, int(parts), str(imbalance).encode(), C.byref(rep),
+                                             loads.ctypes.data))
+        return {
+            "vertices": rep.vertices, "undirected_edges": rep.undirected_edges, "cut": rep.cut,
+            "cut_ratio": rep.cut_ratio, "loads": [int(x) for x in loads[:parts]], "max_load": rep.max_load,
+            "max_load_ratio": rep.max_load_ratio, "capacity": rep.capacity, "capacity_ok": bool(rep.capacity_ok),
+        }
+
+    def analyze(self, labels, parts: int = 0) -> dict:
+        """graph_untils/analyze_partition.cpp:176-294 on the device; parts == 0 infers
+        max label + 1. Weighted metrics use the graph's edge weights. per_part holds
+        the per-part counters as int64 arrays."""
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        if labels.size != self.n:
+            raise GpartError(1, "Partition line count must equal CSR vertex count")
+        cap = max(int(parts), int(labels.max()) + 1 if labels.size else 0, 1)
+        per_part = np.zeros(7 * cap, np.uint64)
+        rep = QualityReport()
+        _check(host_lib().gpart_graph_analyze(self._h, labels, int(parts), C.byref(rep), per_part.ctypes.data, cap))
+        out = {name: getattr(rep, name) for name, _ in QualityReport._fields_ if name != "pad"}
+        k = rep.parts
+        out["per_part"] = {name: per_part[i * k:(i + 1) * k].astype(np.int64)
+                           for i, name in enumerate(PER_PART_COUNTERS)}
+        return out
 
     def partition(self, parts: int, ratio: float = 1.03, seed: int = 0, verify: bool = False,
                   spectral_mode: str = "batched_fp64", want_labels: bool = True):

@@ -9,6 +9,7 @@
 #include "coarsen.hpp"
 #include "initial_partition.hpp"
 #include "partition_capacity.hpp"
+#include "quality.hpp"
 #include "uncoarsen.hpp"
 
 #include <algorithm>
@@ -329,6 +330,74 @@ int gpart_graph_cut(gpart_graph* g, const int32_t* labels, uint64_t* cut) {
                 gpart::DeviceVector<std::int32_t> d(std::vector<std::int32_t>(labels, labels + x.device.vertices()));
                 const auto view = x.device.view();
                 GPART_CHECK(gpart_cut(gpart::detail::context(), &view, d.data(), cut));
+            },
+            g->g);
+    });
+}
+
+int gpart_decimal_capacity(const char* imbalance, int64_t n, int parts, int64_t* capacity) {
+    return guard([&] {
+        if (!imbalance || !capacity) throw std::invalid_argument("null imbalance or output");
+        if (!gpart::decimal_capacity(imbalance, n, parts, capacity))
+            throw std::invalid_argument("invalid parts or imbalance ratio");
+    });
+}
+
+int gpart_graph_verify(gpart_graph* g, const int32_t* labels, int parts, const char* imbalance,
+                       gpart_verify_report* out, int64_t* loads) {
+    return guard([&] {
+        if (!out || !imbalance) throw std::invalid_argument("null output or imbalance");
+        std::visit(
+            [&](auto& x) {
+                const auto n = x.device.vertices();
+                if (n > 0 && !labels) throw std::invalid_argument("null labels");
+                gpart::DeviceVector<std::int32_t> d(std::vector<std::int32_t>(labels, labels + n));
+                const auto view = x.device.view();
+                const auto r = gpart::verify_partition(gpart::detail::context(), view, d.data(), labels, parts,
+                                                       imbalance);
+                *out = gpart_verify_report{r.vertices, r.undirected_edges, r.cut, r.cut_ratio, r.max_load,
+                                           r.max_load_ratio, r.capacity, r.capacity_ok ? 1 : 0, parts};
+                if (loads) std::copy(r.loads.begin(), r.loads.end(), loads);
+            },
+            g->g);
+    });
+}
+
+int gpart_graph_analyze(gpart_graph* g, const int32_t* labels, int parts, gpart_quality_report* out,
+                        uint64_t* per_part, int per_part_capacity) {
+    return guard([&] {
+        if (!out) throw std::invalid_argument("null output");
+        std::visit(
+            [&](auto& x) {
+                const auto n = x.device.vertices();
+                if (n > 0 && !labels) throw std::invalid_argument("null labels");
+                gpart::DeviceVector<std::int32_t> d(std::vector<std::int32_t>(labels, labels + n));
+                const auto view = x.device.view();
+                const auto r = gpart::analyze_partition(gpart::detail::context(), view, d.data(), parts);
+                if (per_part && per_part_capacity < r.parts)
+                    throw std::invalid_argument("per-part buffer holds fewer parts than the labelling");
+                *out = gpart_quality_report{};
+                out->parts = r.parts;
+                out->vertices = r.vertices;
+                out->directed_edges = r.directed_edges;
+                out->edge_cut = r.edge_cut;
+                out->weighted_edge_cut = r.weighted_edge_cut;
+                out->edge_cut_ratio = r.edge_cut_ratio;
+                out->avg_local_cut = r.avg_local_cut;
+                out->max_local_cut = r.max_local_cut;
+                out->weighted_avg_local_cut = r.weighted_avg_local_cut;
+                out->weighted_max_local_cut = r.weighted_max_local_cut;
+                out->vertex_avg = r.vertex_avg;
+                out->min_vertices = r.min_vertices;
+                out->max_vertices = r.max_vertices;
+                out->vertex_imbalance = r.vertex_imbalance;
+                out->vertex_max_deviation = r.vertex_max_deviation;
+                out->edge_set_avg = r.edge_set_avg;
+                out->min_edge_set = r.min_edge_set;
+                out->max_edge_set = r.max_edge_set;
+                out->edge_set_imbalance = r.edge_set_imbalance;
+                out->edge_set_max_deviation = r.edge_set_max_deviation;
+                if (per_part) std::copy(r.per_part.begin(), r.per_part.end(), per_part);
             },
             g->g);
     });

@@ -51,12 +51,18 @@ protected:
     int overflow(int c) override { return c == traits_type::eof() ? 0 : c; }
     std::streamsize xsputn(const char*, std::streamsize n) override { return n; }
 };
-NullBuffer null_buffer;
-std::ostream null_stream(&null_buffer);
 std::atomic<bool> log_enabled{true};
+
+// One discarding stream per thread: an ostream's formatting state is not safe to
+// share between threads (std::cout alone is synchronised by the standard).
+std::ostream& null_stream() {
+    thread_local NullBuffer buffer;
+    thread_local std::ostream stream(&buffer);
+    return stream;
+}
 }  // namespace
 
-std::ostream& log() { return log_enabled.load(std::memory_order_relaxed) ? std::cout : null_stream; }
+std::ostream& log() { return log_enabled.load(std::memory_order_relaxed) ? std::cout : null_stream(); }
 
 gpart_ctx* context() {
     if (!current.ctx) {

@@ -68,6 +68,63 @@ int64_t gpart_graph_vertices(const gpart_graph* g);
 int64_t gpart_graph_edges(const gpart_graph* g);
 int gpart_graph_cut(gpart_graph* g, const int32_t* labels, uint64_t* cut);
 
+/* tools/verify_partition.py:22-65 on the device: labels (host, n entries) must lie
+ * in [0, parts), neighbours in [0, n), the directed cut and non-loop entry counts
+ * must be even; capacity = floor(imbalance * n / parts) with imbalance a decimal
+ * string, exactly as Decimal computes it. Errors carry the reference's messages
+ * ("invalid parts or imbalance ratio", "invalid label: <value>", "invalid neighbor
+ * id", "odd undirected edge/cut count; input must be symmetric"). loads (host, parts
+ * entries) may be NULL. */
+/* floor(imbalance * n / parts) for a decimal string, exactly (the verifier's
+ * capacity, tools/verify_partition.py:57). Host only. Returns GPART_OK, or an
+ * invalid-argument status when imbalance is not a finite decimal >= 1. */
+int gpart_decimal_capacity(const char* imbalance, int64_t n, int parts, int64_t* capacity);
+
+typedef struct gpart_verify_report {
+    int64_t vertices;
+    int64_t undirected_edges;
+    int64_t cut;
+    double cut_ratio;
+    int64_t max_load;
+    double max_load_ratio;
+    int64_t capacity;
+    int32_t capacity_ok;
+    int32_t parts;
+} gpart_verify_report;
+int gpart_graph_verify(gpart_graph* g, const int32_t* labels, int parts, const char* imbalance,
+                       gpart_verify_report* out, int64_t* loads);
+
+/* graph_untils/analyze_partition.cpp:176-294 on the device. parts == 0 infers max
+ * label + 1. The weighted metrics use the graph's edge weights (unit weights give
+ * the tool's default output). per_part (host, may be NULL) receives 7 x out->parts
+ * words, as gpart_part_counters writes them, when per_part_capacity >= out->parts
+ * (otherwise it is an error). */
+typedef struct gpart_quality_report {
+    int32_t parts;
+    int32_t pad;
+    int64_t vertices;
+    int64_t directed_edges;
+    uint64_t edge_cut;
+    uint64_t weighted_edge_cut;
+    double edge_cut_ratio;
+    double avg_local_cut;
+    uint64_t max_local_cut;
+    double weighted_avg_local_cut;
+    uint64_t weighted_max_local_cut;
+    double vertex_avg;
+    int64_t min_vertices;
+    int64_t max_vertices;
+    double vertex_imbalance;
+    double vertex_max_deviation;
+    double edge_set_avg;
+    int64_t min_edge_set;
+    int64_t max_edge_set;
+    double edge_set_imbalance;
+    double edge_set_max_deviation;
+} gpart_quality_report;
+int gpart_graph_analyze(gpart_graph* g, const int32_t* labels, int parts, gpart_quality_report* out,
+                        uint64_t* per_part, int per_part_capacity);
+
 typedef struct gpart_partition_report {
     uint64_t final_cut;
     uint64_t max_part_weight;

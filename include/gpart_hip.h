@@ -162,6 +162,29 @@ typedef struct gpart_degree_classes {
 int gpart_degree_classes_build(gpart_ctx* ctx, const gpart_csr_view* g,
                                gpart_degree_classes* classes);
 
+/* ---------------- partition verification and quality ---------------- */
+
+/* Counters of a labelling for the verifier (tools/verify_partition.py:33-54) and
+ * the quality metrics (graph_untils/analyze_partition.cpp:219-254). */
+typedef struct gpart_part_counters_t {
+    int64_t first_invalid_label;    /* first vertex whose label is outside [0, parts), -1 if none */
+    int64_t first_invalid_neighbor; /* first entry whose neighbour is outside [0, n), -1 if none */
+    int32_t min_label;              /* over all vertices; 0 and -1 when n == 0 */
+    int32_t max_label;
+    uint64_t nonloop_entries;       /* entries with neighbour != row */
+    uint64_t directed_cut;          /* entries whose endpoints carry different valid labels */
+    uint64_t weighted_directed_cut; /* their edge weights */
+} gpart_part_counters_t;
+
+/* Two device passes over the labels (device, n) and the CSR. per_part (host, may
+ * be NULL) receives 7 x parts words: vertices, internal entries, outgoing cut
+ * entries, incoming cut entries, then the edge weights of the last three, each
+ * indexed by part. Entries with an out-of-range neighbour or an endpoint whose
+ * label is outside [0, parts) are left out of the per-part counters. parts == 0
+ * computes the label range only. host out: *out, per_part. */
+int gpart_part_counters(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, int parts,
+                        gpart_part_counters_t* out, uint64_t* per_part);
+
 /* ---------------- synthetic graphs on the device ---------------- */
 
 /* The host generators' graphs (gpart_generate_rmat/grid2d/mesh3d of gpart.h),
