@@ -395,9 +395,9 @@ class Hierarchy:
 
 
 def partition_batch(graphs, parts, ratios=None, seeds=None, spectral_mode: str = "batched_fp32",
-                    workers: int = 4, want_labels: bool = True):
-    """Partition many graphs, largest first, on `workers` host threads (one HIP
-    stream and allocator each). Returns (labels list or None, report dicts, wall
+                    workers: int = 1, want_labels: bool = True):
+    """Partition many graphs, largest first, in the calling thread (workers must
+    be 1: the concurrent form is switched off). Returns (labels list or None, report dicts, wall
     seconds from the first coarsening start to the last labels on the device)."""
     count = len(graphs)
     parts = list(parts)

@@ -51,6 +51,29 @@ def generate(hx, spec: dict):
     return hx.generate_rmat(scale, edge_factor, 0.57, 0.19, 0.19, spec.get("seed", 0))
 
 
+def device_graph(hx, spec: dict):
+    """The workload graph of `spec`, generated on the GPU (identical to generate())."""
+    if spec["kind"] == "mesh3d":
+        return hx.Graph.generate_mesh3d(*spec["size"])
+    if spec["kind"] == "grid2d":
+        return hx.Graph.generate_grid2d(*spec["size"])
+    scale, edge_factor = spec["size"]
+    return hx.Graph.generate_rmat(scale, edge_factor, 0.57, 0.19, 0.19, spec.get("seed", 0))
+
+
+def assign_largest_first(specs, sizes, workers: int):
+    """Spec indices per worker: graphs by decreasing size, each to the least loaded
+    worker (size = vertices + entries)."""
+    order = sorted(range(len(specs)), key=lambda i: -sizes[i])
+    load = [0] * workers
+    out = [[] for _ in range(workers)]
+    for i in order:
+        w = min(range(workers), key=lambda j: load[j])
+        out[w].append(i)
+        load[w] += sizes[i]
+    return out
+
+
 def upload_mix(hx, specs, threads: int = 8):
     """Generate and upload every graph of a mix; host arrays are dropped once the
     graph holds its own copies. Returns the hx.Graph list in spec order."""
