@@ -43,8 +43,14 @@ int main(int argc, char** argv) {
         const int mode = mode_env ? std::atoi(mode_env) : 1;
         if (mode < 0 || mode > 2) throw std::invalid_argument("GPART_SPECTRAL_MODE must be 0, 1 or 2");
         const auto load_start = std::chrono::steady_clock::now();
-        auto graph = gpart::load_weighted_graph<Types>(argv[1], argv[2]);
-        gpart::partition_capacity(static_cast<std::uint64_t>(graph.vertices()), parts, imbalance);
+        // Without --verify the input is narrowed and shape-checked on the device (only
+        // the int64 files cross PCIe); --verify keeps the host copy for the symmetry check.
+        gpart::WeightedGraph<Types> graph;
+        gpart::DeviceWeightedGraph<Types> device_graph;
+        if (verify) graph = gpart::load_weighted_graph<Types>(argv[1], argv[2]);
+        else device_graph = gpart::load_device_graph<Types>(argv[1], argv[2]);
+        const std::int64_t vertices = verify ? graph.vertices() : device_graph.vertices();
+        gpart::partition_capacity(static_cast<std::uint64_t>(vertices), parts, imbalance);
         const auto load_end = std::chrono::steady_clock::now();
         std::cout << "verify=" << (verify ? 1 : 0) << '\n';
         gpart::CoarsenOptions coarsen_options;
@@ -57,9 +63,10 @@ int main(int argc, char** argv) {
         std::cout << "pipeline_mode=incremental_bucket cluster_size_factor=" << coarsen_options.cluster_size_factor
                   << " pack_isolated=" << coarsen_options.pack_isolated << " spectral_mode=" << mode << '\n';
         const auto prepare_start = std::chrono::steady_clock::now();
-        if (verify) gpart::validate_weighted_csr(graph, true);
-        else gpart::validate_weighted_shape(graph, true);
-        auto device_graph = gpart::make_device_weighted(graph);
+        if (verify) {
+            gpart::validate_weighted_csr(graph, true);
+            device_graph = gpart::make_device_weighted(graph);
+        }
         GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context()));
         const auto coarsen_start = std::chrono::steady_clock::now();
         auto hierarchy = gpart::coarsen<Types>(std::move(device_graph), coarsen_options);

@@ -252,10 +252,16 @@ int gpart_graph_upload(int width, int64_t n, const int64_t* indptr, int64_t m, c
             auto build = [&](auto tag) {
                 using T = decltype(tag);
                 GraphOf<T> holder;
-                holder.host = host_graph<T>(n, indptr, m, indices, edge_weights, vertex_weights);
-                if (validate) gpart::validate_weighted_csr(holder.host, true);
-                else gpart::validate_weighted_shape(holder.host, true);
-                holder.device = gpart::make_device_weighted(holder.host);
+                if (!edge_weights && !vertex_weights && !validate) {
+                    // unit weights, shape checks only: narrowed on the device
+                    holder.device = gpart::device_graph_from_int64<T>(n, indptr, m, indices);
+                    holder.host_ready = false;
+                } else {
+                    holder.host = host_graph<T>(n, indptr, m, indices, edge_weights, vertex_weights);
+                    if (validate) gpart::validate_weighted_csr(holder.host, true);
+                    else gpart::validate_weighted_shape(holder.host, true);
+                    holder.device = gpart::make_device_weighted(holder.host);
+                }
                 g->g = std::move(holder);
             };
             if (width == GPART_WIDTH_STANDARD) build(gpart::StandardTypes{});
@@ -296,10 +302,14 @@ int gpart_graph_load(int width, const char* indptr_path, const char* indices_pat
             auto build = [&](auto tag) {
                 using T = decltype(tag);
                 GraphOf<T> holder;
-                holder.host = gpart::load_weighted_graph<T>(indptr_path, indices_path);
-                if (validate) gpart::validate_weighted_csr(holder.host, true);
-                else gpart::validate_weighted_shape(holder.host, true);
-                holder.device = gpart::make_device_weighted(holder.host);
+                if (!validate) {
+                    holder.device = gpart::load_device_graph<T>(indptr_path, indices_path);
+                    holder.host_ready = false;
+                } else {
+                    holder.host = gpart::load_weighted_graph<T>(indptr_path, indices_path);
+                    gpart::validate_weighted_csr(holder.host, true);
+                    holder.device = gpart::make_device_weighted(holder.host);
+                }
                 g->g = std::move(holder);
             };
             if (width == GPART_WIDTH_STANDARD) build(gpart::StandardTypes{});

@@ -1,6 +1,7 @@
 // graph.cpp -- input boundary: int64 CSR files, validation, host cut, transfers
 // (reference: src/graph.cpp:47-241; same checks, messages and unit weights).
 #include "log.hpp"
+#include "check.hpp"
 #include "graph.hpp"
 
 #include <algorithm>
@@ -135,29 +136,45 @@ void validate_weighted_shape(const WeightedGraph<Types>& graph, bool allow_self_
     const auto n = graph.vertices();
     for (std::int64_t v = 0; v < n; ++v)
         if (graph.offsets[v] > graph.offsets[v + 1]) throw std::runtime_error("weighted CSR offsets are not monotone");
+    const auto m = graph.edges();
     unsigned invalid = 0;
-    for (const auto u : graph.neighbors) invalid |= static_cast<unsigned>(u < 0 || u >= n);
+#pragma omp parallel for reduction(| : invalid) schedule(static)
+    for (std::int64_t e = 0; e < m; ++e) invalid |= static_cast<unsigned>(graph.neighbors[e] < 0 || graph.neighbors[e] >= n);
     if (invalid) throw std::runtime_error("weighted CSR contains an invalid neighbor");
     if (allow_self_loops) return;
+    unsigned loops = 0;
+#pragma omp parallel for reduction(| : loops) schedule(dynamic, 4096)
     for (std::int64_t v = 0; v < n; ++v)
-        for (auto e = graph.offsets[v]; e < graph.offsets[v + 1]; ++e)
-            if (graph.neighbors[e] == v) throw std::runtime_error("contracted graph contains a self loop");
+        for (auto e = graph.offsets[v]; e < graph.offsets[v + 1]; ++e) loops |= static_cast<unsigned>(graph.neighbors[e] == v);
+    if (loops) throw std::runtime_error("contracted graph contains a self loop");
 }
 
 template <typename Types>
 void validate_weighted_csr(const WeightedGraph<Types>& graph, bool allow_self_loops) {
     validate_weighted_shape(graph, allow_self_loops);
     const auto n = graph.vertices();
-    for (std::int64_t v = 0; v < n; ++v) {
+    // 0 when every entry of row v has its reverse entry with the same weight, else 1
+    // (no reverse entry) or 2 (weights differ) for the row's first failing entry
+    auto row_error = [&](std::int64_t v) -> int {
         for (auto e = graph.offsets[v]; e < graph.offsets[v + 1]; ++e) {
             const auto u = graph.neighbors[e];
             const auto rb = graph.neighbors.begin() + graph.offsets[u];
             const auto re = graph.neighbors.begin() + graph.offsets[u + 1];
             const auto r = std::lower_bound(rb, re, static_cast<typename Types::VertexT>(v));
-            if (r == re || *r != v) throw std::runtime_error("weighted CSR is not bidirectional");
+            if (r == re || *r != v) return 1;
             if (graph.edge_weights[e] != graph.edge_weights[static_cast<std::size_t>(r - graph.neighbors.begin())])
-                throw std::runtime_error("weighted CSR reverse edge weights differ");
+                return 2;
         }
+        return 0;
+    };
+    // the first failing row in vertex order decides the message, as a serial scan would
+    std::int64_t first = n;
+#pragma omp parallel for reduction(min : first) schedule(dynamic, 4096)
+    for (std::int64_t v = 0; v < n; ++v)
+        if (v < first && row_error(v)) first = v;
+    if (first < n) {
+        if (row_error(first) == 1) throw std::runtime_error("weighted CSR is not bidirectional");
+        throw std::runtime_error("weighted CSR reverse edge weights differ");
     }
 }
 
@@ -171,6 +188,63 @@ std::uint64_t host_cut(const WeightedGraph<Types>& graph, const std::vector<type
             if (labels[v] != labels[graph.neighbors[e]]) directed += static_cast<std::uint64_t>(graph.edge_weights[e]);
     if (directed & 1ULL) throw std::runtime_error("weighted CSR cut is not symmetric");
     return directed / 2;
+}
+
+namespace {
+// gpart_csr_from_int64 status bits, mapped to the host loader's messages in the
+// host loader's order of checks.
+template <typename Types>
+[[noreturn]] void throw_int64_status(int status) {
+    if (status & 8) throw std::runtime_error("CSR offsets do not match indices");
+    if (status & 1) throw std::runtime_error("CSR offsets are not monotone");
+    if (status & 2) {
+        if constexpr (std::is_same_v<Types, StandardTypes>) standard_limit_error();
+        throw std::runtime_error("CSR offset exceeds configured graph type");
+    }
+    throw std::runtime_error("CSR contains an invalid neighbor");
+}
+
+template <typename Types>
+DeviceWeightedGraph<Types> adopt_csr(const gpart_device_csr& csr) {
+    using O = typename Types::OffsetT;
+    using W = typename Types::WeightT;
+    DeviceWeightedGraph<Types> out;
+    const auto n = static_cast<std::size_t>(csr.n), m = static_cast<std::size_t>(csr.m);
+    out.offsets = DeviceVector<O>::adopt(static_cast<O*>(csr.offsets), n + 1);
+    out.neighbors = DeviceVector<std::int32_t>::adopt(csr.neighbors, m);
+    out.edge_weights = DeviceVector<W>::adopt(static_cast<W*>(csr.edge_weights), m);
+    out.vertex_weights = DeviceVector<W>::adopt(static_cast<W*>(csr.vertex_weights), n);
+    return out;
+}
+}  // namespace
+
+template <typename Types>
+DeviceWeightedGraph<Types> device_graph_from_int64(std::int64_t vertex_count, const std::int64_t* indptr,
+                                                   std::int64_t entries, const std::int64_t* indices) {
+    if (vertex_count < 0) throw std::runtime_error("empty CSR indptr");
+    check_dimensions<Types>(vertex_count, entries);
+    gpart_device_csr csr{};
+    std::int32_t status = 0;
+    GPART_CHECK(gpart_csr_from_int64(detail::context(), Types::kWidth, vertex_count, indptr, entries, indices, &csr,
+                                     &status));
+    if (status) throw_int64_status<Types>(status);
+    return adopt_csr<Types>(csr);
+}
+
+template <typename Types>
+DeviceWeightedGraph<Types> load_device_graph(const std::string& indptr_path, const std::string& indices_path) {
+    const auto offset_count = element_count(indptr_path, sizeof(std::int64_t));
+    const auto edge_count = element_count(indices_path, sizeof(std::int64_t));
+    if (offset_count < 1) throw std::runtime_error("empty CSR indptr");
+    const auto vertex_count = offset_count - 1;
+    check_dimensions<Types>(vertex_count, edge_count);
+    gpart_device_csr csr{};
+    std::int32_t status = 0;
+    GPART_CHECK(gpart_csr_from_int64_files(detail::context(), Types::kWidth, indptr_path.c_str(),
+                                           indices_path.c_str(), vertex_count, edge_count, &csr, &status));
+    if (status) throw_int64_status<Types>(status);
+    detail::log() << "vertices=" << vertex_count << " edge_entries=" << edge_count << " device_count=1\n";
+    return adopt_csr<Types>(csr);
 }
 
 template <typename Types>
@@ -201,6 +275,9 @@ WeightedGraph<Types> copy_device_weighted(const DeviceWeightedGraph<Types>& grap
     template void validate_weighted_csr<T>(const WeightedGraph<T>&, bool);                                     \
     template std::uint64_t host_cut<T>(const WeightedGraph<T>&, const std::vector<typename T::VertexT>&);     \
     template DeviceWeightedGraph<T> make_device_weighted<T>(const WeightedGraph<T>&);                          \
+    template DeviceWeightedGraph<T> device_graph_from_int64<T>(std::int64_t, const std::int64_t*, std::int64_t, \
+                                                               const std::int64_t*);                            \
+    template DeviceWeightedGraph<T> load_device_graph<T>(const std::string&, const std::string&);              \
     template WeightedGraph<T> copy_device_weighted<T>(const DeviceWeightedGraph<T>&);
 GPART_INSTANTIATE_GRAPH(StandardTypes)
 GPART_INSTANTIATE_GRAPH(BigTypes)

@@ -162,6 +162,34 @@ typedef struct gpart_degree_classes {
 int gpart_degree_classes_build(gpart_ctx* ctx, const gpart_csr_view* g,
                                gpart_degree_classes* classes);
 
+/* ---------------- int64 CSR input on the device ---------------- */
+
+/* Device arrays of one CSR, context allocations owned by the caller. */
+typedef struct gpart_device_csr {
+    void* offsets;          /* n + 1 entries of the width's offset type */
+    int32_t* neighbors;     /* m */
+    void* edge_weights;     /* m entries of the width's weight type, all 1 */
+    void* vertex_weights;   /* n entries of the width's weight type, all 1 */
+    int64_t n;
+    int64_t m;
+} gpart_device_csr;
+
+/* The narrowing of src/graph.cpp:95-116 (int64 indptr / indices to the width's
+ * offsets and int32 neighbours, unit weights), done on the device: the int64
+ * arrays stream through device staging in 64 MiB chunks and a kernel narrows each.
+ * indptr has n + 1 entries, indices m; host memory (pageable). host out: *status, a
+ * bit set: 1 offsets decrease or are negative, 2 an offset exceeds the width's
+ * offset type, 4 a neighbour lies outside [0, n), 8 indptr[0] != 0 or indptr[n] !=
+ * m. On a non-zero status nothing is allocated; dimension limits are the caller's
+ * check. */
+int gpart_csr_from_int64(gpart_ctx* ctx, int width, int64_t n, const int64_t* indptr, int64_t m,
+                         const int64_t* indices, gpart_device_csr* out, int32_t* status);
+/* The same from two int64 files of n + 1 and m entries (src/graph.cpp:47-122),
+ * read through two pinned staging buffers that alternate, so that reading the next
+ * chunk overlaps the transfer and narrowing of the last. */
+int gpart_csr_from_int64_files(gpart_ctx* ctx, int width, const char* indptr_path, const char* indices_path,
+                               int64_t n, int64_t m, gpart_device_csr* out, int32_t* status);
+
 /* ---------------- partition verification and quality ---------------- */
 
 /* Counters of a labelling for the verifier (tools/verify_partition.py:33-54) and

@@ -63,6 +63,16 @@ template <typename Types>
 WeightedGraph<Types> graph_from_int64(std::int64_t vertices, const std::int64_t* indptr,
                                       std::int64_t entries, const std::int64_t* indices);
 
+// The same inputs and checks (and messages), narrowed on the device: only the int64
+// arrays cross PCIe, the weights are filled on the device, and no host copy is
+// kept. Shape validation (monotone offsets, neighbours in range) is included;
+// symmetry is not (validate_weighted_csr needs a host copy).
+template <typename Types>
+DeviceWeightedGraph<Types> device_graph_from_int64(std::int64_t vertices, const std::int64_t* indptr,
+                                                   std::int64_t entries, const std::int64_t* indices);
+template <typename Types>
+DeviceWeightedGraph<Types> load_device_graph(const std::string& indptr_path, const std::string& indices_path);
+
 template <typename Types>
 void validate_weighted_shape(const WeightedGraph<Types>& graph, bool allow_self_loops);
 
