@@ -27,6 +27,12 @@ struct gpart_ctx {
     std::int64_t released_blocks = 0;
     // Small pinned staging buffer for host-out scalars.
     void* pinned = nullptr;
+    // Fine-grained host memory that a kernel writes a refinement round's status
+    // into: word 0 is a sequence number the host waits for by polling, the rest the
+    // status. Saves the copy engine and the blocking wait of a read-back per round.
+    std::uint64_t* status_host = nullptr;
+    std::uint64_t* status_device = nullptr;  // the same memory, device address
+    std::uint64_t status_seq = 0;
     // Profiling per family.
     struct Profile {
         bool enabled = false;
@@ -120,6 +126,10 @@ private:
 
 // Host-out readback through the context's pinned staging buffer (synchronises).
 void read_back(gpart_ctx* ctx, void* host, const void* device, std::size_t bytes);
+// Waits until status word 0 equals seq: polls for a short while, then synchronises
+// the stream (which also reports a failed kernel) and checks again.
+void wait_status(gpart_ctx* ctx, std::uint64_t seq);
+constexpr int kStatusWords = 512;  // 4 KiB of status memory per context
 
 template <typename T>
 T read_scalar(gpart_ctx* ctx, const T* device) {

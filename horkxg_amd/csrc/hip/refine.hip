@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
+#include <cstring>
 
 namespace gpart_hip {
 namespace {
@@ -550,15 +552,32 @@ __global__ void __launch_bounds__(kBlock) commit_kernel(const RoundState* st, co
     }
 }
 
-__global__ void round_status_kernel(RoundState* st, const unsigned long long* part_weights, int k) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    unsigned long long maximum = 0;
-    for (int p = 0; p < k; ++p) {
-        st->pw[p] = part_weights[p];
-        maximum = part_weights[p] > maximum ? part_weights[p] : maximum;
+// The round's status, also copied to the host-visible status words (word 0: seq,
+// stored last, after a system-scope fence).
+// One wave: lane p < k holds part p's weight; every lane writes one status word.
+__global__ void __launch_bounds__(kWave) round_status_kernel(RoundState* st, const unsigned long long* part_weights,
+                                                             int k, unsigned long long* status,
+                                                             unsigned long long seq) {
+    constexpr int kHead = static_cast<int>(offsetof(RoundState, pw) / 8);  // words before pw
+    const int lane = threadIdx.x;
+    const unsigned long long w = lane < k ? part_weights[lane] : 0;
+    unsigned long long maximum = w;
+    for (int m = kWave / 2; m > 0; m /= 2) {
+        const unsigned long long o = shfl_xor_u64(maximum, m, kWave);
+        maximum = o > maximum ? o : maximum;
     }
-    st->max_weight = maximum;
+    if (lane < 32) st->pw[lane] = w;
+    if (lane == 0) st->max_weight = maximum;
+    __syncthreads();
+    // words of the head (with max_weight) and pw[0, k): a store per lane, in parallel
+    const auto* words = reinterpret_cast<const unsigned long long*>(st);
+    if (lane < kHead + k) status[1 + lane] = words[lane];
+    __threadfence_system();
+    __syncthreads();
+    if (lane == 0) __hip_atomic_store(status, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+static_assert(sizeof(RoundState) % 8 == 0 && sizeof(RoundState) / 8 + 1 <= kStatusWords, "status words");
+static_assert(offsetof(RoundState, pw) / 8 + 32 <= static_cast<std::size_t>(kWave), "one wave writes the status");
 
 // Movers are split by degree so the cache update can give heavy rows a workgroup.
 struct MoverPair {
@@ -638,7 +657,8 @@ __device__ __forceinline__ int gain_bucket(double gain, const PowerTable& t) {
 // would otherwise take every workgroup's atomics on the same addresses. The
 // bucket scan sums the copies.
 constexpr int kReplicas = 16;
-constexpr int kTableSlots = 2048;
+constexpr int kTableBits = 9;
+constexpr int kTableSlots = 1 << kTableBits;
 constexpr unsigned kTableEmpty = 0xffffffffU;
 struct BucketTable {
     unsigned key[kTableSlots];
@@ -654,7 +674,7 @@ __device__ __forceinline__ void table_clear(BucketTable& t) {
 
 __device__ __forceinline__ void table_add(BucketTable& t, unsigned key, unsigned long long w,
                                           unsigned long long* global) {
-    const unsigned h = (key * 2654435761U) >> 21;  // 11 bits
+    const unsigned h = (key * 2654435761U) >> (32 - kTableBits);  // multiplicative hash, top bits
 #pragma unroll 1
     for (int probe = 0; probe < 64; ++probe) {
         const unsigned slot = (h + probe) & (kTableSlots - 1);
@@ -1330,10 +1350,13 @@ struct Ops {
     static void finish(gpart_ctx* ctx, const gpart_csr_view* g, RoundState* st, const unsigned long long* part_weights,
                        int k,
                        gpart_refine_status* out) {
-        hipLaunchKernelGGL(round_status_kernel, dim3(1), dim3(kWave), 0, ctx->stream, st, part_weights, k);
+        const std::uint64_t seq = ++ctx->status_seq;
+        hipLaunchKernelGGL(round_status_kernel, dim3(1), dim3(kWave), 0, ctx->stream, st, part_weights, k,
+                           reinterpret_cast<unsigned long long*>(ctx->status_device), seq);
         GPART_LAUNCH_CHECK();
-        RoundState h;
-        read_back(ctx, &h, st, sizeof(RoundState));
+        wait_status(ctx, seq);
+        RoundState h{};
+        std::memcpy(&h, ctx->status_host + 1, offsetof(RoundState, pw) + 8 * static_cast<std::size_t>(k));
         out->movers = h.light + h.heavy;
         out->cut_delta = h.delta;
         out->candidates = h.candidates;
@@ -1390,13 +1413,14 @@ struct Ops {
         const auto* vw = static_cast<const W*>(g->vertex_weights);
         static const PowerTable powers = make_power_table();
         const std::int64_t width = static_cast<std::int64_t>(k) * kBalanceWidth;
-        Scratch<RoundState> st(ctx, 1);
-        Scratch<unsigned long long> replicas(ctx, 2 * kReplicas * width), counts(ctx, 2 * width),
-            offsets(ctx, 2 * width), thresholds(ctx, 64);
-        fill_bytes(ctx, st.get(), 0, sizeof(RoundState));
-        fill_bytes(ctx, replicas.get(), 0, static_cast<std::size_t>(2 * kReplicas * width) * 8);
-        unsigned long long* src_replicas = replicas.get();
-        unsigned long long* dst_replicas = replicas.get() + kReplicas * width;
+        // the round state and both replica arrays in one block, cleared by one fill
+        constexpr std::int64_t kStateWords = sizeof(RoundState) / 8;
+        Scratch<unsigned long long> cleared(ctx, kStateWords + 2 * kReplicas * width);
+        Scratch<unsigned long long> counts(ctx, 2 * width), offsets(ctx, 2 * width), thresholds(ctx, 64);
+        fill_bytes(ctx, cleared.get(), 0, static_cast<std::size_t>(cleared.size()) * 8);
+        auto* st_ptr = reinterpret_cast<RoundState*>(cleared.get());
+        unsigned long long* src_replicas = cleared.get() + kStateWords;
+        unsigned long long* dst_replicas = src_replicas + kReplicas * width;
         unsigned long long* src_counts = counts.get();
         unsigned long long* dst_counts = counts.get() + width;
         unsigned long long* src_offsets = offsets.get();
@@ -1409,13 +1433,13 @@ struct Ops {
             prefix(ctx, static_cast<std::int64_t>(slices) * 32);
 
         device_scan<long long, SumOp<long long>>(ctx, OverloadedLoad{labels, part_weights, capacity},
-                                                 CandidateStore{cand.get(), flags, n, st.get()}, n,
+                                                 CandidateStore{cand.get(), flags, n, st_ptr}, n,
                                                  SumOp<long long>{});
         const int G = group_for_k(k);
-        const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), 2 * kRoundBlocks));
+        const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), 4 * kRoundBlocks));
         profile_begin(ctx, GPART_PROFILE_REBALANCE);
 #define GPART_BALANCE_PROPOSALS(GG)                                                                              \
-    hipLaunchKernelGGL((balance_proposals_kernel<W, GG, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, st.get(),   \
+    hipLaunchKernelGGL((balance_proposals_kernel<W, GG, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, st_ptr,   \
                        cand.get(), cache, labels, vw, k, part_weights, capacity, powers, targets, cbucket.get(), \
                        src_replicas)
         switch (G) {
@@ -1428,28 +1452,28 @@ struct Ops {
         profile_end(ctx, GPART_PROFILE_REBALANCE, 0.0);
         hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, src_replicas, k, src_counts,
                            src_offsets, part_weights, capacity, false, src_thresholds);
-        hipLaunchKernelGGL(balance_sources_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st.get(),
+        hipLaunchKernelGGL(balance_sources_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st_ptr,
                            cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_counts, src_offsets,
                            src_thresholds, sums.get(), dst_replicas, width);
         hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, sums.get(), slices,
                            prefix.get());
-        hipLaunchKernelGGL(balance_source_ranks_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st.get(),
+        hipLaunchKernelGGL(balance_source_ranks_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st_ptr,
                            cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_offsets, src_thresholds,
                            prefix.get(), dst_replicas, width);
         hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, dst_replicas, k, dst_counts,
                            dst_offsets, part_weights, capacity, true, dst_thresholds);
         hipLaunchKernelGGL((balance_destinations_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
-                           st.get(), cand.get(), dbucket.get(), vw, off, dst_counts, dst_offsets, dst_thresholds,
+                           st_ptr, cand.get(), dbucket.get(), vw, off, dst_counts, dst_offsets, dst_thresholds,
                            sums.get(), light.get(), heavy.get(), flags);
         hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBlock), 0, ctx->stream, sums.get(), slices,
                            prefix.get());
         hipLaunchKernelGGL((balance_destination_ranks_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
-                           st.get(), cand.get(), dbucket.get(), vw, off, dst_offsets, dst_thresholds, prefix.get(),
+                           st_ptr, cand.get(), dbucket.get(), vw, off, dst_offsets, dst_thresholds, prefix.get(),
                            light.get(), heavy.get(), flags);
         GPART_LAUNCH_CHECK();
-        apply(ctx, g, c, st.get(), light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, nullptr,
+        apply(ctx, g, c, st_ptr, light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, nullptr,
               false);
-        finish(ctx, g, st.get(), part_weights, k, out);
+        finish(ctx, g, st_ptr, part_weights, k, out);
         // per candidate: id, label, weight, cache row, target and bucket writes
         profile_add_bytes(ctx, GPART_PROFILE_REBALANCE,
                           static_cast<double>(out->candidates) * (16.0 + sizeof(W) + sizeof(C) * k));

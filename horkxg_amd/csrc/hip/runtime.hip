@@ -107,6 +107,20 @@ void ctx_free(gpart_ctx* ctx, void* p) {
     }
 }
 
+void wait_status(gpart_ctx* ctx, std::uint64_t seq) {
+    using clock = std::chrono::steady_clock;
+    const auto start = clock::now();
+    for (unsigned spin = 0;; ++spin) {
+        if (__atomic_load_n(ctx->status_host, __ATOMIC_ACQUIRE) == seq) return;
+        // poll for up to 200 us: a round's kernels usually finish within that
+        if ((spin & 255) == 255 && clock::now() - start > std::chrono::microseconds(200)) break;
+        __builtin_ia32_pause();
+    }
+    GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (__atomic_load_n(ctx->status_host, __ATOMIC_ACQUIRE) != seq)
+        throw Error(GPART_ERR_RUNTIME, "round status did not reach the host");
+}
+
 void read_back(gpart_ctx* ctx, void* host, const void* device, std::size_t bytes) {
     if (bytes <= 4096) {
         GPART_HIP_CHECK(hipMemcpyAsync(ctx->pinned, device, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -210,6 +224,10 @@ int gpart_ctx_create(int device, gpart_ctx** out) {
             // Default priority: no stream priorities anywhere in this library.
             GPART_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
             GPART_HIP_CHECK(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+            GPART_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ctx->status_host), kStatusWords * 8,
+                                          hipHostMallocCoherent | hipHostMallocMapped));
+            GPART_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->status_device), ctx->status_host, 0));
+            std::memset(ctx->status_host, 0, kStatusWords * 8);
             SegmentSource source;
             source.allocate = [ctx](std::size_t bytes) -> void* {
                 void* p = nullptr;
@@ -226,6 +244,7 @@ int gpart_ctx_create(int device, gpart_ctx** out) {
             ctx->cache = std::make_unique<BlockCache>(std::move(source));
         } catch (...) {
             if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+            if (ctx->status_host) (void)hipHostFree(ctx->status_host);
             if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
             delete ctx;
             throw;
@@ -249,6 +268,7 @@ int gpart_ctx_destroy(gpart_ctx* ctx) {
             }
         for (auto e : ctx->spare_events) GPART_HIP_CHECK(hipEventDestroy(e));
         if (ctx->pinned) GPART_HIP_CHECK(hipHostFree(ctx->pinned));
+        if (ctx->status_host) GPART_HIP_CHECK(hipHostFree(ctx->status_host));
         GPART_HIP_CHECK(hipStreamDestroy(ctx->stream));
         delete ctx;
     });
