@@ -37,6 +37,8 @@ FAMILY_KERNELS = {
     "refine_filter": "priority_filter_kernel",
     "refine_cache": "build_cache_kernel",
     "spectral": "spectral_depth_kernel",
+    "sclp_affinity": "sclp_(low|medium|high|part|hub_insert|hub_scan|hub_decode)_kernel",
+    "contract": "(cross_count|encode|row_offsets)_kernel",
 }
 
 # spectral: the bisection form each configuration is quoted with (FP32 diffusion
@@ -74,8 +76,8 @@ def cpu_baseline(indptr, indices, k, ratio, seed, threads):
 
 
 def copy_calibration(mib: int) -> int:
-    """Five device-to-device copies of `mib` MiB on the library stream: a plain copy
-    kernel of known size, whose FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
+    """Five device-to-device copies of `mib` MiB on the library stream (the library's
+    copy_vec_kernel): a plain copy kernel of known size, whose FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
     counters (run under rocprofv3 --pmc)."""
     import ctypes as C
 
@@ -114,20 +116,32 @@ def pmc_per_dispatch(csv_path: Path, kernel_regex: str, counter: str):
     return sum(values) / len(values), len(values)
 
 
+def pmc_total(csv_path: Path, kernel_regex: str, counter: str) -> float:
+    """Sum of `counter` over every dispatch of kernels matching kernel_regex."""
+    mean, count = pmc_per_dispatch(csv_path, kernel_regex, counter)
+    return mean * count
+
+
 def traffic_summary(args) -> int:
     """Writes profiles/traffic_<workload>.json from two PMC runs of the bench
     (FETCH_SIZE, WRITE_SIZE) and two of the copy calibration."""
     copy_bytes = args.copy_mib << 20
-    cf, _ = pmc_per_dispatch(Path(args.copy_fetch), "copyBuffer", "FETCH_SIZE")
-    cw, _ = pmc_per_dispatch(Path(args.copy_write), "copyBuffer", "WRITE_SIZE")
+    cf, _ = pmc_per_dispatch(Path(args.copy_fetch), "copy_vec_kernel", "FETCH_SIZE")
+    cw, _ = pmc_per_dispatch(Path(args.copy_write), "copy_vec_kernel", "WRITE_SIZE")
     f_scale = copy_bytes / (cf * 1024.0)
     w_scale = copy_bytes / (cw * 1024.0)
     kf, nf = pmc_per_dispatch(Path(args.fetch), args.kernel, "FETCH_SIZE")
     kw, nw = pmc_per_dispatch(Path(args.write), args.kernel, "WRITE_SIZE")
+    if args.family_launches:
+        # a family launch runs several kernels: its traffic is the total over the
+        # run's dispatches divided by the family's launches in the run
+        kf = kf * nf / args.family_launches
+        kw = kw * nw / args.family_launches
     out = {
         "workload": args.workload,
         "kernel": args.kernel,
         "dispatches": {"fetch": nf, "write": nw},
+        "family_launches": args.family_launches or None,
         "fetch_kib_per_dispatch": round(kf, 1),
         "write_kib_per_dispatch": round(kw, 1),
         "copy_calibration": {"bytes": copy_bytes, "fetch_kib": round(cf, 1), "write_kib": round(cw, 1),
@@ -147,7 +161,8 @@ def recorded_traffic(workload: str, family: str):
     if not path.exists() or family not in FAMILY_KERNELS:
         return None
     rec = json.loads(path.read_text())
-    if not re.search(rec.get("kernel", "^$"), FAMILY_KERNELS[family]):
+    kernel = rec.get("kernel", "^$")
+    if kernel != FAMILY_KERNELS[family] and not re.search(kernel, FAMILY_KERNELS[family]):
         return None
     return rec["traffic_bytes_per_dispatch"]
 
@@ -429,10 +444,14 @@ def main() -> int:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
     p.add_argument("--no-profile", action="store_true", help="skip the live per-family event timing")
+    p.add_argument("--no-precision", action="store_true",
+                   help="skip the FP32-against-FP64 comparison (a second coarsening and partition)")
     p.add_argument("--copy-calibration", type=int, metavar="MIB", help="only run the copy calibration")
     p.add_argument("--traffic-summary", action="store_true",
                    help="only summarise rocprofv3 --pmc CSVs into profiles/traffic_<workload>.json")
     p.add_argument("--kernel", default="lp_proposals_kernel", help="kernel regex for --traffic-summary")
+    p.add_argument("--family-launches", type=int, default=0,
+                   help="--traffic-summary: divide the run's total by this many family launches")
     p.add_argument("--fetch", help="counter_collection.csv of the FETCH_SIZE run")
     p.add_argument("--write", help="counter_collection.csv of the WRITE_SIZE run")
     p.add_argument("--copy-fetch", help="counter_collection.csv of the calibration FETCH_SIZE run")
@@ -557,7 +576,7 @@ def main() -> int:
     precision = None
     # (skipped for graphs generated on the GPU: their host hierarchy copy is the
     # size of the input; the R-MAT 22 line carries the comparison)
-    if args.spectral_mode == "batched_fp32" and indptr is not None:
+    if args.spectral_mode == "batched_fp32" and indptr is not None and not args.no_precision:
         # FP32 against FP64 diffusion on the coarsest level of the production
         # hierarchy, and the final cut of one FP64 partition (both untimed)
         csf = max(64, (512 + k - 1) // k)
