@@ -222,6 +222,35 @@ __global__ void __launch_bounds__(kBlock) build_cache_hub_kernel(const std::int6
 // for k <= 8, 16, 32. A wave covers 64/G rows per pass.
 constexpr int kRowSpan = 8;
 
+// The lane's kRowSpan entries of a cache row, entries at or past k as 0: 16-byte
+// loads when the span lies inside the row and is 16-byte aligned (k a multiple of
+// 4 or 2 entries), else one load per entry.
+template <typename C>
+__device__ __forceinline__ void load_span(const C* row, int j, int k, unsigned long long (&out)[kRowSpan]) {
+    static_assert(sizeof(C) == 4 || sizeof(C) == 8, "4- or 8-byte cache entries");
+    const int first = j * kRowSpan;
+    const C* span = row + first;
+    if (first + kRowSpan <= k && (reinterpret_cast<std::uintptr_t>(span) & 15) == 0) {
+        constexpr int kPer = 16 / static_cast<int>(sizeof(C));
+#pragma unroll
+        for (int q = 0; q < kRowSpan / kPer; ++q) {
+            const uint4 x = reinterpret_cast<const uint4*>(span)[q];
+            if constexpr (sizeof(C) == 4) {
+                out[4 * q] = x.x;
+                out[4 * q + 1] = x.y;
+                out[4 * q + 2] = x.z;
+                out[4 * q + 3] = x.w;
+            } else {
+                out[2 * q] = static_cast<unsigned long long>(x.x) | static_cast<unsigned long long>(x.y) << 32;
+                out[2 * q + 1] = static_cast<unsigned long long>(x.z) | static_cast<unsigned long long>(x.w) << 32;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < kRowSpan; ++e) out[e] = first + e < k ? static_cast<unsigned long long>(span[e]) : 0;
+    }
+}
+
 // incremental_lp_proposals (:27-46): argmax over p != source of cache > 0,
 // lowest index on ties; accepted if best >= current or within the filter.
 template <int G, typename C>
@@ -236,10 +265,12 @@ __global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, co
     const C* row = cache + v * k;
     unsigned long long best = 0, current = 0;
     int best_p = 0x7fffffff;
+    unsigned long long span[kRowSpan];
+    load_span(row, j, k, span);
 #pragma unroll
     for (int e = 0; e < kRowSpan; ++e) {
         const int p = j * kRowSpan + e;
-        const unsigned long long value = p < k ? row[p] : 0;
+        const unsigned long long value = span[e];
         if (p == source) current = value;
         else if (value > best) {  // strict: the lowest index keeps a tie
             best = value;
@@ -832,11 +863,13 @@ __global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundSt
         unsigned long long slack[kRowSpan];
         unsigned long long conn = 0, best_slack = 0, current = 0;
         int bp = 0x7fffffff;
+        unsigned long long span[kRowSpan];
+        load_span(row, j, k, span);
         // lane-local best by (conn, slack), lowest index on ties (ascending p)
 #pragma unroll
         for (int e = 0; e < kRowSpan; ++e) {
             const int p = j * kRowSpan + e;
-            const unsigned long long value = p < k ? row[p] : 0;
+            const unsigned long long value = span[e];
             if (p == source) current = value;
             const bool ok = p < k && p != source && pw[p] < capacity && weight <= capacity - pw[p];
             slack[e] = ok ? capacity - pw[p] : 0;
