@@ -54,6 +54,23 @@ __global__ void fill_u8_kernel(std::uint8_t* dst, std::uint8_t value, std::int64
     if (i < bytes) dst[i] = value;
 }
 
+// Copies bytes into status words 1.. (host-visible), then stores seq in word 0.
+__global__ void __launch_bounds__(kWave) status_copy_kernel(const std::uint8_t* src, std::int64_t bytes,
+                                                            unsigned long long* status, unsigned long long seq) {
+    auto* dst = reinterpret_cast<std::uint8_t*>(status + 1);
+    if ((reinterpret_cast<std::uintptr_t>(src) & 7) == 0) {
+        const auto* s8 = reinterpret_cast<const unsigned long long*>(src);
+        const std::int64_t words = bytes / 8;
+        for (std::int64_t i = threadIdx.x; i < words; i += kWave) status[1 + i] = s8[i];
+        for (std::int64_t i = words * 8 + threadIdx.x; i < bytes; i += kWave) dst[i] = src[i];
+    } else {
+        for (std::int64_t i = threadIdx.x; i < bytes; i += kWave) dst[i] = src[i];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(status, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void sequence_kernel(std::int32_t* out, std::int64_t n) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = static_cast<std::int32_t>(i);
@@ -121,11 +138,19 @@ void wait_status(gpart_ctx* ctx, std::uint64_t seq) {
         throw Error(GPART_ERR_RUNTIME, "round status did not reach the host");
 }
 
+// Small read-backs: a one-wave kernel copies the bytes into the fine-grained
+// status words and stores the sequence number after them; the host polls for it
+// (no copy engine, no blocking wait).
 void read_back(gpart_ctx* ctx, void* host, const void* device, std::size_t bytes) {
-    if (bytes <= 4096) {
-        GPART_HIP_CHECK(hipMemcpyAsync(ctx->pinned, device, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        std::memcpy(host, ctx->pinned, bytes);
+    if (bytes == 0) return;
+    if (bytes <= static_cast<std::size_t>(kStatusWords - 1) * 8) {
+        const std::uint64_t seq = ++ctx->status_seq;
+        hipLaunchKernelGGL(status_copy_kernel, dim3(1), dim3(kWave), 0, ctx->stream,
+                           static_cast<const std::uint8_t*>(device), static_cast<std::int64_t>(bytes),
+                           reinterpret_cast<unsigned long long*>(ctx->status_device), seq);
+        GPART_LAUNCH_CHECK();
+        wait_status(ctx, seq);
+        std::memcpy(host, ctx->status_host + 1, bytes);
     } else {
         GPART_HIP_CHECK(hipMemcpyAsync(host, device, bytes, hipMemcpyDeviceToHost, ctx->stream));
         GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
