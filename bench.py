@@ -431,7 +431,7 @@ def main() -> int:
     p.add_argument("--warmup", type=int, default=None, help="default 1")
     p.add_argument("--workload", default="mesh128_k32", choices=sorted(WORKLOADS) + ["mix"])
     p.add_argument("--mix-count", type=int, default=256, help="graphs of the mix workload")
-    p.add_argument("--mix-procs", type=int, default=4,
+    p.add_argument("--mix-procs", type=int, default=2,
                    help="host processes sharing the GPU in the batched mix (1: the in-process batch)")
     p.add_argument("--mix-hw-queues", type=int, default=1, choices=range(0, 5),
                    help="GPU_MAX_HW_QUEUES of each mix process (0: the runtime's default)")
