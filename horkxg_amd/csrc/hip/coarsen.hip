@@ -562,9 +562,9 @@ __global__ void __launch_bounds__(kPartThreads) sclp_part_kernel(const std::int6
         const std::int64_t begin = offsets[v], end = offsets[v + 1];
         const std::int32_t source = clusters[v];
         int slots = kPartSlots;
-        if (parts == 1) {  // degree <= kPartKeys: 2 * degree slots hold every key
+        if (parts == 1) {  // 2 * degree slots (at most kPartSlots) for a single pass
             slots = 1;
-            while (slots < 2 * (end - begin)) slots <<= 1;
+            while (slots < 2 * (end - begin) && slots < kPartSlots) slots <<= 1;
         }
         const int slot_mask = slots - 1;
         unsigned long long current = 0, best = 0, best_tie = 0;
