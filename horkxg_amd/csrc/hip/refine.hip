@@ -956,16 +956,16 @@ __global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundSt
 // its overload (sources) or slack (destinations).
 // One workgroup per part, a thread per bucket: the 16 replica loads of a thread
 // are independent and all in flight at once.
-constexpr int kScanThreads = 1024;
-static_assert(kScanThreads >= kBalanceWidth, "a thread per bucket");
-static_assert(kRoundBlocks % kScanThreads == 0, "column scan items per thread");
-__global__ void __launch_bounds__(kScanThreads) balance_scan_kernel(const unsigned long long* replicas, int k,
+constexpr int kBucketScanThreads = 1024;
+static_assert(kBucketScanThreads >= kBalanceWidth, "a thread per bucket");
+static_assert(kRoundBlocks % kBucketScanThreads == 0, "column scan items per thread");
+__global__ void __launch_bounds__(kBucketScanThreads) balance_scan_kernel(const unsigned long long* replicas, int k,
                                                                     unsigned long long* counts,
                                                                     unsigned long long* offsets,
                                                                     const unsigned long long* part_weights,
                                                                     std::uint64_t capacity, bool destinations,
                                                                     unsigned long long* thresholds) {
-    __shared__ unsigned long long waves[kScanThreads / kWave + 1];
+    __shared__ unsigned long long waves[kBucketScanThreads / kWave + 1];
     constexpr int kPer = 1;
     const long long width = static_cast<long long>(k) * kBalanceWidth;
     unsigned long long items[kPer];
@@ -1069,10 +1069,10 @@ __global__ void __launch_bounds__(kBlock) balance_sources_kernel(const RoundStat
 }
 
 // Exclusive scan of each part's column of per-workgroup sums (blocks <= 1024).
-__global__ void __launch_bounds__(kScanThreads) column_scan_kernel(const unsigned long long* sums, int blocks,
+__global__ void __launch_bounds__(kBucketScanThreads) column_scan_kernel(const unsigned long long* sums, int blocks,
                                                                    unsigned long long* prefix) {
-    __shared__ unsigned long long waves[kScanThreads / kWave + 1];
-    constexpr int kPer = kRoundBlocks / kScanThreads;
+    __shared__ unsigned long long waves[kBucketScanThreads / kWave + 1];
+    constexpr int kPer = kRoundBlocks / kBucketScanThreads;
     const int column = blockIdx.x;
     unsigned long long items[kPer];
     unsigned long long local = 0;
@@ -1489,22 +1489,22 @@ struct Ops {
 #undef GPART_BALANCE_PROPOSALS
         GPART_LAUNCH_CHECK();
         profile_end(ctx, GPART_PROFILE_REBALANCE, 0.0);
-        hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kScanThreads), 0, ctx->stream, src_replicas, k,
+        hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, src_replicas, k,
                            src_counts, src_offsets, part_weights, capacity, false, src_thresholds);
         hipLaunchKernelGGL(balance_sources_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st_ptr,
                            cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_counts, src_offsets,
                            src_thresholds, sums.get(), dst_replicas, width);
-        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kScanThreads), 0, ctx->stream, sums.get(), slices,
+        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, sums.get(), slices,
                            prefix.get());
         hipLaunchKernelGGL(balance_source_ranks_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st_ptr,
                            cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_offsets, src_thresholds,
                            prefix.get(), dst_replicas, width);
-        hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kScanThreads), 0, ctx->stream, dst_replicas, k,
+        hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, dst_replicas, k,
                            dst_counts, dst_offsets, part_weights, capacity, true, dst_thresholds);
         hipLaunchKernelGGL((balance_destinations_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
                            st_ptr, cand.get(), dbucket.get(), vw, off, dst_counts, dst_offsets, dst_thresholds,
                            sums.get(), light.get(), heavy.get(), flags);
-        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kScanThreads), 0, ctx->stream, sums.get(), slices,
+        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, sums.get(), slices,
                            prefix.get());
         hipLaunchKernelGGL((balance_destination_ranks_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
                            st_ptr, cand.get(), dbucket.get(), vw, off, dst_offsets, dst_thresholds, prefix.get(),
