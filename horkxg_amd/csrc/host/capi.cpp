@@ -118,7 +118,13 @@ gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, dou
     co.strict_verify = verify != 0;
     co.cluster_size_factor = std::max(64, (512 + parts - 1) / parts);
     co.pack_isolated = true;
-    gpart::DeviceWeightedGraph<Types> input = g.device;  // device copy, outside the timed scope
+    // The hierarchy's level 0 is a read-only view of the caller's device graph: no
+    // copy, and the input's memory is not held twice.
+    gpart::DeviceWeightedGraph<Types> input;
+    input.offsets = decltype(input.offsets)::borrow(g.device.offsets);
+    input.neighbors = decltype(input.neighbors)::borrow(g.device.neighbors);
+    input.edge_weights = decltype(input.edge_weights)::borrow(g.device.edge_weights);
+    input.vertex_weights = decltype(input.vertex_weights)::borrow(g.device.vertex_weights);
     GPART_CHECK(gpart_ctx_synchronize(ctx));
     const auto t0 = clock::now();
     auto hierarchy = gpart::coarsen<Types>(std::move(input), co);

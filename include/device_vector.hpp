@@ -12,6 +12,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <stdexcept>
 #include <utility>
 #include <vector>
 
@@ -40,10 +41,11 @@ public:
     DeviceVector(const std::vector<T>& host) { assign(host); }
     DeviceVector(const DeviceVector& other) { copy_from(other); }
     DeviceVector(DeviceVector&& other) noexcept
-        : ctx_(other.ctx_), data_(other.data_), size_(other.size_) {
+        : ctx_(other.ctx_), data_(other.data_), size_(other.size_), borrowed_(other.borrowed_) {
         other.ctx_ = nullptr;
         other.data_ = nullptr;
         other.size_ = 0;
+        other.borrowed_ = false;
     }
     DeviceVector& operator=(const DeviceVector& other) {
         if (this != &other) copy_from(other);
@@ -55,9 +57,11 @@ public:
             ctx_ = other.ctx_;
             data_ = other.data_;
             size_ = other.size_;
+            borrowed_ = other.borrowed_;
             other.ctx_ = nullptr;
             other.data_ = nullptr;
             other.size_ = 0;
+            other.borrowed_ = false;
         }
         return *this;
     }
@@ -76,6 +80,17 @@ public:
         return v;
     }
 
+    // A read-only view of another vector's memory, never freed by this object; the
+    // owner must outlive it. Copies of a view own their memory.
+    static DeviceVector borrow(const DeviceVector& owner) {
+        DeviceVector v;
+        v.ctx_ = owner.ctx_;
+        v.data_ = const_cast<T*>(owner.data_);
+        v.size_ = owner.size_;
+        v.borrowed_ = true;
+        return v;
+    }
+
     std::size_t size() const { return size_; }
     bool empty() const { return size_ == 0; }
     T* data() { return data_; }
@@ -83,7 +98,7 @@ public:
 
     // Resize without preserving contents (device arrays here are rebuilt, not grown).
     void resize_discard(std::size_t n) {
-        if (n == size_) return;
+        if (n == size_ && !borrowed_) return;  // a view is replaced, never written
         release();
         allocate(n);
     }
@@ -109,6 +124,7 @@ public:
         std::swap(ctx_, other.ctx_);
         std::swap(data_, other.data_);
         std::swap(size_, other.size_);
+        std::swap(borrowed_, other.borrowed_);
     }
 
 private:
@@ -121,9 +137,10 @@ private:
         data_ = static_cast<T*>(p);
     }
     void release() {
-        if (data_) gpart_device_free(ctx_, data_);
+        if (data_ && !borrowed_) gpart_device_free(ctx_, data_);
         data_ = nullptr;
         size_ = 0;
+        borrowed_ = false;
     }
     void copy_from(const DeviceVector& other) {
         resize_discard(other.size_);
@@ -133,12 +150,14 @@ private:
 
     gpart_ctx* ctx_ = nullptr;
     T* data_ = nullptr;
+    bool borrowed_ = false;
     std::size_t size_ = 0;
 };
 
 template <typename T>
 void DeviceVector<T>::fill(const T& value) {
     if (!size_) return;
+    if (borrowed_) throw std::logic_error("fill of a borrowed device view");
     bool uniform_bytes = true;
     const auto* bytes = reinterpret_cast<const unsigned char*>(&value);
     for (std::size_t i = 1; i < sizeof(T); ++i) uniform_bytes = uniform_bytes && bytes[i] == bytes[0];
