@@ -462,7 +462,7 @@ def main() -> int:
         return traffic_summary(args)
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the partitioner is a single-GPU path; see DESIGN.md for how it would shard)",
+              "(the partitioner is a single-GPU path)",
               file=sys.stderr)
         return 2
 
@@ -518,7 +518,7 @@ def main() -> int:
             hx.profile_enable(f, True)
     reports = []
     # The context keeps its device segments across runs (freshly recycled device
-    # memory is scrubbed by the driver on allocation); the first, cold step is
+    # memory is scrubbed by the GPU kernel driver on allocation); the first, cold step is
     # reported beside the steady-state value.
     alloc_before = hx.cache_info()
     t0 = time.perf_counter()

@@ -1,11 +1,11 @@
 // block_cache.hpp -- best-fit caching of device memory in segments held by the
 // context.
 //
-// Device memory that the driver hands out again after a free is scrubbed on the
+// Device memory that the GPU kernel driver hands out again after a free is scrubbed on the
 // way (measured on MI355X under ROCm 7.2: once roughly 64 GiB have been recycled,
 // every further GiB costs about 30 ms of host time inside the allocation call).
 // A multilevel hierarchy of a billion-entry graph needs ~200 GB, allocated and
-// freed level by level, so handing memory back to the driver between levels or
+// freed level by level, so handing memory back to the kernel driver between levels or
 // between runs costs seconds. The cache therefore keeps every segment it takes
 // until release_free_segments() (an explicit release, context teardown, or an
 // allocation the device cannot serve otherwise).
