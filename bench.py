@@ -37,7 +37,7 @@ FAMILY_KERNELS = {
     "refine_filter": "priority_filter_kernel",
     "refine_cache": "build_cache_kernel",
     "spectral": "spectral_depth_kernel",
-    "sclp_affinity": "sclp_(low|medium|high|part|hub_insert|hub_scan|hub_decode)_kernel",
+    "sclp_affinity": "sclp_(low|medium|high|part|split_count|split_scatter|hub_insert|hub_scan|hub_decode)_kernel",
     "contract": "(cross_count|encode|row_offsets)_kernel",
 }
 

@@ -522,6 +522,7 @@ constexpr std::int64_t kPartKeys = 4096;
 constexpr std::int64_t kPartMaxDegree = 32768;
 constexpr int kPartCap = kPartSlots - kPartThreads - 1;  // keys held + one in flight per thread < slots
 constexpr int kPartLaunchCap = 1024;
+constexpr int kPartUnroll = 4;
 
 template <typename O>
 struct PartLoad {
@@ -539,6 +540,185 @@ struct PartLoad {
     }
 };
 
+// Rows of more than one item (kPartKeys < degree <= kPartMaxDegree) are grouped
+// once per round, so that item p reads only its own group of the row instead of
+// the whole row with the cluster gather of every entry. The row's region of
+// entries starts at the exclusive prefix of SplitLoad over the high list.
+//
+// The split itself runs as P chunk items per row (split item j = chunk p of row
+// i, entries [begin + p * len, ...) with len = ceil(degree / P)), so no workgroup
+// walks a whole row: sclp_split_count_kernel stores each entry's cluster in row
+// order (-1 for a self loop) and counts, per chunk, the entries of each group
+// q = mix32(cluster) & (P - 1); sclp_split_scatter_kernel places chunk p's
+// entries of group q after those of the groups before q and of the chunks before
+// p. Ranks within a chunk come from a ballot per wave and one LDS atomic per wave
+// and group; the LDS tables that read a group sum exactly, in any order.
+constexpr int kMaxParts = static_cast<int>(kPartMaxDegree / kPartKeys);
+
+template <typename O>
+struct SplitLoad {
+    const std::int32_t* high;
+    const O* offsets;
+    std::int64_t count;
+    __device__ __forceinline__ std::int64_t operator()(std::int64_t i) const {
+        if (i >= count) return 0;
+        const std::int32_t v = high[i];
+        const std::int64_t d = static_cast<std::int64_t>(offsets[v + 1] - offsets[v]);
+        return d > kPartKeys && d <= kPartMaxDegree ? d : 0;
+    }
+};
+
+// Chunk items of a split row: its item count; 0 for rows of one item.
+template <typename O>
+struct SplitItemLoad {
+    PartLoad<O> parts;
+    __device__ __forceinline__ std::int64_t operator()(std::int64_t i) const {
+        const std::int64_t p = parts(i);
+        return p > 1 ? p : 0;
+    }
+};
+
+struct SplitChunk {
+    std::int32_t i, v, p, parts;
+    std::int64_t row_begin, begin, end;
+};
+
+template <typename O>
+__device__ __forceinline__ SplitChunk split_chunk(std::int64_t j, const std::int32_t* split_row,
+                                                  const std::int64_t* split_start, const std::int32_t* list,
+                                                  const O* offsets) {
+    SplitChunk c;
+    c.i = split_row[j];
+    const std::int64_t first = split_start[c.i];
+    c.parts = static_cast<std::int32_t>(split_start[c.i + 1] - first);
+    c.p = static_cast<std::int32_t>(j - first);
+    c.v = list[c.i];
+    c.row_begin = static_cast<std::int64_t>(offsets[c.v]);
+    const std::int64_t d = static_cast<std::int64_t>(offsets[c.v + 1]) - c.row_begin;
+    const std::int64_t len = (d + c.parts - 1) / c.parts;
+    c.begin = c.row_begin + (c.p * len < d ? c.p * len : d);
+    c.end = c.row_begin + ((c.p + 1) * len < d ? (c.p + 1) * len : d);
+    return c;
+}
+
+template <typename O>
+__global__ void __launch_bounds__(kPartThreads) sclp_split_count_kernel(
+    const std::int64_t* split_total, const std::int32_t* split_row, const std::int64_t* split_start,
+    const std::int32_t* list, const O* offsets, const std::int32_t* neighbors, const std::int32_t* clusters,
+    const std::int64_t* region, std::int32_t* row_clusters, std::uint32_t* chunk_counts) {
+    __shared__ unsigned int counts[kMaxParts];
+    const int lane = lane_id();
+    const std::int64_t total = *split_total;
+    for (std::int64_t j = blockIdx.x; j < total; j += gridDim.x) {
+        const SplitChunk c = split_chunk(j, split_row, split_start, list, offsets);
+        const std::int64_t base = region[c.i] - c.row_begin;
+        const auto pmask = static_cast<std::uint32_t>(c.parts - 1);
+        if (threadIdx.x < kMaxParts) counts[threadIdx.x] = 0;
+        __syncthreads();
+        for (std::int64_t e0 = c.begin; e0 < c.end; e0 += kPartThreads * kPartUnroll) {  // same trip count in every lane
+            std::int32_t t[kPartUnroll];
+#pragma unroll
+            for (int k = 0; k < kPartUnroll; ++k) {
+                const std::int64_t e = e0 + threadIdx.x + k * kPartThreads;
+                t[k] = -1;
+                if (e < c.end) {
+                    const std::int32_t u = neighbors[e];
+                    if (u != c.v) t[k] = u;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kPartUnroll; ++k) {
+                const std::int64_t e = e0 + threadIdx.x + k * kPartThreads;
+                if (t[k] >= 0) t[k] = clusters[t[k]];
+                if (e < c.end) row_clusters[base + e] = t[k];
+            }
+#pragma unroll
+            for (int k = 0; k < kPartUnroll; ++k) {
+                const int q = t[k] >= 0 ? static_cast<int>(mix32(static_cast<std::uint32_t>(t[k])) & pmask) : -1;
+                for (int g = 0; g < c.parts; ++g) {
+                    const std::uint64_t m = __ballot(q == g);
+                    if (lane == 0 && m) atomicAdd(&counts[g], static_cast<unsigned>(__popcll(m)));
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < c.parts) chunk_counts[j * kMaxParts + threadIdx.x] = counts[threadIdx.x];
+        __syncthreads();  // counts is reused by the next chunk
+    }
+}
+
+template <typename O, typename W>
+__global__ void __launch_bounds__(kPartThreads) sclp_split_scatter_kernel(
+    const std::int64_t* split_total, const std::int32_t* split_row, const std::int64_t* split_start,
+    const std::int32_t* list, const O* offsets, const W* weights, const std::int64_t* region,
+    const std::uint32_t* chunk_counts, const std::int32_t* row_clusters, const std::int64_t* item_start,
+    std::int32_t* group_keys, W* group_weights, std::int64_t* group_begin, std::int64_t* group_end) {
+    __shared__ unsigned long long cursor[kMaxParts];
+    __shared__ unsigned int counts[kMaxParts * kMaxParts];  // [chunk][group] of the row
+    __shared__ unsigned int sizes[kMaxParts];
+    const int lane = lane_id();
+    const std::int64_t total = *split_total;
+    for (std::int64_t j = blockIdx.x; j < total; j += gridDim.x) {
+        const SplitChunk c = split_chunk(j, split_row, split_start, list, offsets);
+        const std::int64_t first_chunk = j - c.p;
+        const std::int64_t base = region[c.i];
+        const auto pmask = static_cast<std::uint32_t>(c.parts - 1);
+        if (threadIdx.x < c.parts * c.parts) {  // one load per thread
+            const int p = threadIdx.x / c.parts, g = threadIdx.x % c.parts;
+            counts[p * kMaxParts + g] = chunk_counts[(first_chunk + p) * kMaxParts + g];
+        }
+        __syncthreads();
+        unsigned int before = 0;
+        if (threadIdx.x < c.parts) {
+            unsigned int size = 0;
+            for (int p = 0; p < c.parts; ++p) {
+                const unsigned int n = counts[p * kMaxParts + threadIdx.x];
+                size += n;
+                if (p < c.p) before += n;
+            }
+            sizes[threadIdx.x] = size;
+        }
+        __syncthreads();
+        if (threadIdx.x < c.parts) {
+            std::int64_t at = base;
+            for (int g = 0; g < static_cast<int>(threadIdx.x); ++g) at += sizes[g];
+            cursor[threadIdx.x] = static_cast<unsigned long long>(at + before);
+            if (c.p == 0) {
+                group_begin[item_start[c.i] + threadIdx.x] = at;
+                group_end[item_start[c.i] + threadIdx.x] = at + sizes[threadIdx.x];
+            }
+        }
+        __syncthreads();
+        for (std::int64_t e0 = c.begin; e0 < c.end; e0 += kPartThreads) {
+            const std::int64_t e = e0 + threadIdx.x;
+            int q = -1;
+            std::int32_t t = -1;
+            if (e < c.end) {
+                t = row_clusters[base + (e - c.row_begin)];
+                if (t >= 0) q = static_cast<int>(mix32(static_cast<std::uint32_t>(t)) & pmask);
+            }
+            for (int g = 0; g < c.parts; ++g) {
+                const std::uint64_t m = __ballot(q == g);
+                if (!m) continue;  // uniform over the wave
+                const int leader = __ffsll(static_cast<long long>(m)) - 1;
+                unsigned long long at = 0;
+                if (lane == leader) at = atomicAdd(&cursor[g], static_cast<unsigned long long>(__popcll(m)));
+                at = shfl_u64(at, leader, kWave);
+                if (q == g) {
+                    const auto pos = static_cast<std::int64_t>(at) + __popcll(m & lanemask_lt());
+                    group_keys[pos] = t;
+                    group_weights[pos] = weights[e];
+                }
+            }
+        }
+        __syncthreads();  // cursor is reused by the next chunk
+    }
+}
+
+// Items of one-item rows read the row and gather its clusters; items of split
+// rows read their group. Each new key is counted once per wave (a ballot), so at
+// most kPartThreads insertions are uncounted at any time and a table holding
+// fewer than kPartCap counted keys always has a free slot.
 template <typename O, typename W, typename V>
 __global__ void __launch_bounds__(kPartThreads) sclp_part_kernel(const std::int64_t* item_total,
                                                                  const std::int64_t* item_start,
@@ -546,6 +726,10 @@ __global__ void __launch_bounds__(kPartThreads) sclp_part_kernel(const std::int6
                                                                  const std::int32_t* list, const O* offsets,
                                                                  const std::int32_t* neighbors, const W* weights,
                                                                  const std::int32_t* clusters, RoundParams r,
+                                                                 const std::int32_t* group_keys,
+                                                                 const W* group_weights,
+                                                                 const std::int64_t* group_begin,
+                                                                 const std::int64_t* group_end,
                                                                  unsigned long long* partial_current,
                                                                  unsigned long long* partial_best,
                                                                  unsigned long long* partial_tie) {
@@ -553,13 +737,16 @@ __global__ void __launch_bounds__(kPartThreads) sclp_part_kernel(const std::int6
     __shared__ V values[kPartSlots];
     __shared__ int held;
     const std::int64_t items = *item_total;
+    const int lane = lane_id();
     for (std::int64_t w = blockIdx.x; w < items; w += gridDim.x) {
         const std::int32_t i = item_vertex[w];
         const std::int32_t v = list[i];
         const std::int64_t first = item_start[i];
         const auto parts = static_cast<std::uint32_t>(item_start[i + 1] - first);
         const auto p = static_cast<std::uint32_t>(w - first);
-        const std::int64_t begin = offsets[v], end = offsets[v + 1];
+        const bool grouped = parts > 1;
+        const std::int64_t begin = grouped ? group_begin[w] : static_cast<std::int64_t>(offsets[v]);
+        const std::int64_t end = grouped ? group_end[w] : static_cast<std::int64_t>(offsets[v + 1]);
         const std::int32_t source = clusters[v];
         int slots = kPartSlots;
         if (parts == 1) {  // 2 * degree slots (at most kPartSlots) for a single pass
@@ -580,24 +767,52 @@ __global__ void __launch_bounds__(kPartThreads) sclp_part_kernel(const std::int6
                 if (threadIdx.x == 0) held = 0;
                 __syncthreads();
                 const std::uint32_t select = p + parts * q;
-                for (std::int64_t e = begin + threadIdx.x; e < end; e += kPartThreads) {
-                    const std::int32_t u = neighbors[e];
-                    if (u == v) continue;
-                    const std::int32_t t = clusters[u];
-                    const std::uint32_t h = mix32(static_cast<std::uint32_t>(t));
-                    if ((h & mask) != select) continue;
-                    if (__hip_atomic_load(&held, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= kPartCap)
-                        continue;  // this pass is redone finer
-                    const auto wt = static_cast<V>(weights[e]);
-                    int slot = static_cast<int>(mix32(h ^ 0x9e3779b9U)) & slot_mask;
-                    while (true) {
-                        const std::int32_t found = atomicCAS(keys + slot, -1, t);
-                        if (found == -1 || found == t) {
-                            if (found == -1) atomicAdd(&held, 1);
-                            atomicAdd(values + slot, wt);
-                            break;
+                // kPartUnroll entries per lane and trip: their loads are issued together
+                for (std::int64_t e0 = begin; e0 < end; e0 += kPartThreads * kPartUnroll) {  // same trip count in every lane
+                    std::int32_t t[kPartUnroll];
+                    V wt[kPartUnroll];
+#pragma unroll
+                    for (int k = 0; k < kPartUnroll; ++k) {
+                        const std::int64_t e = e0 + threadIdx.x + k * kPartThreads;
+                        t[k] = -1;
+                        wt[k] = 0;
+                        if (e < end) {
+                            if (grouped) {
+                                t[k] = group_keys[e];
+                                wt[k] = static_cast<V>(group_weights[e]);
+                            } else {
+                                const std::int32_t u = neighbors[e];
+                                if (u != v) t[k] = u;
+                                wt[k] = static_cast<V>(weights[e]);
+                            }
                         }
-                        slot = (slot + 1) & slot_mask;
+                    }
+                    if (!grouped) {
+#pragma unroll
+                        for (int k = 0; k < kPartUnroll; ++k)
+                            if (t[k] >= 0) t[k] = clusters[t[k]];
+                    }
+#pragma unroll
+                    for (int k = 0; k < kPartUnroll; ++k) {
+                        bool fresh = false;
+                        const std::uint32_t h = mix32(static_cast<std::uint32_t>(t[k]));
+                        // a full table: this pass is redone finer
+                        if (t[k] >= 0 && (h & mask) == select &&
+                            __hip_atomic_load(&held, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < kPartCap) {
+                            int slot = static_cast<int>(mix32(h ^ 0x9e3779b9U)) & slot_mask;
+                            while (true) {
+                                const std::int32_t found = atomicCAS(keys + slot, -1, t[k]);
+                                if (found == -1 || found == t[k]) {
+                                    fresh = found == -1;
+                                    atomicAdd(values + slot, wt[k]);
+                                    break;
+                                }
+                                slot = (slot + 1) & slot_mask;
+                            }
+                        }
+                        // counted before the next insertion: at most one uncounted key per lane
+                        const std::uint64_t added = __ballot(fresh);
+                        if (lane == 0 && added) atomicAdd(&held, static_cast<int>(__popcll(added)));
                     }
                 }
                 __syncthreads();
@@ -989,10 +1204,14 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
     const bool parted = !single && c->high_count > 0 && c->max_degree > kHighLdsDegree;
     const std::int64_t hub_degree = single ? kHighLdsDegree : kPartMaxDegree;
     const bool hubs = c->high_count > 0 && c->max_degree > hub_degree;
-    HubChunks chunks, items;
+    HubChunks chunks, items, split_items;
     Scratch<std::int64_t> table_offset;
     Scratch<std::int32_t> gkeys;
     Scratch<unsigned long long> gvalues, pcurrent, pbest, ptie, icurrent, ibest, itie;
+    Scratch<std::int64_t> region, group_begin, group_end;
+    Scratch<std::int32_t> row_clusters, group_keys;
+    Scratch<W> group_weights;
+    Scratch<std::uint32_t> chunk_counts;
     if (parted) {
         // sum over rows of 2^ceil(log2(ceil(d / kPartKeys))) <= 2 * (entries / kPartKeys + rows)
         const std::int64_t bound = 2 * (c->high_edges / kPartKeys + 1) + 2 * c->high_count;
@@ -1000,6 +1219,23 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
         icurrent.reset(ctx, items.bound);
         ibest.reset(ctx, items.bound);
         itie.reset(ctx, items.bound);
+        if (c->max_degree > kPartKeys) {  // some row takes more than one item
+            region.reset(ctx, c->high_count + 1);
+            device_scan<std::int64_t, SumOp<std::int64_t>>(ctx, SplitLoad<O>{c->high, off, c->high_count},
+                                                           ExclusiveStore<std::int64_t>{region.get()},
+                                                           c->high_count + 1, SumOp<std::int64_t>{});
+            // at most high_edges / (kPartKeys + 1) rows exceed kPartKeys entries
+            // the chunk items of the split rows are a subset of the items
+            build_work_items(ctx, SplitItemLoad<O>{PartLoad<O>{c->high, off, c->high_count}}, c->high_count,
+                             items.bound, split_items);
+            chunk_counts.reset(ctx, items.bound * kMaxParts);
+            const std::int64_t split_edges = read_scalar(ctx, region.get() + c->high_count);
+            row_clusters.reset(ctx, split_edges);
+            group_keys.reset(ctx, split_edges);
+            group_weights.reset(ctx, split_edges);
+            group_begin.reset(ctx, items.bound);
+            group_end.reset(ctx, items.bound);
+        }
     }
     if (hubs) {
         build_hub_chunks(ctx, off, c->high, c->high_count, c->high_edges, hub_degree, chunks);
@@ -1052,16 +1288,29 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
         GPART_LAUNCH_CHECK();
     }
     if (parted) {
+        if (row_clusters.get()) {
+            const unsigned split_grid = static_cast<unsigned>(std::min<std::int64_t>(split_items.bound, kPartLaunchCap));
+            hipLaunchKernelGGL((sclp_split_count_kernel<O>), dim3(split_grid), dim3(kPartThreads), 0, ctx->stream,
+                               split_items.total(), split_items.hub.get(), split_items.start.get(), c->high, off,
+                               g->neighbors, clusters, region.get(), row_clusters.get(), chunk_counts.get());
+            hipLaunchKernelGGL((sclp_split_scatter_kernel<O, W>), dim3(split_grid), dim3(kPartThreads), 0,
+                               ctx->stream, split_items.total(), split_items.hub.get(), split_items.start.get(),
+                               c->high, off, w, region.get(), chunk_counts.get(), row_clusters.get(),
+                               items.start.get(), group_keys.get(), group_weights.get(), group_begin.get(),
+                               group_end.get());
+        }
         const unsigned grid = static_cast<unsigned>(std::min<std::int64_t>(items.bound, kPartLaunchCap));
         // connection sums fit 32 bits when every row weight does
         if (round->value_bound != 0 && round->value_bound <= 0xffffffffULL)
             hipLaunchKernelGGL((sclp_part_kernel<O, W, unsigned>), dim3(grid), dim3(kPartThreads), 0, ctx->stream,
                                items.total(), items.start.get(), items.hub.get(), c->high, off, g->neighbors, w,
-                               clusters, r, icurrent.get(), ibest.get(), itie.get());
+                               clusters, r, group_keys.get(), group_weights.get(), group_begin.get(), group_end.get(),
+                               icurrent.get(), ibest.get(), itie.get());
         else
             hipLaunchKernelGGL((sclp_part_kernel<O, W, unsigned long long>), dim3(grid), dim3(kPartThreads), 0,
                                ctx->stream, items.total(), items.start.get(), items.hub.get(), c->high, off,
-                               g->neighbors, w, clusters, r, icurrent.get(), ibest.get(), itie.get());
+                               g->neighbors, w, clusters, r, group_keys.get(), group_weights.get(),
+                               group_begin.get(), group_end.get(), icurrent.get(), ibest.get(), itie.get());
         hipLaunchKernelGGL(sclp_hub_decode_kernel, dim3(grid_for(c->high_count)), dim3(kBlock), 0, ctx->stream,
                            c->high_count, items.start.get(), c->high, clusters, r, icurrent.get(), ibest.get(),
                            itie.get(), proposals, gains, ties);
