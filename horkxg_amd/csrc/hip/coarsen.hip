@@ -519,7 +519,7 @@ __global__ void sclp_hub_decode_kernel(std::int64_t count, const std::int64_t* c
 constexpr int kPartThreads = 512;
 constexpr int kPartSlots = 8192;
 constexpr std::int64_t kPartKeys = 4096;
-constexpr std::int64_t kPartMaxDegree = 32768;
+constexpr std::int64_t kPartMaxDegree = 65536;
 constexpr int kPartCap = kPartSlots - kPartThreads - 1;  // keys held + one in flight per thread < slots
 constexpr int kPartLaunchCap = 1024;
 constexpr int kPartUnroll = 4;
