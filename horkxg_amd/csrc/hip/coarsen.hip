@@ -519,7 +519,7 @@ __global__ void sclp_hub_decode_kernel(std::int64_t count, const std::int64_t* c
 constexpr int kPartThreads = 512;
 constexpr int kPartSlots = 8192;
 constexpr std::int64_t kPartKeys = 4096;
-constexpr std::int64_t kPartMaxDegree = 65536;
+constexpr std::int64_t kPartMaxDegree = 131072;
 constexpr int kPartCap = kPartSlots - kPartThreads - 1;  // keys held + one in flight per thread < slots
 constexpr int kPartLaunchCap = 1024;
 constexpr int kPartUnroll = 4;
@@ -663,8 +663,8 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_scatter_kernel(
         const std::int64_t first_chunk = j - c.p;
         const std::int64_t base = region[c.i];
         const auto pmask = static_cast<std::uint32_t>(c.parts - 1);
-        if (threadIdx.x < c.parts * c.parts) {  // one load per thread
-            const int p = threadIdx.x / c.parts, g = threadIdx.x % c.parts;
+        for (int x = threadIdx.x; x < c.parts * c.parts; x += kPartThreads) {
+            const int p = x / c.parts, g = x % c.parts;
             counts[p * kMaxParts + g] = chunk_counts[(first_chunk + p) * kMaxParts + g];
         }
         __syncthreads();
