@@ -519,7 +519,7 @@ __global__ void sclp_hub_decode_kernel(std::int64_t count, const std::int64_t* c
 constexpr int kPartThreads = 512;
 constexpr int kPartSlots = 8192;
 constexpr std::int64_t kPartKeys = 4096;
-constexpr std::int64_t kPartMaxDegree = 131072;
+constexpr std::int64_t kPartMaxDegree = 262144;
 constexpr int kPartCap = kPartSlots - kPartThreads - 1;  // keys held + one in flight per thread < slots
 constexpr int kPartLaunchCap = 1024;
 constexpr int kPartUnroll = 4;
@@ -554,6 +554,7 @@ struct PartLoad {
 // p. Ranks within a chunk come from a ballot per wave and one LDS atomic per wave
 // and group; the LDS tables that read a group sum exactly, in any order.
 constexpr int kMaxParts = static_cast<int>(kPartMaxDegree / kPartKeys);
+constexpr int kBallotParts = 8;  // more groups: one LDS atomic per entry instead of a ballot per group
 
 template <typename O>
 struct SplitLoad {
@@ -635,9 +636,13 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_count_kernel(
 #pragma unroll
             for (int k = 0; k < kPartUnroll; ++k) {
                 const int q = t[k] >= 0 ? static_cast<int>(mix32(static_cast<std::uint32_t>(t[k])) & pmask) : -1;
-                for (int g = 0; g < c.parts; ++g) {
-                    const std::uint64_t m = __ballot(q == g);
-                    if (lane == 0 && m) atomicAdd(&counts[g], static_cast<unsigned>(__popcll(m)));
+                if (c.parts <= kBallotParts) {  // uniform over the workgroup
+                    for (int g = 0; g < c.parts; ++g) {
+                        const std::uint64_t m = __ballot(q == g);
+                        if (lane == 0 && m) atomicAdd(&counts[g], static_cast<unsigned>(__popcll(m)));
+                    }
+                } else if (q >= 0) {
+                    atomicAdd(&counts[q], 1u);
                 }
             }
         }
@@ -696,6 +701,14 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_scatter_kernel(
             if (e < c.end) {
                 t = row_clusters[base + (e - c.row_begin)];
                 if (t >= 0) q = static_cast<int>(mix32(static_cast<std::uint32_t>(t)) & pmask);
+            }
+            if (c.parts > kBallotParts) {  // uniform over the workgroup
+                if (q >= 0) {
+                    const auto pos = static_cast<std::int64_t>(atomicAdd(&cursor[q], 1ULL));
+                    group_keys[pos] = t;
+                    group_weights[pos] = weights[e];
+                }
+                continue;
             }
             for (int g = 0; g < c.parts; ++g) {
                 const std::uint64_t m = __ballot(q == g);
