@@ -554,7 +554,7 @@ struct PartLoad {
 // p. Ranks within a chunk come from a ballot per wave and one LDS atomic per wave
 // and group; the LDS tables that read a group sum exactly, in any order.
 constexpr int kMaxParts = static_cast<int>(kPartMaxDegree / kPartKeys);
-constexpr int kBallotParts = 8;  // more groups: one LDS atomic per entry instead of a ballot per group
+constexpr int kBallotParts = 2;  // more groups: one LDS atomic per entry instead of a ballot per group
 
 template <typename O>
 struct SplitLoad {

@@ -22,7 +22,7 @@ def need_gpu(hx):
 
 def hub_graph(n=6000, hub_degrees=(3000, 600), seed=1):
     """Ring + hubs: vertex i gets hub_degrees[i] random neighbours. Degrees above
-    1024 take the chunked global-table SCLP path, above 4096 several chunks (and
+    1024 take the grouped LDS SCLP path (in singleton rounds, chunks), above 4096 several items (and
     chunked cache rows in refinement)."""
     rng = np.random.default_rng(seed)
     rows = [set() for _ in range(n)]
