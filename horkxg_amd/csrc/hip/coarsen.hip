@@ -519,7 +519,7 @@ __global__ void sclp_hub_decode_kernel(std::int64_t count, const std::int64_t* c
 constexpr int kPartThreads = 512;
 constexpr int kPartSlots = 8192;
 constexpr std::int64_t kPartKeys = 4096;
-constexpr std::int64_t kPartMaxDegree = 262144;
+constexpr std::int64_t kPartMaxDegree = 524288;
 constexpr int kPartCap = kPartSlots - kPartThreads - 1;  // keys held + one in flight per thread < slots
 constexpr int kPartLaunchCap = 1024;
 constexpr int kPartUnroll = 4;
@@ -551,10 +551,10 @@ struct PartLoad {
 // order (-1 for a self loop) and counts, per chunk, the entries of each group
 // q = mix32(cluster) & (P - 1); sclp_split_scatter_kernel places chunk p's
 // entries of group q after those of the groups before q and of the chunks before
-// p. Ranks within a chunk come from a ballot per wave and one LDS atomic per wave
-// and group; the LDS tables that read a group sum exactly, in any order.
+// p. Ranks within a chunk come from one LDS atomic per entry (measured faster
+// than a ballot per group and wave, from 2 groups up); the LDS tables that read a
+// group sum exactly, in any order.
 constexpr int kMaxParts = static_cast<int>(kPartMaxDegree / kPartKeys);
-constexpr int kBallotParts = 2;  // more groups: one LDS atomic per entry instead of a ballot per group
 
 template <typename O>
 struct SplitLoad {
@@ -608,7 +608,6 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_count_kernel(
     const std::int32_t* list, const O* offsets, const std::int32_t* neighbors, const std::int32_t* clusters,
     const std::int64_t* region, std::int32_t* row_clusters, std::uint32_t* chunk_counts) {
     __shared__ unsigned int counts[kMaxParts];
-    const int lane = lane_id();
     const std::int64_t total = *split_total;
     for (std::int64_t j = blockIdx.x; j < total; j += gridDim.x) {
         const SplitChunk c = split_chunk(j, split_row, split_start, list, offsets);
@@ -636,14 +635,7 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_count_kernel(
 #pragma unroll
             for (int k = 0; k < kPartUnroll; ++k) {
                 const int q = t[k] >= 0 ? static_cast<int>(mix32(static_cast<std::uint32_t>(t[k])) & pmask) : -1;
-                if (c.parts <= kBallotParts) {  // uniform over the workgroup
-                    for (int g = 0; g < c.parts; ++g) {
-                        const std::uint64_t m = __ballot(q == g);
-                        if (lane == 0 && m) atomicAdd(&counts[g], static_cast<unsigned>(__popcll(m)));
-                    }
-                } else if (q >= 0) {
-                    atomicAdd(&counts[q], 1u);
-                }
+                if (q >= 0) atomicAdd(&counts[q], 1u);
             }
         }
         __syncthreads();
@@ -659,25 +651,19 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_scatter_kernel(
     const std::uint32_t* chunk_counts, const std::int32_t* row_clusters, const std::int64_t* item_start,
     std::int32_t* group_keys, W* group_weights, std::int64_t* group_begin, std::int64_t* group_end) {
     __shared__ unsigned long long cursor[kMaxParts];
-    __shared__ unsigned int counts[kMaxParts * kMaxParts];  // [chunk][group] of the row
     __shared__ unsigned int sizes[kMaxParts];
-    const int lane = lane_id();
     const std::int64_t total = *split_total;
     for (std::int64_t j = blockIdx.x; j < total; j += gridDim.x) {
         const SplitChunk c = split_chunk(j, split_row, split_start, list, offsets);
         const std::int64_t first_chunk = j - c.p;
         const std::int64_t base = region[c.i];
         const auto pmask = static_cast<std::uint32_t>(c.parts - 1);
-        for (int x = threadIdx.x; x < c.parts * c.parts; x += kPartThreads) {
-            const int p = x / c.parts, g = x % c.parts;
-            counts[p * kMaxParts + g] = chunk_counts[(first_chunk + p) * kMaxParts + g];
-        }
-        __syncthreads();
+        // thread g: group g's size over the row's chunks, and its entries in chunks before p
         unsigned int before = 0;
         if (threadIdx.x < c.parts) {
             unsigned int size = 0;
             for (int p = 0; p < c.parts; ++p) {
-                const unsigned int n = counts[p * kMaxParts + threadIdx.x];
+                const unsigned int n = chunk_counts[(first_chunk + p) * kMaxParts + threadIdx.x];
                 size += n;
                 if (p < c.p) before += n;
             }
@@ -702,26 +688,10 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_scatter_kernel(
                 t = row_clusters[base + (e - c.row_begin)];
                 if (t >= 0) q = static_cast<int>(mix32(static_cast<std::uint32_t>(t)) & pmask);
             }
-            if (c.parts > kBallotParts) {  // uniform over the workgroup
-                if (q >= 0) {
-                    const auto pos = static_cast<std::int64_t>(atomicAdd(&cursor[q], 1ULL));
-                    group_keys[pos] = t;
-                    group_weights[pos] = weights[e];
-                }
-                continue;
-            }
-            for (int g = 0; g < c.parts; ++g) {
-                const std::uint64_t m = __ballot(q == g);
-                if (!m) continue;  // uniform over the wave
-                const int leader = __ffsll(static_cast<long long>(m)) - 1;
-                unsigned long long at = 0;
-                if (lane == leader) at = atomicAdd(&cursor[g], static_cast<unsigned long long>(__popcll(m)));
-                at = shfl_u64(at, leader, kWave);
-                if (q == g) {
-                    const auto pos = static_cast<std::int64_t>(at) + __popcll(m & lanemask_lt());
-                    group_keys[pos] = t;
-                    group_weights[pos] = weights[e];
-                }
+            if (q >= 0) {
+                const auto pos = static_cast<std::int64_t>(atomicAdd(&cursor[q], 1ULL));
+                group_keys[pos] = t;
+                group_weights[pos] = weights[e];
             }
         }
         __syncthreads();  // cursor is reused by the next chunk
@@ -1238,10 +1208,10 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
                                                            ExclusiveStore<std::int64_t>{region.get()},
                                                            c->high_count + 1, SumOp<std::int64_t>{});
             // at most high_edges / (kPartKeys + 1) rows exceed kPartKeys entries
-            // the chunk items of the split rows are a subset of the items
+            // a split row of d entries has fewer than 2 d / kPartKeys items
             build_work_items(ctx, SplitItemLoad<O>{PartLoad<O>{c->high, off, c->high_count}}, c->high_count,
-                             items.bound, split_items);
-            chunk_counts.reset(ctx, items.bound * kMaxParts);
+                             std::min<std::int64_t>(items.bound, 2 * (c->high_edges / kPartKeys) + 1), split_items);
+            chunk_counts.reset(ctx, split_items.bound * kMaxParts);
             const std::int64_t split_edges = read_scalar(ctx, region.get() + c->high_count);
             row_clusters.reset(ctx, split_edges);
             group_keys.reset(ctx, split_edges);

@@ -51,9 +51,9 @@ GRAPHS = {
     "rmat12": lambda hx: hx.generate_rmat(12, 16),
     "hub": lambda hx: hub_graph(),
     "bighub": lambda hx: hub_graph(60000, (45000, 20000, 9000, 5000, 1500, 700), seed=3),
-    # rows above 262144 entries: the per-hub global tables outside singleton rounds;
-    # 16 to 64 groups per split row: one LDS atomic per entry
-    "hugehub": lambda hx: hub_graph(500000, (400000, 300000, 200000, 100000, 70000), seed=5),
+    # rows above 524288 entries: the per-hub global tables outside singleton rounds;
+    # split rows of 16 to 128 groups
+    "hugehub": lambda hx: hub_graph(700000, (600000, 400000, 300000, 100000, 70000), seed=5),
 }
 
 
