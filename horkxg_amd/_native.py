@@ -187,6 +187,8 @@ def host_lib():
         L.gpart_hierarchy_free.argtypes = [C.c_void_p]
         L.gpart_hierarchy_uncoarsen.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_double, C.c_int, C.c_double,
                                                 C.c_int, _i32p, C.POINTER(C.c_uint64)]
+        L.gpart_hierarchy_uncoarsen_host.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_double, C.c_int, C.c_double,
+                                                     C.c_int, _i32p]
         L.gpart_graph_initial_partition.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_int, _i32p,
                                                     C.c_void_p]
         L.gpart_graph_refine.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
@@ -384,10 +386,25 @@ class Hierarchy:
         _check(host_lib().gpart_hierarchy_write(self._h, str(path).encode()))
 
     def uncoarsen(self, coarsest_labels, parts: int, ratio: float = 1.03, max_rounds: int = 32,
-                  growth: float = 2.0, verify: bool = False):
+                  growth: float = 2.0, verify: bool = False, entry: str = "refine_hierarchy_device"):
+        """Projection + refinement from labels of the coarsest level. entry
+        "refine_hierarchy_device" returns (labels, final cut); entry "uncoarsen" goes
+        through the host uncoarsen overload and returns (labels, None)."""
         lab = np.ascontiguousarray(coarsest_labels, dtype=np.int32)
-        n0 = self.level(0)[0].shape[0] - 1
-        out = np.empty(n0, np.int32)
+        n0 = C.c_int64()
+        m0 = C.c_int64()
+        _check(host_lib().gpart_hierarchy_level_size(self._h, 0, C.byref(n0), C.byref(m0)))
+        n_coarse = C.c_int64()
+        _check(host_lib().gpart_hierarchy_level_size(self._h, len(self) - 1, C.byref(n_coarse), C.byref(m0)))
+        if lab.shape != (n_coarse.value,):
+            raise GpartError(1, "coarsest labels must have one entry per coarsest vertex")
+        out = np.empty(n0.value, np.int32)
+        if entry == "uncoarsen":
+            _check(host_lib().gpart_hierarchy_uncoarsen_host(self._h, lab, parts, ratio, max_rounds, growth,
+                                                             int(verify), out))
+            return out, None
+        if entry != "refine_hierarchy_device":
+            raise ValueError(f"unknown entry {entry!r}")
         cut = C.c_uint64()
         _check(host_lib().gpart_hierarchy_uncoarsen(self._h, lab, parts, ratio, max_rounds, growth, int(verify),
                                                     out, C.byref(cut)))

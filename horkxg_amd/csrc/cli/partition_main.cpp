@@ -1,6 +1,17 @@
-// partition_main.cpp -- gpart_partition_{standard,big}: the drop-in CLI
-// (reference: app/partition_main.cu:20-142; same arguments, exit codes, timed
-// scope and key log lines).
+// partition_main.cpp -- gpart_partition_{standard,big}: the drop-in partitioning tool.
+//
+//   gpart_partition_<width> <indptr.bin> <indices.bin> <parts> <partition.out>
+//                           [imbalance_ratio=1.03] [seed] [--verify]
+//
+// Same arguments, defaults, exit codes, timed scope and report keys as
+// app/partition_main.cu:20-142. algorithm_total_seconds runs from the start of
+// coarsening (input resident on the device, device idle) to the hierarchy's
+// release; it leaves out loading, validation, the upload, the final label
+// download and the file write.
+//
+// GPART_SPECTRAL_MODE selects the bisection form: 0 plain FP64 (one task at a
+// time), 1 batched FP64 (default, bit-identical labels), 2 batched FP32.
+#include "cli_support.hpp"
 #include "coarsen.hpp"
 #include "graph.hpp"
 #include "initial_partition.hpp"
@@ -8,115 +19,165 @@
 #include "uncoarsen.hpp"
 
 #include <algorithm>
-#include <chrono>
-#include <cstdint>
 #include <cstdlib>
 #include <fstream>
-#include <iomanip>
-#include <iostream>
 #include <stdexcept>
-#include <string>
+
+namespace {
+
+using gpart::cli::Arguments;
+using gpart::detail::ReportLine;
+using Types = gpart::ActiveTypes;
+using Vertex = Types::VertexT;
+
+const gpart::cli::Grammar kGrammar{
+    {"indptr.bin", "indices.bin", "parts", "partition.out"},
+    {"imbalance_ratio=1.03", "seed"},
+    "--verify",
+    true,
+};
+
+enum Position : std::size_t { kIndptr, kIndices, kParts, kOutput, kRatio, kSeed };
+
+struct Settings {
+    int parts = 0;
+    double ratio = 1.03;
+    std::uint32_t seed = 0;
+    bool verify = false;
+    int spectral_mode = 1;
+};
+
+Settings settings_from(const Arguments& args) {
+    Settings s;
+    s.parts = args.value_or(kParts, 0, gpart::cli::to_int);
+    s.ratio = args.value_or(kRatio, 1.03, gpart::cli::to_double);
+    s.seed = args.value_or<std::uint32_t>(kSeed, 0u, gpart::cli::to_ulong);
+    s.verify = args.flag();
+    if (const char* mode = std::getenv("GPART_SPECTRAL_MODE")) s.spectral_mode = std::atoi(mode);
+    if (s.spectral_mode < 0 || s.spectral_mode > 2)
+        throw std::invalid_argument("GPART_SPECTRAL_MODE must be 0, 1 or 2");
+    return s;
+}
+
+// The production coarsening options: at least 64 coarse clusters per part and
+// packed isolated vertices (app/partition_main.cu:47-56).
+gpart::CoarsenOptions coarsen_options(const Settings& s) {
+    gpart::CoarsenOptions o;
+    o.parts = s.parts;
+    o.seed = s.seed;
+    o.strict_verify = s.verify;
+    o.cluster_size_factor = std::max(64, (512 + s.parts - 1) / s.parts);
+    o.pack_isolated = true;
+    return o;
+}
+
+gpart::RefineOptions refine_options(const Settings& s) {
+    gpart::RefineOptions o;
+    o.parts = s.parts;
+    o.imbalance_ratio = s.ratio;
+    o.max_rounds = 32;
+    o.strict_verify = s.verify;
+    o.refinement_growth_ratio = 2.0;
+    return o;
+}
+
+void device_idle() { GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context())); }
+
+void write_labels(const std::string& path, const std::vector<Vertex>& labels) {
+    std::ofstream file(path, std::ios::binary | std::ios::trunc);
+    if (!file) throw std::runtime_error("cannot open partition output");
+    file.write(reinterpret_cast<const char*>(labels.data()),
+               static_cast<std::streamsize>(labels.size() * sizeof(Vertex)));
+    file.close();
+    if (!file) throw std::runtime_error("cannot write partition output");
+}
+
+int partition_tool(int argc, char** argv) {
+    gpart::cli::Marks t;
+    t.mark("wall");
+    const auto args = Arguments::parse(argc, argv, kGrammar);
+    if (!args) {
+        std::cerr << gpart::cli::usage_text(argv[0], kGrammar);
+        return gpart::cli::kUsage;
+    }
+    const Settings s = settings_from(*args);
+
+    // Input. Without --verify the int64 files stream to the device, where they are
+    // narrowed and shape-checked (only the files cross PCIe). --verify keeps a host
+    // copy for the full CSR validation (sorted rows, symmetry).
+    t.mark("load");
+    gpart::WeightedGraph<Types> host_graph;
+    gpart::DeviceWeightedGraph<Types> graph;
+    if (s.verify) {
+        host_graph = gpart::load_weighted_graph<Types>((*args)[kIndptr], (*args)[kIndices]);
+    } else {
+        graph = gpart::load_device_graph<Types>((*args)[kIndptr], (*args)[kIndices]);
+    }
+    const std::int64_t n = s.verify ? host_graph.vertices() : graph.vertices();
+    gpart::partition_capacity(static_cast<std::uint64_t>(n), s.parts, s.ratio);
+    t.mark("loaded");
+    ReportLine().field("verify", s.verify ? 1 : 0).write(std::cout);
+
+    const gpart::CoarsenOptions co = coarsen_options(s);
+    ReportLine()
+        .field("pipeline_mode", "incremental_bucket")
+        .field("cluster_size_factor", co.cluster_size_factor)
+        .field("pack_isolated", co.pack_isolated ? 1 : 0)
+        .field("spectral_mode", s.spectral_mode)
+        .write(std::cout);
+
+    t.mark("prepare");
+    if (s.verify) {
+        gpart::validate_weighted_csr(host_graph, true);
+        graph = gpart::make_device_weighted(host_graph);
+    }
+    device_idle();
+
+    // ---- timed scope ----
+    t.mark("coarsen");
+    auto hierarchy = gpart::coarsen<Types>(std::move(graph), co);
+    device_idle();
+    t.mark("initial");
+    auto coarsest = gpart::initial_partition_gpu<Types>(hierarchy.levels.back(), s.parts, s.ratio, s.seed,
+                                                        static_cast<gpart::SpectralMode>(s.spectral_mode));
+    device_idle();
+    t.mark("uncoarsen");
+    auto result = gpart::uncoarsen<Types>(hierarchy, std::move(coarsest), refine_options(s));
+    device_idle();
+    t.mark("free");
+    hierarchy.levels.clear();
+    hierarchy.fine_to_coarse.clear();
+    auto labels_on_device = std::move(result.device_partition);
+    gpart::detail::release_context_cache();
+    t.mark("freed");
+    // ---- end of the timed scope ----
+
+    const std::vector<Vertex> labels = labels_on_device.to_host();
+    t.mark("downloaded");
+    t.mark("output");
+    write_labels((*args)[kOutput], labels);
+    t.mark("written");
+
+    const double algorithm = t.span("coarsen", "freed");
+    ReportLine("partition_timing")
+        .field("input_load_seconds", t.span("load", "loaded"))
+        .field("coarsen_seconds", t.span("coarsen", "initial"))
+        .field("initial_partition_seconds", t.span("initial", "uncoarsen"))
+        .field("uncoarsen_seconds", t.span("uncoarsen", "free"))
+        .field("input_prepare_seconds", t.span("prepare", "coarsen"))
+        .field("hierarchy_free_seconds", t.span("free", "freed"))
+        .field("final_partition_d2h_seconds", t.span("freed", "downloaded"))
+        .field("timing_scope", "jet_device_input_to_device_partition")
+        .field("partition_output_seconds", t.span("output", "written"))
+        .field("partition_total_seconds", algorithm)
+        .field("algorithm_total_seconds", algorithm)
+        .field("wall_total_seconds", t.span("wall", "written"))
+        .write(std::cout);
+    return gpart::cli::kSuccess;
+}
+
+}  // namespace
 
 int main(int argc, char** argv) {
-    try {
-        const auto wall_start = std::chrono::steady_clock::now();
-        bool verify = false;
-        int positional_argc = argc;
-        while (positional_argc > 1) {
-            const std::string flag(argv[positional_argc - 1]);
-            if (flag == "--verify") verify = true;
-            else break;
-            --positional_argc;
-        }
-        if (positional_argc < 5 || positional_argc > 7) {
-            std::cerr << "Usage: " << argv[0] << " <indptr.bin> <indices.bin> <parts> <partition.out>"
-                      << " [imbalance_ratio=1.03] [seed] [--verify]\n";
-            return 2;
-        }
-        using Types = gpart::ActiveTypes;
-        const int parts = std::stoi(argv[3]);
-        const double imbalance = positional_argc >= 6 ? std::stod(argv[5]) : 1.03;
-        const auto seed = positional_argc >= 7 ? static_cast<std::uint32_t>(std::stoul(argv[6])) : 0U;
-        // GPART_SPECTRAL_MODE selects the bisection form: 0 plain FP64 (the
-        // reference's launch sequence), 1 batched FP64 (default), 2 batched FP32.
-        const char* mode_env = std::getenv("GPART_SPECTRAL_MODE");
-        const int mode = mode_env ? std::atoi(mode_env) : 1;
-        if (mode < 0 || mode > 2) throw std::invalid_argument("GPART_SPECTRAL_MODE must be 0, 1 or 2");
-        const auto load_start = std::chrono::steady_clock::now();
-        // Without --verify the input is narrowed and shape-checked on the device (only
-        // the int64 files cross PCIe); --verify keeps the host copy for the symmetry check.
-        gpart::WeightedGraph<Types> graph;
-        gpart::DeviceWeightedGraph<Types> device_graph;
-        if (verify) graph = gpart::load_weighted_graph<Types>(argv[1], argv[2]);
-        else device_graph = gpart::load_device_graph<Types>(argv[1], argv[2]);
-        const std::int64_t vertices = verify ? graph.vertices() : device_graph.vertices();
-        gpart::partition_capacity(static_cast<std::uint64_t>(vertices), parts, imbalance);
-        const auto load_end = std::chrono::steady_clock::now();
-        std::cout << "verify=" << (verify ? 1 : 0) << '\n';
-        gpart::CoarsenOptions coarsen_options;
-        coarsen_options.parts = parts;
-        coarsen_options.seed = seed;
-        coarsen_options.strict_verify = verify;
-        // at least 64 coarse clusters per part (app/partition_main.cu:50-56)
-        coarsen_options.cluster_size_factor = std::max(64, (512 + parts - 1) / parts);
-        coarsen_options.pack_isolated = true;
-        std::cout << "pipeline_mode=incremental_bucket cluster_size_factor=" << coarsen_options.cluster_size_factor
-                  << " pack_isolated=" << coarsen_options.pack_isolated << " spectral_mode=" << mode << '\n';
-        const auto prepare_start = std::chrono::steady_clock::now();
-        if (verify) {
-            gpart::validate_weighted_csr(graph, true);
-            device_graph = gpart::make_device_weighted(graph);
-        }
-        GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context()));
-        const auto coarsen_start = std::chrono::steady_clock::now();
-        auto hierarchy = gpart::coarsen<Types>(std::move(device_graph), coarsen_options);
-        GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context()));
-        const auto coarsen_end = std::chrono::steady_clock::now();
-        auto coarse_partition = gpart::initial_partition_gpu<Types>(hierarchy.levels.back(), parts, imbalance, seed,
-                                                                    static_cast<gpart::SpectralMode>(mode));
-        GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context()));
-        const auto initial_end = std::chrono::steady_clock::now();
-        gpart::RefineOptions refine_options;
-        refine_options.parts = parts;
-        refine_options.imbalance_ratio = imbalance;
-        refine_options.max_rounds = 32;
-        refine_options.strict_verify = verify;
-        refine_options.refinement_growth_ratio = 2.0;
-        auto result = gpart::uncoarsen<Types>(hierarchy, std::move(coarse_partition), refine_options);
-        GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context()));
-        const auto uncoarsen_end = std::chrono::steady_clock::now();
-        hierarchy.levels.clear();
-        hierarchy.fine_to_coarse.clear();
-        auto device_partition = std::move(result.device_partition);
-        gpart::detail::release_context_cache();
-        const auto free_end = std::chrono::steady_clock::now();
-        const auto partition = device_partition.to_host();
-        const auto final_d2h_end = std::chrono::steady_clock::now();
-        const auto output_start = std::chrono::steady_clock::now();
-        std::ofstream output(argv[4], std::ios::binary | std::ios::trunc);
-        if (!output) throw std::runtime_error("cannot open partition output");
-        output.write(reinterpret_cast<const char*>(partition.data()),
-                     static_cast<std::streamsize>(partition.size() * sizeof(typename Types::VertexT)));
-        output.close();
-        if (!output) throw std::runtime_error("cannot write partition output");
-        const auto output_end = std::chrono::steady_clock::now();
-        auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
-        const double algorithm_seconds = secs(coarsen_start, free_end);
-        std::cout << std::setprecision(10) << "partition_timing input_load_seconds=" << secs(load_start, load_end)
-                  << " coarsen_seconds=" << secs(coarsen_start, coarsen_end)
-                  << " initial_partition_seconds=" << secs(coarsen_end, initial_end)
-                  << " uncoarsen_seconds=" << secs(initial_end, uncoarsen_end)
-                  << " input_prepare_seconds=" << secs(prepare_start, coarsen_start)
-                  << " hierarchy_free_seconds=" << secs(uncoarsen_end, free_end)
-                  << " final_partition_d2h_seconds=" << secs(free_end, final_d2h_end)
-                  << " timing_scope=jet_device_input_to_device_partition"
-                  << " partition_output_seconds=" << secs(output_start, output_end)
-                  << " partition_total_seconds=" << algorithm_seconds
-                  << " algorithm_total_seconds=" << algorithm_seconds
-                  << " wall_total_seconds=" << secs(wall_start, output_end) << '\n';
-        return 0;
-    } catch (const std::exception& error) {
-        std::cerr << "gpart_partition: " << error.what() << '\n';
-        return 1;
-    }
+    return gpart::cli::run_tool("gpart_partition", [&] { return partition_tool(argc, argv); });
 }

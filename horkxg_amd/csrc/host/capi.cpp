@@ -664,6 +664,26 @@ int gpart_hierarchy_uncoarsen(const gpart_hierarchy* h, const int32_t* coarsest_
     });
 }
 
+int gpart_hierarchy_uncoarsen_host(const gpart_hierarchy* h, const int32_t* coarsest_labels, int parts,
+                                   double ratio, int max_rounds, double growth, int verify, int32_t* labels) {
+    return guard([&] {
+        std::visit(
+            [&](const auto& x) {
+                gpart::RefineOptions o;
+                o.parts = parts;
+                o.imbalance_ratio = ratio;
+                o.max_rounds = max_rounds;
+                o.refinement_growth_ratio = growth;
+                o.strict_verify = verify != 0;
+                const auto& coarsest = x.levels.back();
+                const std::vector<std::int32_t> c(coarsest_labels, coarsest_labels + coarsest.vertices());
+                const auto out = gpart::uncoarsen(x, c, o);
+                std::copy(out.begin(), out.end(), labels);
+            },
+            h->h);
+    });
+}
+
 int gpart_graph_initial_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int spectral_mode,
                                   int32_t* labels, double* coords) {
     return guard([&] {

@@ -339,7 +339,6 @@ void validate_coarsening_step(const WeightedGraph<Types>& fine, const WeightedGr
 template <typename Types, bool Resident>
 auto coarsen_impl(const WeightedGraph<Types>* graph, DeviceWeightedGraph<Types> current,
                   const CoarsenOptions& options) {
-    using VertexT = typename Types::VertexT;
     if (options.parts < 2 || options.parts > 32 || options.cluster_size_factor < 2 || options.max_levels < 0 ||
         !std::isfinite(options.stop_contraction_ratio) || options.stop_contraction_ratio <= 0 ||
         options.stop_contraction_ratio > 1)

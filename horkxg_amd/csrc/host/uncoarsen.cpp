@@ -1,70 +1,97 @@
-// uncoarsen.cpp -- projection + refinement driver and its report lines
-// (reference: src/uncoarsen.cu:39-112).
+// uncoarsen.cpp -- the uncoarsen entry points: projection and refinement over a
+// whole hierarchy (done by refine_hierarchy_device), then one report line per
+// refined level and the final_* lines.
+//
+// Reference interface: include/uncoarsen.hpp:10-22, src/uncoarsen.cu:39-112. The
+// log keys and their meaning are the drop-in contract that scripts parse.
 #include "log.hpp"
 #include "uncoarsen.hpp"
 
-#include <algorithm>
 #include <chrono>
-#include <iomanip>
-#include <iostream>
-#include <limits>
 #include <stdexcept>
 
 namespace gpart {
 namespace {
 
-std::uint64_t sum_weights(const std::vector<std::uint64_t>& weights) {
+// Total, heaviest part and imbalance (heaviest * parts / total) of the final
+// part weights; the total must fit 64 bits.
+struct PartLoadSummary {
     std::uint64_t total = 0;
-    for (const auto w : weights) {
-        if (total > std::numeric_limits<std::uint64_t>::max() - w) throw std::overflow_error("total vertex weight overflow");
-        total += w;
+    std::uint64_t heaviest = 0;
+    double imbalance = 0.0;
+};
+
+PartLoadSummary summarize(const std::vector<std::uint64_t>& loads) {
+    PartLoadSummary s;
+    for (const std::uint64_t w : loads) {
+        if (__builtin_add_overflow(s.total, w, &s.total)) throw std::overflow_error("total vertex weight overflow");
+        if (w > s.heaviest) s.heaviest = w;
     }
-    return total;
+    if (s.total != 0)
+        s.imbalance = static_cast<double>(s.heaviest) * static_cast<double>(loads.size()) /
+                      static_cast<double>(s.total);
+    return s;
 }
 
-double imbalance_of(const std::vector<std::uint64_t>& weights, std::uint64_t total) {
-    if (total == 0) return 0.0;
-    const auto maximum = *std::max_element(weights.begin(), weights.end());
-    return static_cast<double>(maximum) * weights.size() / static_cast<double>(total);
+void report_level(const RefineLevelResult& r) {
+    const RefineStats& s = r.refinement;
+    const RefineLevelTimings& t = r.timings;
+    detail::ReportLine line;
+    line.field("uncoarsen_level", r.level)
+        .field("vertices", r.vertices)
+        .field("edge_entries", r.edge_entries)
+        .field("projection_cut", r.projection_cut)
+        .field("projection_max_part_weight", r.projection_max_part_weight)
+        .field("projection_imbalance", r.projection_imbalance)
+        .field("refine_rounds", s.rounds)
+        .field("refine_proposals", s.proposals)
+        .field("refine_accepted", s.accepted)
+        .field("admission_iterations", s.admission_iterations)
+        .field("admission_fallbacks", s.admission_fallbacks)
+        .field("refined_cut", s.final_cut)
+        .field("refined_max_part_weight", s.final_max_part_weight)
+        .field("graph_h2d_seconds", t.graph_h2d_seconds)
+        .field("workspace_resize_seconds", t.workspace_resize_seconds)
+        .field("projection_gpu_seconds", t.projection_gpu_seconds)
+        .field("refinement_seconds", t.refinement_seconds)
+        .field("verification_seconds", t.verification_seconds)
+        // beyond the reference's keys: the apply path's volume and the rounds
+        // that rebuilt the connectivity cache
+        .field("mover_entries", s.mover_entries)
+        .field("cache_rebuilds", s.cache_rebuilds);
+    line.write(detail::log());
 }
 
-template <typename Types, typename H, typename P>
-DeviceUncoarsenResult<Types> uncoarsen_impl(H& hierarchy, P&& coarsest_partition, const RefineOptions& options) {
-    const auto start = std::chrono::steady_clock::now();
-    if (hierarchy.levels.empty() || hierarchy.fine_to_coarse.size() + 1 != hierarchy.levels.size())
-        throw std::invalid_argument("uncoarsen requires a complete hierarchy");
-    auto result = refine_hierarchy_device(hierarchy, std::forward<P>(coarsest_partition), options);
-    for (const auto& level : result.levels) {
-        const auto& s = level.refinement;
-        const auto& t = level.timings;
-        gpart::detail::log() << std::setprecision(10) << "uncoarsen_level=" << level.level << " vertices=" << level.vertices
-                  << " edge_entries=" << level.edge_entries << " projection_cut=" << level.projection_cut
-                  << " projection_max_part_weight=" << level.projection_max_part_weight
-                  << " projection_imbalance=" << level.projection_imbalance << " refine_rounds=" << s.rounds
-                  << " refine_proposals=" << s.proposals << " refine_accepted=" << s.accepted
-                  << " admission_iterations=" << s.admission_iterations
-                  << " admission_fallbacks=" << s.admission_fallbacks << " refined_cut=" << s.final_cut
-                  << " refined_max_part_weight=" << s.final_max_part_weight
-                  << " graph_h2d_seconds=" << t.graph_h2d_seconds
-                  << " workspace_resize_seconds=" << t.workspace_resize_seconds
-                  << " projection_gpu_seconds=" << t.projection_gpu_seconds
-                  << " refinement_seconds=" << t.refinement_seconds
-                  << " verification_seconds=" << t.verification_seconds << " mover_entries=" << s.mover_entries
-                  << " cache_rebuilds=" << s.cache_rebuilds << '\n';
-    }
-    const auto total = sum_weights(result.final_part_weights);
-    const auto maximum = *std::max_element(result.final_part_weights.begin(), result.final_part_weights.end());
-    gpart::detail::log() << std::setprecision(10) << "final_edge_cut=" << result.final_cut << '\n'
-              << "final_cut_ratio="
-              << (result.total_edge_weight == 0
-                      ? 0.0
-                      : static_cast<double>(result.final_cut) / static_cast<double>(result.total_edge_weight))
-              << '\n'
-              << "final_max_part_weight=" << maximum << '\n'
-              << "final_imbalance=" << imbalance_of(result.final_part_weights, total) << '\n'
-              << "final_d2h_seconds=" << result.final_d2h_seconds << '\n'
-              << "uncoarsen_total_seconds="
-              << std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count() << '\n';
+template <typename Types>
+void report_final(const DeviceUncoarsenResult<Types>& result, double seconds) {
+    if (result.final_part_weights.empty()) throw std::runtime_error("uncoarsen produced no part weights");
+    const PartLoadSummary loads = summarize(result.final_part_weights);
+    const double cut_ratio = result.total_edge_weight == 0
+                                 ? 0.0
+                                 : static_cast<double>(result.final_cut) /
+                                       static_cast<double>(result.total_edge_weight);
+    auto& out = detail::log();
+    detail::ReportLine().field("final_edge_cut", result.final_cut).write(out);
+    detail::ReportLine().field("final_cut_ratio", cut_ratio).write(out);
+    detail::ReportLine().field("final_max_part_weight", loads.heaviest).write(out);
+    detail::ReportLine().field("final_imbalance", loads.imbalance).write(out);
+    detail::ReportLine().field("final_d2h_seconds", result.final_d2h_seconds).write(out);
+    detail::ReportLine().field("uncoarsen_total_seconds", seconds).write(out);
+}
+
+// H is a host Hierarchy (levels uploaded as they are refined) or a DeviceHierarchy
+// (levels moved out); Labels is the matching coarsest partition.
+template <typename Types, typename H, typename Labels>
+DeviceUncoarsenResult<Types> refine_and_report(H& hierarchy, Labels&& coarsest, const RefineOptions& options) {
+    using clock = std::chrono::steady_clock;
+    const auto begin = clock::now();
+    const bool complete = !hierarchy.levels.empty() &&
+                          hierarchy.levels.size() == hierarchy.fine_to_coarse.size() + 1;
+    if (!complete) throw std::invalid_argument("uncoarsen requires a complete hierarchy");
+    DeviceUncoarsenResult<Types> result =
+        refine_hierarchy_device(hierarchy, std::forward<Labels>(coarsest), options);
+    for (const RefineLevelResult& level : result.levels) report_level(level);
+    report_final(result, std::chrono::duration<double>(clock::now() - begin).count());
     return result;
 }
 
@@ -72,27 +99,25 @@ DeviceUncoarsenResult<Types> uncoarsen_impl(H& hierarchy, P&& coarsest_partition
 
 template <typename Types>
 std::vector<typename Types::VertexT> uncoarsen(const Hierarchy<Types>& hierarchy,
-                                               const std::vector<typename Types::VertexT>& partition,
+                                               const std::vector<typename Types::VertexT>& coarsest_partition,
                                                const RefineOptions& options) {
-    auto result = uncoarsen_impl<Types>(hierarchy, partition, options);
-    return std::move(result.partition);
+    return refine_and_report<Types>(hierarchy, coarsest_partition, options).partition;
 }
 
 template <typename Types>
 DeviceUncoarsenResult<Types> uncoarsen(DeviceHierarchy<Types>& hierarchy,
-                                       DeviceVector<typename Types::VertexT>&& partition,
+                                       DeviceVector<typename Types::VertexT>&& coarsest_partition,
                                        const RefineOptions& options) {
-    return uncoarsen_impl<Types>(hierarchy, std::move(partition), options);
+    return refine_and_report<Types>(hierarchy, std::move(coarsest_partition), options);
 }
 
-template std::vector<std::int32_t> uncoarsen<StandardTypes>(const Hierarchy<StandardTypes>&,
-                                                            const std::vector<std::int32_t>&, const RefineOptions&);
-template std::vector<std::int32_t> uncoarsen<BigTypes>(const Hierarchy<BigTypes>&, const std::vector<std::int32_t>&,
-                                                       const RefineOptions&);
-template DeviceUncoarsenResult<StandardTypes> uncoarsen<StandardTypes>(DeviceHierarchy<StandardTypes>&,
-                                                                       DeviceVector<std::int32_t>&&,
-                                                                       const RefineOptions&);
-template DeviceUncoarsenResult<BigTypes> uncoarsen<BigTypes>(DeviceHierarchy<BigTypes>&, DeviceVector<std::int32_t>&&,
-                                                             const RefineOptions&);
+#define GPART_UNCOARSEN_INSTANCES(T)                                                                        \
+    template std::vector<std::int32_t> uncoarsen<T>(const Hierarchy<T>&, const std::vector<std::int32_t>&, \
+                                                    const RefineOptions&);                                \
+    template DeviceUncoarsenResult<T> uncoarsen<T>(DeviceHierarchy<T>&, DeviceVector<std::int32_t>&&,     \
+                                                   const RefineOptions&);
+GPART_UNCOARSEN_INSTANCES(StandardTypes)
+GPART_UNCOARSEN_INSTANCES(BigTypes)
+#undef GPART_UNCOARSEN_INSTANCES
 
 }  // namespace gpart

@@ -41,7 +41,7 @@ public:
     DeviceVector(const std::vector<T>& host) { assign(host); }
     DeviceVector(const DeviceVector& other) { copy_from(other); }
     DeviceVector(DeviceVector&& other) noexcept
-        : ctx_(other.ctx_), data_(other.data_), size_(other.size_), borrowed_(other.borrowed_) {
+        : ctx_(other.ctx_), data_(other.data_), borrowed_(other.borrowed_), size_(other.size_) {
         other.ctx_ = nullptr;
         other.data_ = nullptr;
         other.size_ = 0;

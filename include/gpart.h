@@ -14,7 +14,10 @@
  *   gpart_graph_aggregate         aggregate() (include/coarsen.hpp:80-85; src/coarsen.cu:937-1659)
  *   gpart_graph_initial_partition initial_partition_gpu (include/initial_partition.hpp:9-12)
  *   gpart_graph_refine            refine_partition (include/refine.hpp:62-66)
- *   gpart_hierarchy_uncoarsen     uncoarsen (include/uncoarsen.hpp:10-13)
+ *   gpart_hierarchy_uncoarsen     refine_hierarchy_device, host hierarchy + host labels
+ *                                 (include/refine.hpp:69-73; src/refine.cu:648-653)
+ *   gpart_hierarchy_uncoarsen_host uncoarsen, host hierarchy + host labels -> host labels
+ *                                 (include/uncoarsen.hpp:10-14; src/uncoarsen.cu:100-106)
  *   gpart_hierarchy_write         write_hierarchy (include/hierarchy.hpp:42-126)
  *   gpart_partition_capacity_c    partition_capacity (include/partition_capacity.hpp:12-23)
  */
@@ -180,8 +183,15 @@ int gpart_hierarchy_map_copy(const gpart_hierarchy* h, int level, int32_t* map);
 const char* gpart_hierarchy_stop_reason(const gpart_hierarchy* h);
 int gpart_hierarchy_write(const gpart_hierarchy* h, const char* path);
 void gpart_hierarchy_free(gpart_hierarchy* h);
+/* Projection + refinement of an (imported) hierarchy from labels of its coarsest
+ * level: levels are uploaded one at a time as they are refined. labels receives the
+ * level-0 partition (n_0 entries); final_cut may be NULL. */
 int gpart_hierarchy_uncoarsen(const gpart_hierarchy* h, const int32_t* coarsest_labels, int parts, double ratio,
                               int max_rounds, double growth, int verify, int32_t* labels, uint64_t* final_cut);
+/* The same through the host uncoarsen overload, which also writes the per-level
+ * and final_* report lines while logging is on. */
+int gpart_hierarchy_uncoarsen_host(const gpart_hierarchy* h, const int32_t* coarsest_labels, int parts,
+                                   double ratio, int max_rounds, double growth, int verify, int32_t* labels);
 
 /* Initial partition of a graph; coords (n doubles) may be NULL. */
 int gpart_graph_initial_partition(gpart_graph* g, int parts, double ratio, uint32_t seed, int spectral_mode,

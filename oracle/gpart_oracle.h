@@ -58,6 +58,14 @@ int gpo_aggregate(const gpo_graph* g, int parts, uint32_t seed, int level,
                   int32_t* map, uint64_t* coarse_weights, int64_t* coarse,
                   uint64_t* capacity, int* rounds);
 
+/* The same level, traced: the clusters (n entries per step, at most max_steps
+ * steps) after every LP round (kind 0), isolated packing (1) and two-hop pairing
+ * (2), with each step's round number and each LP round's mover threshold. */
+int gpo_aggregate_trace(const gpo_graph* g, int parts, uint32_t seed, int level,
+                        int cluster_size_factor, int pack_isolated, int max_steps,
+                        int32_t* clusters, int32_t* kinds, int32_t* rounds,
+                        uint32_t* thresholds, int* steps);
+
 /* Per-vertex SCLP proposal of one LP round (the affinity + decode semantics of
  * src/coarsen.cu:306-598). clusters: n entries. Outputs n entries each. */
 int gpo_sclp_proposals(const gpo_graph* g, const int32_t* clusters, uint32_t seed,

@@ -111,6 +111,23 @@ int gpo_aggregate(const gpo_graph* g, int parts, uint32_t seed, int level,
     });
 }
 
+int gpo_aggregate_trace(const gpo_graph* g, int parts, uint32_t seed, int level, int cluster_size_factor,
+                        int pack_isolated, int max_steps, int32_t* clusters, int32_t* kinds, int32_t* rounds,
+                        uint32_t* thresholds, int* steps) {
+    return guarded([&] {
+        auto r = gpo::aggregate(g->g, parts, seed, level, cluster_size_factor, pack_isolated != 0, true);
+        if (static_cast<int>(r.steps.size()) > max_steps) throw std::invalid_argument("too many aggregate steps");
+        const auto n = g->g.n();
+        for (std::size_t i = 0; i < r.steps.size(); ++i) {
+            std::copy(r.steps[i].clusters.begin(), r.steps[i].clusters.end(), clusters + i * n);
+            kinds[i] = r.steps[i].kind;
+            rounds[i] = r.steps[i].round;
+            thresholds[i] = r.steps[i].threshold;
+        }
+        *steps = static_cast<int>(r.steps.size());
+    });
+}
+
 int gpo_sclp_proposals(const gpo_graph* g, const int32_t* clusters, uint32_t seed, int level,
                        int round, int filter_roles, uint32_t mover_threshold,
                        int32_t* proposals, uint64_t* gains, uint32_t* ties) {

@@ -151,7 +151,7 @@ void sclp_proposals(const Graph& g, const std::vector<std::int32_t>& clusters,
 }
 
 AggregateResult aggregate(const Graph& g, int parts, std::uint32_t seed, int level,
-                          int cluster_size_factor, bool pack_isolated) {
+                          int cluster_size_factor, bool pack_isolated, bool trace) {
     // src/coarsen.cu:1661-1671 (option checks) and :937-1659 (aggregate_typed)
     if (parts < 2 || parts > 32 || cluster_size_factor < 2)
         throw std::invalid_argument("invalid SCLP cluster capacity options");
@@ -215,6 +215,7 @@ AggregateResult aggregate(const Graph& g, int parts, std::uint32_t seed, int lev
         }
         cluster_count = count_nonempty(cluster_weights);
         ++out.rounds;
+        if (trace) out.steps.push_back({0, round, mover_threshold, clusters});
         if (static_cast<double>(cluster_count) <= 0.60 * n) break;
     }
 
@@ -253,6 +254,7 @@ AggregateResult aggregate(const Graph& g, int parts, std::uint32_t seed, int lev
             }
             cluster_count = count_nonempty(cluster_weights);
         }
+        if (trace) out.steps.push_back({1, 0, 0, clusters});
     }
 
     // Two-hop pairing of singletons sharing a favourite (:1443-1539)
@@ -296,6 +298,7 @@ AggregateResult aggregate(const Graph& g, int parts, std::uint32_t seed, int lev
             }
         }
         cluster_count = count_nonempty(cluster_weights);
+        if (trace) out.steps.push_back({2, 4, 0, clusters});
     }
 
     // Compaction (:1541-1577)

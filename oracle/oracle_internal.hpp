@@ -44,6 +44,16 @@ struct AggregateResult {
     std::uint64_t capacity = 0;
     int rounds = 0;
     std::uint64_t proposals = 0;
+    // Filled only when aggregate() is asked to trace: the clusters after every
+    // step (kind 0: LP round, 1: isolated packing, 2: two-hop pairing), with the
+    // round number and the mover threshold of LP rounds.
+    struct Step {
+        int kind = 0;
+        int round = 0;
+        std::uint32_t threshold = 0;
+        std::vector<std::int32_t> clusters;
+    };
+    std::vector<Step> steps;
 };
 
 void sclp_proposals(const Graph& g, const std::vector<std::int32_t>& clusters,
@@ -52,7 +62,7 @@ void sclp_proposals(const Graph& g, const std::vector<std::int32_t>& clusters,
                     std::vector<std::uint64_t>& gains, std::vector<std::uint32_t>& ties);
 
 AggregateResult aggregate(const Graph& g, int parts, std::uint32_t seed, int level,
-                          int cluster_size_factor, bool pack_isolated);
+                          int cluster_size_factor, bool pack_isolated, bool trace = false);
 
 Graph contract(const Graph& fine, const std::vector<std::int32_t>& map,
                std::int64_t coarse, const std::vector<std::uint64_t>& coarse_weights);

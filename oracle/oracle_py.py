@@ -74,6 +74,8 @@ def lib():
     L.gpo_aggregate.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                 _i32p, _u64p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_int)]
+    L.gpo_aggregate_trace.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      _i32p, _i32p, _i32p, _u32p, C.POINTER(C.c_int)]
     L.gpo_sclp_proposals.argtypes = [C.c_void_p, _i32p, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                      C.c_uint32, _i32p, _u64p, _u32p]
     L.gpo_contract.restype = C.c_void_p
@@ -181,6 +183,24 @@ def aggregate(g: Graph, parts: int, seed: int, level: int, cluster_size_factor: 
     _check(lib().gpo_aggregate(g._h, parts, seed, level, cluster_size_factor, int(pack_isolated), mp, cw,
                                C.byref(coarse), C.byref(cap), C.byref(rounds)))
     return mp, cw[: coarse.value].copy(), coarse.value, cap.value, rounds.value
+
+
+def aggregate_trace(g: Graph, parts: int, seed: int, level: int, cluster_size_factor: int = 256,
+                    pack_isolated: bool = False):
+    """One SCLP level step by step: a list of (kind, round, threshold, clusters)
+    after every LP round ("lp"), isolated packing ("isolated") and two-hop
+    pairing ("two_hop")."""
+    n, cap = g.n, 8
+    clusters = np.empty(cap * n, np.int32)
+    kinds = np.empty(cap, np.int32)
+    rounds = np.empty(cap, np.int32)
+    thresholds = np.empty(cap, np.uint32)
+    steps = C.c_int()
+    _check(lib().gpo_aggregate_trace(g._h, parts, seed, level, cluster_size_factor, int(pack_isolated), cap,
+                                     clusters, kinds, rounds, thresholds, C.byref(steps)))
+    names = {0: "lp", 1: "isolated", 2: "two_hop"}
+    return [(names[int(kinds[i])], int(rounds[i]), int(thresholds[i]), clusters[i * n:(i + 1) * n].copy())
+            for i in range(steps.value)]
 
 
 def sclp_proposals(g: Graph, clusters, seed: int, level: int, rnd: int, filter_roles: bool,

@@ -58,6 +58,57 @@ def test_cli_usage_exit_code(cli):
     assert "Usage:" in run.stderr
 
 
+USAGE = {
+    "gpart_partition": " <indptr.bin> <indices.bin> <parts> <partition.out> [imbalance_ratio=1.03] [seed] [--verify]\n",
+    "gpart_coarsen": " <indptr.bin> <indices.bin> <parts> <hierarchy.out> [seed=0] [stop_contraction_ratio=0.90]"
+                     " [max_levels=24] [--verify]\n",
+}
+
+
+@pytest.mark.parametrize("cli,argv", [
+    ("gpart_partition_standard", ["a", "b", "4"]),
+    ("gpart_partition_big", ["a", "b", "4", "o", "1.03", "0", "x"]),
+    ("gpart_coarsen_standard", ["a", "b", "4", "o", "0", "0.9", "24", "x"]),
+    # one trailing --verify is a flag for the coarsen tool; a second one counts as a positional
+    ("gpart_coarsen_big", ["a", "b", "4", "o", "0", "0.9", "24", "--verify", "--verify"]),
+])
+def test_cli_usage_text(cli, argv):
+    """The usage line of app/partition_main.cu:30-32 and app/coarsen_main.cu:19-21,
+    word for word, for too few and too many positionals."""
+    run = subprocess.run([str(BIN / cli), *argv], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 2
+    tool = cli.rsplit("_", 1)[0]
+    assert run.stderr == f"Usage: {BIN / cli}{USAGE[tool]}"
+
+
+@pytest.mark.parametrize("argv,message", [
+    (["4", "--verify", "--verify"], "cannot open"),  # the partition tool strips every trailing --verify
+    (["x"], "stoi"),
+])
+def test_partition_cli_arguments(tmp_path, argv, message):
+    run = subprocess.run([str(BIN / "gpart_partition_standard"), str(tmp_path / "a"), str(tmp_path / "b"),
+                          argv[0], str(tmp_path / "o"), *argv[1:]], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 1
+    assert run.stderr.startswith("gpart_partition: ") and message in run.stderr
+
+
+@pytest.mark.parametrize("extra,message", [
+    (["1"], "parts must be between 2 and 32"),
+    (["33"], "parts must be between 2 and 32"),
+    (["4", "0", "0"], "invalid coarsening stop ratio"),
+    (["4", "0", "1.5"], "invalid coarsening stop ratio"),
+    (["4", "0", "0.9", "0"], "invalid coarsening max levels"),
+    (["4", "0", "0.9", "201"], "invalid coarsening max levels"),
+])
+def test_coarsen_cli_option_checks(tmp_path, extra, message):
+    """app/coarsen_main.cu:30-38: option ranges are checked before the input is
+    read (the input files here do not exist)."""
+    run = subprocess.run([str(BIN / "gpart_coarsen_standard"), str(tmp_path / "a"), str(tmp_path / "b"),
+                          extra[0], str(tmp_path / "o"), *extra[1:]], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 1
+    assert run.stderr == f"gpart_coarsen: {message}\n"
+
+
 def test_cli_missing_input_exit_code(tmp_path):
     """A load failure is reported on stderr with exit 1, before any device work."""
     run = subprocess.run([str(BIN / "gpart_partition_standard"), str(tmp_path / "none.bin"),
