@@ -91,6 +91,10 @@ int guarded(F&& body) {
     }
 }
 
+// Resolves the calling device's copy of gpart_cut_noalloc's scratch array, once
+// per device; the device must be current (refine.hip).
+void cut_scratch_setup(int device);
+
 void* ctx_alloc(gpart_ctx* ctx, std::size_t bytes);
 void ctx_free(gpart_ctx* ctx, void* p);
 

@@ -1242,25 +1242,34 @@ PowerTable make_power_table() {
 template <typename O, typename W, typename C = unsigned long long>
 struct Ops {
     static void cut(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_t* labels, std::uint64_t* out) {
+        const std::int64_t tiles = g->n > 0 && g->m > 0 ? tile_count(g->m) : 0;
         Scratch<unsigned long long> d(ctx, 1);
-        fill_bytes(ctx, d.get(), 0, 8);
+        Scratch<std::int32_t> tile_row(ctx, tiles);
+        Scratch<unsigned long long> partials(ctx, std::min<std::int64_t>(tiles, kReduceMaxBlocks));
+        cut_with(ctx, g, labels, d.get(), tile_row.get(), partials.get(), out);
+    }
+
+    // The cut over caller-provided device scratch: total (1 word), tile_row
+    // (tile_count(m) entries) and partials (min(tiles, kReduceMaxBlocks) words).
+    static void cut_with(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_t* labels,
+                         unsigned long long* total, std::int32_t* tile_row, unsigned long long* partials,
+                         std::uint64_t* out) {
+        fill_bytes(ctx, total, 0, 8);
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
         if (g->n > 0 && g->m > 0) {
             // edge tiles, capped grid, partials folded by one workgroup
             const std::int64_t tiles = tile_count(g->m);
-            Scratch<std::int32_t> tile_row(ctx, tiles);
             hipLaunchKernelGGL(tile_rows_kernel<O>, dim3(grid_for(g->n)), dim3(kBlock), 0, ctx->stream, g->n, off,
-                               tile_row.get());
+                               tile_row);
             const int blocks = static_cast<int>(std::min<std::int64_t>(tiles, kReduceMaxBlocks));
-            Scratch<unsigned long long> partials(ctx, blocks);
             hipLaunchKernelGGL((cut_kernel<O, W>), dim3(blocks), dim3(kTileLanes), 0, ctx->stream, g->n, g->m, tiles,
-                               off, tile_row.get(), g->neighbors, w, labels, partials.get());
+                               off, tile_row, g->neighbors, w, labels, partials);
             hipLaunchKernelGGL((reduce_finish_kernel<unsigned long long, SumU64>), dim3(1), dim3(kReduceThreads), 0,
-                               ctx->stream, partials.get(), blocks, SumU64{}, d.get());
+                               ctx->stream, partials, blocks, SumU64{}, total);
             GPART_LAUNCH_CHECK();
         }
-        const auto directed = read_scalar(ctx, d.get());
+        const auto directed = read_scalar(ctx, total);
         if (directed & 1ULL) throw Error(GPART_ERR_RUNTIME, "refinement cut is not symmetric");
         *out = directed / 2;
     }
@@ -1522,6 +1531,43 @@ struct Ops {
 }  // namespace
 }  // namespace gpart_hip
 
+// Device scratch of gpart_cut_noalloc, one copy per GPU for the life of the
+// process: the directed total, the per-workgroup partials and the tile rows.
+// Host code never passes this name to a kernel; it passes the device address
+// that cut_scratch_setup looks up for each device.
+namespace gpart_hip {
+namespace {
+constexpr std::int64_t kCutScratchTiles = std::int64_t{1} << 20;  // CSRs up to 2^20 * 4096 entries
+constexpr std::size_t kCutScratchTotalOffset = 0;
+constexpr std::size_t kCutScratchPartialsOffset = 256;
+constexpr std::size_t kCutScratchTilesOffset = kCutScratchPartialsOffset + 8 * kReduceMaxBlocks;
+constexpr std::size_t kCutScratchBytes = kCutScratchTilesOffset + 4 * kCutScratchTiles;  // 4 MiB + 8.25 KiB
+constexpr int kCutScratchDevices = 64;
+
+std::mutex& cut_scratch_mutex() {
+    static std::mutex mutex;
+    return mutex;
+}
+void* cut_scratch_address[kCutScratchDevices] = {};
+}  // namespace
+}  // namespace gpart_hip
+
+static __device__ __align__(256) unsigned char scratch[gpart_hip::kCutScratchBytes];
+
+namespace gpart_hip {
+// Called once per context with its device current (gpart_ctx_create); the first
+// call on a device resolves that device's copy of the array.
+void cut_scratch_setup(int device) {
+    if (device < 0 || device >= kCutScratchDevices)
+        throw Error(GPART_ERR_RUNTIME, "device ordinal beyond the no-allocation scratch table");
+    std::lock_guard<std::mutex> lock(cut_scratch_mutex());
+    if (cut_scratch_address[device]) return;
+    void* address = nullptr;
+    GPART_HIP_CHECK(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)));
+    cut_scratch_address[device] = address;
+}
+}  // namespace gpart_hip
+
 using namespace gpart_hip;
 
 #define GPART_WIDTH_DISPATCH(width, call)                            \
@@ -1571,6 +1617,29 @@ int gpart_refine_set_rebuild(gpart_ctx* ctx, int64_t min_entries, int64_t share)
 int gpart_cut(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, uint64_t* cut) {
     return guarded([&] { GPART_WIDTH_DISPATCH(g->width, OpsT::cut(ctx, g, labels, cut)); });
 }
+
+int gpart_cut_noalloc(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, uint64_t* cut) {
+    return guarded([&] {
+        if (!ctx || !g || !cut) throw Error(GPART_ERR_INVALID_ARGUMENT, "null context, graph or output");
+        if (g->m > gpart_cut_noalloc_max_entries())
+            throw Error(GPART_ERR_INVALID_ARGUMENT,
+                        "gpart_cut_noalloc: " + std::to_string(g->m) + " entries exceed its limit of " +
+                            std::to_string(gpart_cut_noalloc_max_entries()));
+        void* base = nullptr;
+        if (ctx->device >= 0 && ctx->device < kCutScratchDevices) {
+            std::lock_guard<std::mutex> lock(cut_scratch_mutex());
+            base = cut_scratch_address[ctx->device];
+        }
+        if (!base) throw Error(GPART_ERR_RUNTIME, "gpart_cut_noalloc: scratch of this device was not set up");
+        auto* bytes = static_cast<unsigned char*>(base);
+        auto* total = reinterpret_cast<unsigned long long*>(bytes + kCutScratchTotalOffset);
+        auto* partials = reinterpret_cast<unsigned long long*>(bytes + kCutScratchPartialsOffset);
+        auto* tile_row = reinterpret_cast<std::int32_t*>(bytes + kCutScratchTilesOffset);
+        GPART_WIDTH_DISPATCH(g->width, OpsT::cut_with(ctx, g, labels, total, tile_row, partials, cut));
+    });
+}
+
+int64_t gpart_cut_noalloc_max_entries(void) { return kCutScratchTiles * kTileEdges; }
 
 int gpart_part_weights(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, int k,
                        uint64_t* part_weights) {

@@ -248,6 +248,7 @@ int gpart_ctx_create(int device, gpart_ctx** out) {
             GPART_HIP_CHECK(hipSetDevice(device));
             // Default priority: no stream priorities anywhere in this library.
             GPART_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+            cut_scratch_setup(device);
             GPART_HIP_CHECK(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
             GPART_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ctx->status_host), kStatusWords * 8,
                                           hipHostMallocCoherent | hipHostMallocMapped));

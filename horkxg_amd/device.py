@@ -23,6 +23,9 @@ def _bind():
     H.gpart_exclusive_scan_u64.argtypes = [vp, vp, vp, i64]
     H.gpart_compact_flags.argtypes = [vp, vp, i64, vp, vp]
     H.gpart_contract_record_bounds.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+    H.gpart_cut.argtypes = [vp, vp, vp, vp]
+    H.gpart_cut_noalloc.argtypes = [vp, vp, vp, vp]
+    H.gpart_cut_noalloc_max_entries.restype = i64
 
 
 _bind()
@@ -169,3 +172,16 @@ def compact(flags):
     cnt = C.c_int64()
     _check(hip_lib().gpart_compact_flags(_ctx(), C.c_void_p(f.ptr), f.count, C.c_void_p(out.ptr), C.byref(cnt)))
     return out.get()[: cnt.value]
+
+
+def cut(g: DeviceGraph, labels: DeviceArray, noalloc: bool = False) -> int:
+    """Undirected cut of device labels: gpart_cut, or gpart_cut_noalloc (no
+    allocation; scratch in a static device array)."""
+    out = C.c_uint64()
+    fn = hip_lib().gpart_cut_noalloc if noalloc else hip_lib().gpart_cut
+    _check(fn(_ctx(), C.byref(g.view), C.c_void_p(labels.ptr), C.byref(out)))
+    return out.value
+
+
+def cut_noalloc_max_entries() -> int:
+    return int(hip_lib().gpart_cut_noalloc_max_entries())

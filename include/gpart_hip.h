@@ -358,6 +358,17 @@ int64_t gpart_spectral_depth_max_task(void);
 /* refine_cut_kernel + device_cut_value (src/refine.cu:52-87,117-142).
  * host out: *cut (undirected; odd directed total is an error). */
 int gpart_cut(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, uint64_t* cut);
+/* gpart_cut without any allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid allocation). The same kernels as gpart_cut,
+ * with their scratch (about 4 MiB: total, workgroup partials, tile rows) in one
+ * static device array per GPU, held until the process ends and resolved when a
+ * context is created on that GPU. The CSR may hold at most
+ * gpart_cut_noalloc_max_entries() entries (2^32); larger graphs are refused with
+ * GPART_ERR_INVALID_ARGUMENT. Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (two contexts or threads at once) race. host out: *cut. */
+int gpart_cut_noalloc(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, uint64_t* cut);
+int64_t gpart_cut_noalloc_max_entries(void);
 /* accumulate_part_weights_kernel (src/refine.cu:102-115); part_weights (device,
  * k entries) is overwritten. */
 int gpart_part_weights(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, int k,
