@@ -36,6 +36,8 @@ from ._native import (  # noqa: F401
     cache_stats,
     set_log,
     set_refine_rebuild,
+    set_refine_engine,
+    refine_keep_ratio,
     trim,
 )
 from . import device  # noqa: F401

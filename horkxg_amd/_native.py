@@ -131,6 +131,8 @@ def hip_lib():
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         H.gpart_refine_set_rebuild.argtypes = [C.c_void_p, C.c_int64, C.c_int64]
         H.gpart_cache_info.argtypes = [C.c_void_p, C.c_void_p]
+        H.gpart_refine_set_engine.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int64]
+        H.gpart_refine_keep_ratio.argtypes = [C.c_uint64, C.c_uint64]
         _hip = H
     return _hip
 
@@ -279,6 +281,23 @@ def set_refine_rebuild(min_entries: int = 1 << 23, share: int = 16) -> None:
     entries rebuild the affinity cache when their movers hold more than
     1/share of the entries (share <= 0: never). Results do not depend on it."""
     _check(hip_lib().gpart_refine_set_rebuild(_ctx(), int(min_entries), int(share)))
+
+
+def set_refine_engine(max_vertices: int = 0, vertices_per_group: int = 512, max_grid: int = 128,
+                      max_degree: int = 4096) -> None:
+    """The calling thread's single-launch refinement engine: levels of at most
+    max_vertices vertices whose rows have at most max_degree entries are refined
+    in one cooperative launch of ceil(n / vertices_per_group) workgroups, at most
+    max_grid (<= 128); max_vertices = 0 (the default) switches it off. Results do
+    not depend on it."""
+    _check(hip_lib().gpart_refine_set_engine(_ctx(), int(max_vertices), int(vertices_per_group), int(max_grid),
+                                             int(max_degree)))
+
+
+def refine_keep_ratio(current: int, best: int) -> bool:
+    """The stale-round test (long double)current < 0.999L * best, as the engine
+    evaluates it (host evaluation of the same code)."""
+    return bool(hip_lib().gpart_refine_keep_ratio(C.c_uint64(current), C.c_uint64(best)))
 
 
 def set_log(enabled: bool) -> None:

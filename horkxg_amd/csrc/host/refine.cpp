@@ -105,6 +105,34 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
 
     RefineStats stats;
     stats.initial_cut = level.cut();
+    const double filter = maximum_edge <= 1 ? 0.25 : 0.75;
+    auto finish = [&](std::uint64_t best_cut) {
+        level.host_part_weights = level.part_weights.to_host();
+        stats.final_cut = level.cut();
+        stats.final_max_part_weight = level.maximum();
+        if (stats.final_cut != best_cut || stats.final_cut > stats.initial_cut || level.maximum() > level.capacity)
+            throw std::runtime_error("incremental refinement postcondition failed");
+        stats.seconds = since(start);
+        return stats;
+    };
+    // Levels within the engine's tuning run the whole loop below in one launch.
+    int grid = 0;
+    if (!options.strict_verify)
+        GPART_CHECK(gpart_refine_engine_grid(ctx, &view, &classes, cache_bytes, &grid));
+    if (grid > 0) {
+        gpart_refine_level_stats e{};
+        GPART_CHECK(gpart_refine_level(ctx, &view, &classes, cache.data(), cache_bytes, level.partition.data(), k,
+                                       level.part_weights.data(), level.capacity, filter, rounds,
+                                       options.stale_rounds, stats.initial_cut, targets.data(), gains.data(),
+                                       flags.data(), locks.data(), grid, &e));
+        stats.rounds = e.rounds;
+        stats.proposals = e.proposals;
+        stats.accepted = e.accepted;
+        stats.admission_iterations = e.admission_iterations;
+        stats.admission_fallbacks = e.admission_fallbacks;
+        stats.mover_entries = e.mover_entries;
+        return finish(e.best_cut);
+    }
     auto current_cut = stats.initial_cut, best_cut = current_cut;
     DeviceVector<std::int32_t> best_partition(level.partition);
     DeviceVector<std::uint64_t> best_part_weights(level.part_weights);
@@ -149,7 +177,6 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
         }
         return level.maximum() <= level.capacity;
     };
-    const double filter = maximum_edge <= 1 ? 0.25 : 0.75;
     int stale = 0, empty = 0;
     for (int phase = 0; phase < rounds && stale < options.stale_rounds; ++phase) {
         stats.rounds = phase + 1;
@@ -179,13 +206,7 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     }
     GPART_CHECK(gpart_memcpy_dtod(ctx, level.partition.data(), best_partition.data(), N * sizeof(std::int32_t)));
     GPART_CHECK(gpart_memcpy_dtod(ctx, level.part_weights.data(), best_part_weights.data(), k * sizeof(std::uint64_t)));
-    level.host_part_weights = level.part_weights.to_host();
-    stats.final_cut = level.cut();
-    stats.final_max_part_weight = level.maximum();
-    if (stats.final_cut != best_cut || stats.final_cut > stats.initial_cut || level.maximum() > level.capacity)
-        throw std::runtime_error("incremental refinement postcondition failed");
-    stats.seconds = since(start);
-    return stats;
+    return finish(best_cut);
 }
 
 template <typename Types>

@@ -417,6 +417,49 @@ int gpart_refine_rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g,
                                  const gpart_degree_classes* classes, void* cache, int cache_bytes,
                                  int32_t* labels, int k, uint64_t* part_weights, uint64_t capacity,
                                  int32_t* targets, int32_t* flags, gpart_refine_status* status);
+/* Outcome of gpart_refine_level. */
+typedef struct gpart_refine_level_stats {
+    int32_t rounds;               /* LP rounds run */
+    int32_t admission_iterations; /* rebalance iterations run */
+    int32_t admission_fallbacks;  /* 1: a rebalance could not restore the capacity */
+    int32_t error;                /* 0 (errors are returned as status codes) */
+    uint64_t proposals;           /* movers of LP rounds */
+    uint64_t accepted;            /* movers of applied rounds and iterations */
+    uint64_t mover_entries;       /* CSR entries of those movers */
+    uint64_t best_cut;            /* cut of the labels left behind */
+    uint64_t steps;               /* LP rounds and rebalance iterations with a stage pass */
+} gpart_refine_level_stats;
+
+/* The whole of run_incremental_refinement (src/refine_incremental.cuh:235-361) for
+ * one level in a single launch of `grid` resident workgroups: LP rounds
+ * (:338-352), their rebalance loops (:303-331), the stale-round rule, checkpoints
+ * and the restore of the best labels (:353-354). Inputs as for the per-round entry
+ * points: the cache built from labels, part_weights (device, k) matching labels,
+ * initial_cut the cut of labels; filter 0.25 or 0.75 (:332-333). On return labels
+ * and part_weights hold the best partition and stats->best_cut its cut. grid comes
+ * from gpart_refine_engine_grid. The labels equal those of the per-round entry
+ * points driven by the reference's loop. host out: *stats. */
+int gpart_refine_level(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes, void* cache,
+                       int cache_bytes, int32_t* labels, int k, uint64_t* part_weights, uint64_t capacity,
+                       double filter, int rounds, int stale_rounds, uint64_t initial_cut, int32_t* targets,
+                       int64_t* gains, int32_t* flags, int32_t* locks, int grid, gpart_refine_level_stats* stats);
+/* The grid gpart_refine_level would use for this level, 0 when the level takes the
+ * per-round path (more vertices or a longer row than the context's engine tuning
+ * allows). host out: *grid. */
+int gpart_refine_engine_grid(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* classes,
+                             int cache_bytes, int* grid);
+/* Engine tuning of a context (no counterpart in the reference): levels of at most
+ * max_vertices vertices and rows of at most max_degree entries take the
+ * single-launch engine with ceil(n / vertices_per_group) workgroups, at most
+ * max_grid (<= 128). max_vertices = 0 switches the engine off. Defaults 0 (off),
+ * 512, 128, 4096: measured slower than the per-round path on every level of the
+ * 128^3 mesh. Results do not depend on it. */
+int gpart_refine_set_engine(gpart_ctx* ctx, int64_t max_vertices, int64_t vertices_per_group, int max_grid,
+                            int64_t max_degree);
+/* The stale-round test of the reference's loop, (long double)current < 0.999L *
+ * best (src/refine_incremental.cuh:347-350), as the engine evaluates it on the
+ * device; on the host. best < 2^63. */
+int gpart_refine_keep_ratio(uint64_t current, uint64_t best);
 /* Largest edge weight (src/refine_incremental.cuh:332). host out. */
 int gpart_max_edge_weight(gpart_ctx* ctx, const gpart_csr_view* g, uint64_t* maximum);
 /* Edge weight sums. host out: *checked saturates at UINT64_MAX (CheckedSum,
