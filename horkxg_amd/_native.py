@@ -24,6 +24,8 @@ PROFILE_FAMILIES = {
     "refine_filter": 4,
     "spectral": 5,
     "rebalance": 6,
+    "cache_update": 7,
+    "sclp_commit": 8,
 }
 
 
@@ -122,6 +124,7 @@ def hip_lib():
         H.gpart_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                          C.POINTER(C.c_double)]
         H.gpart_profile_reset.argtypes = [C.c_void_p]
+        H.gpart_profile_read_units.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         H.gpart_device_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
         H.gpart_device_free.argtypes = [C.c_void_p, C.c_void_p]
         H.gpart_memcpy_htod.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
@@ -324,7 +327,9 @@ def profile_read(family: str) -> dict:
     ms, launches, nbytes = C.c_double(), C.c_int64(), C.c_double()
     _check(hip_lib().gpart_profile_read(_ctx(), PROFILE_FAMILIES[family], C.byref(ms), C.byref(launches),
                                         C.byref(nbytes)))
-    return {"total_ms": ms.value, "launches": launches.value, "bytes": nbytes.value}
+    units = C.c_double()
+    _check(hip_lib().gpart_profile_read_units(_ctx(), PROFILE_FAMILIES[family], C.byref(units)))
+    return {"total_ms": ms.value, "launches": launches.value, "bytes": nbytes.value, "atomics": units.value}
 
 
 def profile_reset() -> None:

@@ -1472,13 +1472,20 @@ int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const 
             GPART_LAUNCH_CHECK();
             Scratch<unsigned long long> acc(ctx, 2);
             fill_bytes(ctx, acc.get(), 0, 16);
+            profile_begin(ctx, GPART_PROFILE_SCLP_COMMIT);
             hipLaunchKernelGGL(admit_apply_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
                                ordered_targets.get(), ordered_weights.get(), accept.get(),
                                reinterpret_cast<const unsigned long long*>(gains), clusters, cw, acc.get());
             GPART_LAUNCH_CHECK();
+            profile_end(ctx, GPART_PROFILE_SCLP_COMMIT, 0.0);
             unsigned long long h[2];
             read_back(ctx, h, acc.get(), sizeof(h));
             *accepted = static_cast<std::int64_t>(h[0]);
+            // per proposal its order, target, weight and decision; per accepted move
+            // its gain, cluster read and write and two cluster-weight atomics
+            profile_add_bytes(ctx, GPART_PROFILE_SCLP_COMMIT,
+                              20.0 * static_cast<double>(p) + 48.0 * static_cast<double>(h[0]));
+            profile_add_units(ctx, GPART_PROFILE_SCLP_COMMIT, 2.0 * static_cast<double>(h[0]));
             if (predicted_gain) *predicted_gain = h[1];
         }
         *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));

@@ -24,6 +24,7 @@
 #include "scan.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 
 namespace gpart_hip {
@@ -115,13 +116,23 @@ __device__ Norm finish_block(const Moments* partials, int count, double* shared8
     return *shared_norm;
 }
 
+// Algorithmic HBM bytes of one diffusion step over tasks of `vertices` vertices
+// whose rows hold `entries` CSR entries: their CSR rows, a label and an x gather
+// per entry, and per vertex its id, degree, x and y (spectral_step, :109-136).
+inline double step_bytes(double vertices, int tasks, double entries, std::size_t offset_bytes,
+                         std::size_t weight_bytes, std::size_t value_bytes) {
+    return (vertices + tasks) * static_cast<double>(offset_bytes) +
+           entries * static_cast<double>(4 + weight_bytes) + entries * static_cast<double>(4 + value_bytes) +
+           vertices * static_cast<double>(4 + 8 + 2 * value_bytes);
+}
+
 // ------------------------------------------------------------------ plain form
 
 template <typename O, typename W>
 __global__ void __launch_bounds__(256) spectral_start_kernel(int count, const int* ids, const int* labels, int group,
                                                              const O* offsets, const int* neighbors, const W* weights,
                                                              double* degree, double* x, unsigned seed,
-                                                             Moments* partials) {
+                                                             Moments* partials, unsigned long long* entries) {
     __shared__ Moments vm[8];
     const int lane = threadIdx.x & 31, hw = threadIdx.x >> 5;
     const int i = blockIdx.x * 8 + hw;
@@ -136,6 +147,8 @@ __global__ void __launch_bounds__(256) spectral_start_kernel(int count, const in
         Moments m{0, 0, 0};
         if (i < count) {
             const int v = ids[i];
+            // CSR entries each diffusion step scans (for the family's algorithmic bytes)
+            atomicAdd(entries, static_cast<unsigned long long>(offsets[v + 1] - offsets[v]));
             const double a = static_cast<double>(mix32(static_cast<unsigned>(v) ^ seed)) / 4294967296.0 - 0.5;
             degree[v] = d;
             x[v] = a;
@@ -294,7 +307,8 @@ __device__ __forceinline__ T normalized(const T* x, int v, const Norm& norm) {
 template <typename O, typename W, typename T>
 __global__ void __launch_bounds__(256) depth_start_kernel(DepthTasks dt, const int* order, const int* labels,
                                                           const O* offsets, const int* neighbors, const W* weights,
-                                                          unsigned seed, double* degree, T* x, Moments* partials) {
+                                                          unsigned seed, double* degree, T* x, Moments* partials,
+                                                          unsigned long long* entries) {
     __shared__ Moments vm[8];
     const int t = task_of_block(dt, blockIdx.x);
     const gpart_bisect_task task = dt.task[t];
@@ -314,6 +328,7 @@ __global__ void __launch_bounds__(256) depth_start_kernel(DepthTasks dt, const i
         Moments m{0, 0, 0};
         if (i < task.count) {
             const int v = ids[i];
+            atomicAdd(entries, static_cast<unsigned long long>(offsets[v + 1] - offsets[v]));
             const double a = static_cast<double>(mix32(static_cast<unsigned>(v) ^ tseed)) / 4294967296.0 - 0.5;
             degree[v] = d;
             x[v] = static_cast<T>(a);
@@ -524,10 +539,12 @@ void plain_task(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, 
     const int spectral_blocks = (count + 7) / 8;
     Scratch<Moments> partials(ctx, spectral_blocks);
     Scratch<Norm> norm(ctx, 1);
+    Scratch<unsigned long long> entries(ctx, 1);
+    fill_bytes(ctx, entries.get(), 0, 8);
     profile_begin(ctx, GPART_PROFILE_SPECTRAL);
     hipLaunchKernelGGL((spectral_start_kernel<O, W>), dim3(spectral_blocks), dim3(256), 0, ctx->stream, count, ids,
                        labels, task->first_part, off, g->neighbors, w, degree, x,
-                       seed ^ static_cast<unsigned>(task->first_part), partials.get());
+                       seed ^ static_cast<unsigned>(task->first_part), partials.get(), entries.get());
     hipLaunchKernelGGL(finish_moments_kernel, dim3(1), dim3(256), 0, ctx->stream, spectral_blocks, partials.get(),
                        norm.get());
     GPART_LAUNCH_CHECK();
@@ -542,6 +559,10 @@ void plain_task(gpart_ctx* ctx, const gpart_csr_view* g, const int32_t* labels, 
         std::swap(x, y);
     }
     profile_end(ctx, GPART_PROFILE_SPECTRAL, 0.0);
+    if (ctx->profile[GPART_PROFILE_SPECTRAL].enabled)
+        profile_add_bytes(ctx, GPART_PROFILE_SPECTRAL,
+                          64.0 * step_bytes(count, 1, static_cast<double>(read_scalar(ctx, entries.get())), sizeof(O),
+                                            sizeof(W), sizeof(double)));
     // (x, id) total order: stable sort by id, then stable sort by x
     Scratch<std::uint64_t> keys(ctx, count), keys_alt(ctx, count), vals(ctx, count), vals_alt(ctx, count);
     hipLaunchKernelGGL(sort_keys_kernel, dim3(blocks), dim3(256), 0, ctx->stream, count, ids, x, keys.get(), vals.get(),
@@ -620,8 +641,13 @@ int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* label
         const std::int64_t n = g->n;
         Scratch<double> degree(ctx, n), xa(ctx, n), xb(ctx, n);
         Scratch<Moments> pa(ctx, blocks), pb(ctx, blocks);
-        Scratch<int> counters(ctx, 2);
-        fill_bytes(ctx, counters.get(), 0, 8);
+        // next_count and status (two ints), then the scanned CSR entries
+        Scratch<unsigned long long> counters(ctx, 2);
+        fill_bytes(ctx, counters.get(), 0, 16);
+        int* next_and_status = reinterpret_cast<int*>(counters.get());
+        unsigned long long* entries = counters.get() + 1;
+        int vertices = 0;
+        for (int t = 0; t < task_count; ++t) vertices += tasks[t].count;
         profile_begin(ctx, GPART_PROFILE_SPECTRAL);
         auto launch = [&](auto o_tag, auto w_tag, auto t_tag) {
             using O = decltype(o_tag);
@@ -634,7 +660,7 @@ int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* label
             Moments* p = pa.get();
             Moments* q = pb.get();
             hipLaunchKernelGGL((depth_start_kernel<O, W, T>), dim3(blocks), dim3(256), 0, ctx->stream, dt, order,
-                               labels, off, g->neighbors, w, seed, degree.get(), x, p);
+                               labels, off, g->neighbors, w, seed, degree.get(), x, p, entries);
             for (int step = 0; step < 64; ++step) {
                 hipLaunchKernelGGL((depth_step_kernel<O, W, T>), dim3(blocks), dim3(256), 0, ctx->stream, dt, order,
                                    labels, off, g->neighbors, w, degree.get(), x, y, p, q);
@@ -642,8 +668,8 @@ int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* label
                 std::swap(p, q);
             }
             hipLaunchKernelGGL((depth_finish_kernel<T>), dim3(task_count), dim3(kBatchThreads), 0, ctx->stream, dt,
-                               labels, order, vertex_weights, capacity, x, p, coords, next, counters.get(),
-                               counters.get() + 1);
+                               labels, order, vertex_weights, capacity, x, p, coords, next, next_and_status,
+                               next_and_status + 1);
             GPART_LAUNCH_CHECK();
         };
         const bool fp32 = precision == GPART_SPECTRAL_FP32;
@@ -655,10 +681,16 @@ int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* label
             else launch(std::int64_t{}, std::uint64_t{}, double{});
         }
         profile_end(ctx, GPART_PROFILE_SPECTRAL, 0.0);
-        int h[2];
+        unsigned long long h[2];
         read_back(ctx, h, counters.get(), sizeof(h));
-        *next_count = h[0];
-        *status = h[1];
+        int ns[2];
+        std::memcpy(ns, h, sizeof(ns));
+        *next_count = ns[0];
+        *status = ns[1];
+        const std::size_t o_size = g->width == GPART_WIDTH_STANDARD ? 4 : 8;
+        profile_add_bytes(ctx, GPART_PROFILE_SPECTRAL,
+                          64.0 * step_bytes(vertices, task_count, static_cast<double>(h[1]), o_size, o_size,
+                                            fp32 ? sizeof(float) : sizeof(double)));
     });
 }
 

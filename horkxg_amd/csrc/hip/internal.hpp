@@ -38,6 +38,7 @@ struct gpart_ctx {
         bool enabled = false;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
         double bytes = 0.0;
+        double units = 0.0;  // atomic operations, for the atomic-bound families
         std::int64_t launches = 0;
     } profile[GPART_PROFILE_FAMILIES];
     std::vector<hipEvent_t> spare_events;
@@ -151,6 +152,8 @@ void profile_begin(gpart_ctx* ctx, int family);
 void profile_end(gpart_ctx* ctx, int family, double algorithmic_bytes);
 // Adds bytes to the family's last launch once its size is known on the host.
 void profile_add_bytes(gpart_ctx* ctx, int family, double algorithmic_bytes);
+// Adds atomic operations to the family's count.
+void profile_add_units(gpart_ctx* ctx, int family, double units);
 
 // ---- primitives (primitives.hip) ----
 // Stable LSD radix sort over key bits [begin_bit, end_bit); alternate buffers

@@ -1142,6 +1142,7 @@ struct Ops {
         const auto* w = static_cast<const W*>(g->edge_weights);
         const long long rebuild_above = rebuild_threshold(ctx, g);
         const auto light_blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * 8), 4 * kRoundBlocks));
+        profile_begin(ctx, GPART_PROFILE_CACHE_UPDATE);
         hipLaunchKernelGGL((update_cache_kernel<O, W, 8, C>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
                            &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta,
                            &st->edges, rebuild_above);
@@ -1161,6 +1162,7 @@ struct Ops {
                                st, heavy_start.get(), tile_mover.get(), heavy, off, g->neighbors, w, labels, targets,
                                flags, k, cache, &st->delta, rebuild_above);
         }
+        profile_end(ctx, GPART_PROFILE_CACHE_UPDATE, 0.0);
         hipLaunchKernelGGL(commit_kernel<W>,
                            dim3(static_cast<unsigned>(std::min<std::int64_t>(grid_for(n), kRoundBlocks))),
                            dim3(kBlock), 0, ctx->stream, st, light, heavy, labels, targets,
@@ -1193,6 +1195,14 @@ struct Ops {
         const long long rebuild_above = rebuild_threshold(ctx, g);
         out->rebuilt = rebuild_above >= 0 && h.edges > rebuild_above ? 1 : 0;
         out->cut_after = 0;
+        if (!out->rebuilt) {
+            // the movers' rows and ids, per entry a neighbour, a weight, two cache
+            // atomics (read and write each) and the label, target and flag gathers
+            const double movers = static_cast<double>(out->movers), entries = static_cast<double>(h.edges);
+            profile_add_bytes(ctx, GPART_PROFILE_CACHE_UPDATE,
+                              movers * (2.0 * sizeof(O) + 16.0) + entries * (4.0 + sizeof(W) + 4.0 * sizeof(C) + 12.0));
+            profile_add_units(ctx, GPART_PROFILE_CACHE_UPDATE, 2.0 * entries);
+        }
         if (out->rebuilt) {
             if (h.cut_directed & 1ULL) throw Error(GPART_ERR_RUNTIME, "refinement cut is not symmetric");
             out->cut_after = h.cut_directed / 2;
@@ -1304,6 +1314,8 @@ struct Ops {
         // per candidate: id, label, weight, cache row, target and bucket writes
         profile_add_bytes(ctx, GPART_PROFILE_REBALANCE,
                           static_cast<double>(out->candidates) * (16.0 + sizeof(W) + sizeof(C) * k));
+        // one bucket-histogram add per candidate (balance_source_histogram, :173-186)
+        profile_add_units(ctx, GPART_PROFILE_REBALANCE, static_cast<double>(out->candidates));
     }
 };
 

@@ -226,6 +226,11 @@ void profile_add_bytes(gpart_ctx* ctx, int family, double algorithmic_bytes) {
     if (p.enabled) p.bytes += algorithmic_bytes;
 }
 
+void profile_add_units(gpart_ctx* ctx, int family, double units) {
+    auto& p = ctx->profile[family];
+    if (p.enabled) p.units += units;
+}
+
 }  // namespace gpart_hip
 
 using namespace gpart_hip;
@@ -442,6 +447,14 @@ int gpart_profile_read(gpart_ctx* ctx, int family, double* total_ms, int64_t* la
     });
 }
 
+int gpart_profile_read_units(gpart_ctx* ctx, int family, double* units) {
+    return guarded([&] {
+        if (family < 0 || family >= GPART_PROFILE_FAMILIES)
+            throw Error(GPART_ERR_INVALID_ARGUMENT, "unknown profile family");
+        *units = ctx->profile[family].units;
+    });
+}
+
 int gpart_profile_reset(gpart_ctx* ctx) {
     return guarded([&] {
         GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -452,6 +465,7 @@ int gpart_profile_reset(gpart_ctx* ctx) {
             }
             p.events.clear();
             p.bytes = 0.0;
+            p.units = 0.0;
             p.launches = 0;
         }
     });

@@ -110,12 +110,17 @@ int gpart_widen_weights(gpart_ctx* ctx, const void* weights, int width, int64_t 
 #define GPART_PROFILE_REFINE_PROPOSALS 3
 #define GPART_PROFILE_REFINE_FILTER 4
 #define GPART_PROFILE_SPECTRAL 5
-#define GPART_PROFILE_REBALANCE 6
-#define GPART_PROFILE_FAMILIES 7
+#define GPART_PROFILE_REBALANCE 6      /* balance_proposals_kernel; units: bucket histogram adds (R17) */
+#define GPART_PROFILE_CACHE_UPDATE 7   /* movers' cache updates (R13); units: global atomics */
+#define GPART_PROFILE_SCLP_COMMIT 8    /* admission commit (K18); units: global atomics */
+#define GPART_PROFILE_FAMILIES 9
 int gpart_profile_enable(gpart_ctx* ctx, int family, int enable);
 /* host out: total milliseconds, launch count, algorithmic bytes */
 int gpart_profile_read(gpart_ctx* ctx, int family, double* total_ms, int64_t* launches,
                        double* bytes);
+/* host out: the family's atomic operations over the same launches (0 for families
+ * that are not atomic-bound). */
+int gpart_profile_read_units(gpart_ctx* ctx, int family, double* units);
 int gpart_profile_reset(gpart_ctx* ctx);
 
 /* ---------------- primitives (P1-P13 of the reference's CUB/Thrust calls) ---------------- */
