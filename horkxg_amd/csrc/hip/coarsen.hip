@@ -962,31 +962,55 @@ struct GatherWeightLoad {
 };
 
 // isolated_representative_kernel / isolated_commit_kernel (src/coarsen.cu:741-773)
+// The representative of a bin is the least id among the vertices lying wholly in
+// it (the reference's atomicMin). The list is in id order and the prefix does not
+// decrease along it (after any leading zero-weight vertices, which lie in no bin
+// wholly), so a bin's whole vertices form one run of the list and the least id is
+// the run's first: it alone writes, without atomics (a run of thousands of
+// vertices on one address serialised the atomics).
+__device__ __forceinline__ bool whole_bin(std::uint64_t prefix, std::uint64_t weight, std::uint64_t capacity,
+                                          std::uint64_t* bin) {
+    const std::uint64_t first = (prefix - weight) / capacity;
+    const std::uint64_t last = (prefix - 1) / capacity;
+    *bin = first;
+    return first == last;
+}
+
 __global__ void isolated_rep_kernel(std::int64_t count, const std::int32_t* vertices, const std::uint64_t* prefix,
-                                    const std::uint64_t* weights, std::uint64_t capacity, std::int32_t* reps) {
+                                    const std::uint64_t* weights, std::uint64_t capacity, std::int32_t* reps,
+                                    std::uint64_t* run_base) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const std::int32_t v = vertices[i];
-    const std::uint64_t first = (prefix[i] - weights[v]) / capacity;
-    const std::uint64_t last = (prefix[i] - 1) / capacity;
-    if (first == last) atomicMin(reps + first, v);
+    std::uint64_t bin;
+    if (!whole_bin(prefix[i], weights[v], capacity, &bin)) return;
+    if (i > 0) {
+        std::uint64_t before;
+        if (whole_bin(prefix[i - 1], weights[vertices[i - 1]], capacity, &before) && before == bin) return;
+    }
+    reps[bin] = v;
+    run_base[bin] = prefix[i] - weights[v];  // the weight before the run
 }
 
 __global__ void isolated_commit_kernel(std::int64_t count, const std::int32_t* vertices, const std::uint64_t* prefix,
                                        const std::uint64_t* weights, std::uint64_t capacity,
-                                       const std::int32_t* reps, std::int32_t* clusters,
-                                       unsigned long long* cluster_weights) {
+                                       const std::int32_t* reps, const std::uint64_t* run_base,
+                                       std::int32_t* clusters, unsigned long long* cluster_weights) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const std::int32_t v = vertices[i];
-    const std::uint64_t first = (prefix[i] - weights[v]) / capacity;
-    const std::uint64_t last = (prefix[i] - 1) / capacity;
-    if (first != last) return;
-    const std::int32_t target = reps[first];
-    if (target == v) return;
-    clusters[v] = target;
-    cluster_weights[v] = 0;
-    atomicAdd(cluster_weights + target, static_cast<unsigned long long>(weights[v]));
+    std::uint64_t bin;
+    if (!whole_bin(prefix[i], weights[v], capacity, &bin)) return;
+    const std::int32_t target = reps[bin];
+    if (target != v) {
+        clusters[v] = target;
+        cluster_weights[v] = 0;
+    }
+    // the run's last vertex sets the representative's weight: the run's total
+    // (the representative was a singleton, weighing its own weight)
+    std::uint64_t after = ~0ULL;
+    if (i + 1 < count && whole_bin(prefix[i + 1], weights[vertices[i + 1]], capacity, &after) && after == bin) return;
+    cluster_weights[target] = static_cast<unsigned long long>(prefix[i] - run_base[bin]);
 }
 
 __global__ void fill_i32_kernel(std::int32_t* out, std::int32_t value, std::int64_t n) {
@@ -1517,12 +1541,14 @@ int gpart_sclp_isolated(gpart_ctx* ctx, const gpart_csr_view* g, const uint64_t*
             const std::uint64_t total = read_scalar(ctx, prefix.get() + count - 1);
             const std::int64_t bins = static_cast<std::int64_t>(total / capacity + (total % capacity != 0));
             Scratch<std::int32_t> reps(ctx, bins);
+            Scratch<std::uint64_t> run_base(ctx, bins);
             hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(bins)), dim3(kBlock), 0, ctx->stream, reps.get(),
                                0x7fffffff, bins);
             hipLaunchKernelGGL(isolated_rep_kernel, dim3(grid_for(count)), dim3(kBlock), 0, ctx->stream, count,
-                               list.get(), prefix.get(), vertex_weights, capacity, reps.get());
+                               list.get(), prefix.get(), vertex_weights, capacity, reps.get(), run_base.get());
             hipLaunchKernelGGL(isolated_commit_kernel, dim3(grid_for(count)), dim3(kBlock), 0, ctx->stream, count,
-                               list.get(), prefix.get(), vertex_weights, capacity, reps.get(), clusters, cw);
+                               list.get(), prefix.get(), vertex_weights, capacity, reps.get(), run_base.get(),
+                               clusters, cw);
             GPART_LAUNCH_CHECK();
         }
         *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));

@@ -47,6 +47,11 @@ struct gpart_ctx {
     // more than 1/rebuild_share of them (share <= 0: never).
     std::int64_t rebuild_min_entries = std::int64_t{1} << 23;
     std::int64_t rebuild_share = 16;
+    // The rebalance's bucket-counter replicas, kept between rounds: the bucket
+    // scans zero every counter they read, so a completed round leaves them zero.
+    void* rebalance_counters = nullptr;
+    std::size_t rebalance_counter_bytes = 0;
+    bool rebalance_counters_zero = false;
 };
 
 namespace gpart_hip {

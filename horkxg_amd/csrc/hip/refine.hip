@@ -33,6 +33,26 @@ constexpr int kBlock = 256;
 
 inline unsigned grid_for(std::int64_t threads) { return static_cast<unsigned>((threads + kBlock - 1) / kBlock); }
 
+// ---------------------------------------------------------------- round state
+
+// Device record of one round, read back once at its end.
+struct RoundState {
+    long long light;       // movers with degree <= kHeavyDegree
+    long long heavy;       // movers above
+    long long delta;       // exact cut decrease
+    long long candidates;  // rebalance: vertices of overloaded parts
+    unsigned long long max_weight;
+    long long edges;       // CSR entries of all movers (light + heavy)
+    unsigned long long cut_directed;  // rebuild rounds: directed cut weight after the round
+    unsigned long long pad[1];
+    unsigned long long pw[32];
+};
+
+constexpr int kHeavyDegree = 64;
+constexpr int kRoundBlocks = 1024;  // cap of grid-stride launches whose count lives on the device
+constexpr int kCacheBlocks = 4096;  // cap of the cache build launches (grid-stride)
+
+
 // ---------------------------------------------------------------- cut / weights
 
 // Directed cut weight over edge tiles (grid-stride over tiles); one partial per
@@ -221,7 +241,10 @@ template <int G, typename C>
 __global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, const C* cache,
                                                               const std::int32_t* labels, int k, double filter,
                                                               std::int32_t* locks, std::int32_t* targets,
-                                                              long long* gains) {
+                                                              long long* gains, RoundState* st) {
+    // the round's state starts at zero (the filter appends the movers to it)
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(RoundState) / 8)
+        reinterpret_cast<unsigned long long*>(st)[threadIdx.x] = 0;
     const std::int64_t v = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
     const int j = threadIdx.x & (G - 1);
     if (v >= n) return;  // uniform per group
@@ -348,25 +371,6 @@ __global__ void priority_filter_hub_finish_kernel(std::int64_t count, const std:
     flags[v] = targets[v] >= 0 && gains[v] + sums[i] >= 0;
     sums[i] = 0;
 }
-
-// ---------------------------------------------------------------- round state
-
-// Device record of one round, read back once at its end.
-struct RoundState {
-    long long light;       // movers with degree <= kHeavyDegree
-    long long heavy;       // movers above
-    long long delta;       // exact cut decrease
-    long long candidates;  // rebalance: vertices of overloaded parts
-    unsigned long long max_weight;
-    long long edges;       // CSR entries of all movers (light + heavy)
-    unsigned long long cut_directed;  // rebuild rounds: directed cut weight after the round
-    unsigned long long pad[1];
-    unsigned long long pw[32];
-};
-
-constexpr int kHeavyDegree = 64;
-constexpr int kRoundBlocks = 1024;  // cap of grid-stride launches whose count lives on the device
-constexpr int kCacheBlocks = 4096;  // cap of the cache build launches (grid-stride)
 
 // incremental_update_cache_and_cut (:69-95) over a mover list whose length is on the device.
 template <typename O, typename W, int G, typename C>
@@ -669,7 +673,7 @@ struct CandidateStore {
 // incremental_rebalance_proposals (:111-157) + balance_source_histogram (:174-186).
 // One group of G lanes per candidate, lane j owning parts [8j, 8j + 8).
 template <typename W, int G, typename C>
-__global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundState* st, const std::int32_t* cand,
+__global__ void __launch_bounds__(kBlock) balance_proposals_kernel(RoundState* st, const std::int32_t* cand,
                                                                    const C* cache,
                                                                    const std::int32_t* labels, const W* vertex_weights,
                                                                    int k, const unsigned long long* part_weights,
@@ -678,6 +682,10 @@ __global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundSt
                                                                    unsigned long long* replicas) {
     __shared__ unsigned long long pw[32];
     __shared__ BucketTable table;
+    // the round state past the candidate count starts at zero
+    constexpr unsigned kCandidateWord = offsetof(RoundState, candidates) / 8;
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(RoundState) / 8 && threadIdx.x != kCandidateWord)
+        reinterpret_cast<unsigned long long*>(st)[threadIdx.x] = 0;
     if (threadIdx.x < 32) pw[threadIdx.x] = threadIdx.x < k ? part_weights[threadIdx.x] : 0;
     table_clear(table);
     __syncthreads();
@@ -790,7 +798,7 @@ __global__ void __launch_bounds__(kBlock) balance_proposals_kernel(const RoundSt
 constexpr int kBucketScanThreads = 1024;
 static_assert(kBucketScanThreads >= kBalanceWidth, "a thread per bucket");
 static_assert(kRoundBlocks % kBucketScanThreads == 0, "column scan items per thread");
-__global__ void __launch_bounds__(kBucketScanThreads) balance_scan_kernel(const unsigned long long* replicas, int k,
+__global__ void __launch_bounds__(kBucketScanThreads) balance_scan_kernel(unsigned long long* replicas, int k,
                                                                     unsigned long long* counts,
                                                                     unsigned long long* offsets,
                                                                     const unsigned long long* part_weights,
@@ -809,6 +817,8 @@ __global__ void __launch_bounds__(kBucketScanThreads) balance_scan_kernel(const 
             const long long bucket = static_cast<long long>(blockIdx.x) * kBalanceWidth + b;
 #pragma unroll
             for (int r = 0; r < kReplicas; ++r) total += replicas[r * width + bucket];
+#pragma unroll
+            for (int r = 0; r < kReplicas; ++r) replicas[r * width + bucket] = 0;  // zero for the next round
             counts[bucket] = total;
         }
         items[i] = total;
@@ -1217,15 +1227,15 @@ struct Ops {
                          unsigned long long* part_weights, gpart_refine_status* out) {
         const std::int64_t n = g->n;
         const auto* off = static_cast<const O*>(g->offsets);
+        // the proposals kernel zeroes the round state
         Scratch<RoundState> st(ctx, 1);
-        fill_bytes(ctx, st.get(), 0, sizeof(RoundState));
         const int G = group_for_k(k);
         profile_begin(ctx, GPART_PROFILE_REFINE_PROPOSALS);
-        const unsigned blocks = grid_for(n * G);
+        const unsigned blocks = std::max<unsigned>(1, grid_for(n * G));
         switch (G) {
-            case 1: hipLaunchKernelGGL((lp_proposals_kernel<1, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            case 2: hipLaunchKernelGGL((lp_proposals_kernel<2, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
-            default: hipLaunchKernelGGL((lp_proposals_kernel<4, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains); break;
+            case 1: hipLaunchKernelGGL((lp_proposals_kernel<1, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
+            case 2: hipLaunchKernelGGL((lp_proposals_kernel<2, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
+            default: hipLaunchKernelGGL((lp_proposals_kernel<4, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
         }
         GPART_LAUNCH_CHECK();
         profile_end(ctx, GPART_PROFILE_REFINE_PROPOSALS, static_cast<double>(n) * (20.0 + sizeof(C) * k));
@@ -1250,13 +1260,24 @@ struct Ops {
         const auto* vw = static_cast<const W*>(g->vertex_weights);
         static const PowerTable powers = make_power_table();
         const std::int64_t width = static_cast<std::int64_t>(k) * kBalanceWidth;
-        // the round state and both replica arrays in one block, cleared by one fill
-        constexpr std::int64_t kStateWords = sizeof(RoundState) / 8;
-        Scratch<unsigned long long> cleared(ctx, kStateWords + 2 * kReplicas * width);
+        // Both replica arrays live in the context between rounds and are zero at rest
+        // (the bucket scans clear what they read); the proposals kernel zeroes the
+        // round state after the candidate count.
+        const std::size_t counter_bytes = static_cast<std::size_t>(2 * kReplicas * width) * 8;
+        if (ctx->rebalance_counter_bytes < counter_bytes) {
+            if (ctx->rebalance_counters) ctx_free(ctx, ctx->rebalance_counters);
+            ctx->rebalance_counters = nullptr;
+            ctx->rebalance_counter_bytes = 0;
+            ctx->rebalance_counters = ctx_alloc(ctx, counter_bytes);
+            ctx->rebalance_counter_bytes = counter_bytes;
+            ctx->rebalance_counters_zero = false;
+        }
+        if (!ctx->rebalance_counters_zero) fill_bytes(ctx, ctx->rebalance_counters, 0, ctx->rebalance_counter_bytes);
+        ctx->rebalance_counters_zero = false;  // until this round completes
+        Scratch<RoundState> st(ctx, 1);
         Scratch<unsigned long long> counts(ctx, 2 * width), offsets(ctx, 2 * width), thresholds(ctx, 64);
-        fill_bytes(ctx, cleared.get(), 0, static_cast<std::size_t>(cleared.size()) * 8);
-        auto* st_ptr = reinterpret_cast<RoundState*>(cleared.get());
-        unsigned long long* src_replicas = cleared.get() + kStateWords;
+        auto* st_ptr = st.get();
+        auto* src_replicas = static_cast<unsigned long long*>(ctx->rebalance_counters);
         unsigned long long* dst_replicas = src_replicas + kReplicas * width;
         unsigned long long* src_counts = counts.get();
         unsigned long long* dst_counts = counts.get() + width;
@@ -1311,6 +1332,7 @@ struct Ops {
         apply(ctx, g, c, st_ptr, light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, nullptr,
               false);
         finish(ctx, g, st_ptr, part_weights, k, out);
+        ctx->rebalance_counters_zero = true;
         // per candidate: id, label, weight, cache row, target and bucket writes
         profile_add_bytes(ctx, GPART_PROFILE_REBALANCE,
                           static_cast<double>(out->candidates) * (16.0 + sizeof(W) + sizeof(C) * k));

@@ -73,7 +73,10 @@ struct Level {
 
 // run_incremental_refinement (src/refine_incremental.cuh:235-361)
 template <typename Types>
-RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, int rounds) {
+// known_cut: the cut of the level's labels when the caller knows it exactly (a
+// projection keeps the coarse level's cut), saving a scan of the CSR.
+RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, int rounds,
+                           const std::uint64_t* known_cut = nullptr) {
     const auto start = clock_type::now();
     auto* ctx = detail::context();
     validate_gain_range(level.graph);
@@ -104,7 +107,7 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     GPART_CHECK(gpart_refine_build_cache(ctx, &view, &classes, level.partition.data(), k, cache.data(), cache_bytes));
 
     RefineStats stats;
-    stats.initial_cut = level.cut();
+    stats.initial_cut = known_cut ? *known_cut : level.cut();
     const double filter = maximum_edge <= 1 ? 0.25 : 0.75;
     auto finish = [&](std::uint64_t best_cut) {
         level.host_part_weights = level.part_weights.to_host();
@@ -324,7 +327,8 @@ DeviceUncoarsenResult<Types> refine_hierarchy_impl(H& hierarchy, P&& coarsest_pa
                                                         options.refinement_growth_ratio *
                                                             static_cast<double>(last_refined);
         if (refine_this) {
-            report.refinement = run_refinement(next, options, options.max_rounds);
+            // the projection keeps the cut (the strict checks above confirm it)
+            report.refinement = run_refinement(next, options, options.max_rounds, &last_cut);
             last_refined = report.vertices;
         } else {
             report.refinement.initial_cut = report.refinement.final_cut = last_cut;
