@@ -208,21 +208,27 @@ def traffic_summary(args) -> int:
         # run's dispatches divided by the family's launches in the run
         kf = kf * nf / args.family_launches
         kw = kw * nw / args.family_launches
-    out = {
-        "workload": args.workload,
+    entry = {
         "kernel": args.kernel,
         "dispatches": {"fetch": nf, "write": nw},
         "family_launches": args.family_launches or None,
-        "fetch_kib_per_dispatch": round(kf, 1),
-        "write_kib_per_dispatch": round(kw, 1),
+        # per launch: one dispatch, or one family launch when family_launches is given
+        "fetch_kib_per_launch": round(kf, 1),
+        "write_kib_per_launch": round(kw, 1),
         "copy_calibration": {"bytes": copy_bytes, "fetch_kib": round(cf, 1), "write_kib": round(cw, 1),
                              "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4)},
-        "traffic_bytes_per_dispatch": round(kf * 1024.0 * f_scale + kw * 1024.0 * w_scale),
+        "traffic_bytes_per_launch": round(kf * 1024.0 * f_scale + kw * 1024.0 * w_scale),
     }
+    # one file per workload, one entry per kernel
     PROFILES.mkdir(exist_ok=True)
     path = PROFILES / f"traffic_{args.workload}.json"
+    out = {"workload": args.workload, "kernels": {}}
+    if path.exists():
+        old = json.loads(path.read_text())
+        out["kernels"] = old.get("kernels", {old["kernel"]: old} if "kernel" in old else {})
+    out["kernels"][args.kernel] = entry
     path.write_text(json.dumps(out, indent=1) + "\n")
-    print(json.dumps(out))
+    print(json.dumps(entry))
     return 0
 
 
@@ -232,10 +238,11 @@ def recorded_traffic(workload: str, family: str):
     if not path.exists() or family not in FAMILY_KERNELS:
         return None
     rec = json.loads(path.read_text())
-    kernel = rec.get("kernel", "^$")
-    if kernel != FAMILY_KERNELS[family] and not re.search(kernel, FAMILY_KERNELS[family]):
-        return None
-    return rec["traffic_bytes_per_dispatch"]
+    entries = rec.get("kernels", {rec["kernel"]: rec} if "kernel" in rec else {})
+    for kernel, entry in entries.items():
+        if kernel == FAMILY_KERNELS[family] or re.search(kernel, FAMILY_KERNELS[family]):
+            return entry["traffic_bytes_per_launch"]
+    return None
 
 
 def _mix_worker(rank, specs, order, mode, barrier, counter, queue, hw_queues):
@@ -509,6 +516,8 @@ def main() -> int:
     p.add_argument("--mix-workers", type=int, default=1,
                    help="host threads (streams) of the batched mix; concurrent workers are switched off (1)")
     p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--cut-seeds", type=int, default=5,
+                   help="seeds (from --seed) of the GPU cut median and of the CPU oracle's")
     p.add_argument("--ratio", type=float, default=1.03)
     p.add_argument("--spectral-mode", default=None, choices=["plain_fp64", "batched_fp64", "batched_fp32"],
                    help="default: the workload's own form")
@@ -693,7 +702,7 @@ def main() -> int:
         }
 
     # the GPU's cut over seeds seed..seed+4 (the reference's protocol: median of 5 seeds)
-    seeds = list(range(args.seed, args.seed + 5))
+    seeds = list(range(args.seed, args.seed + max(1, args.cut_seeds)))
     gpu_cuts = [int(cut)]
     for s in seeds[1:]:
         _, rep_s = graph.partition(k, args.ratio, s, spectral_mode=args.spectral_mode, want_labels=False)
