@@ -136,6 +136,8 @@ def hip_lib():
         H.gpart_cache_info.argtypes = [C.c_void_p, C.c_void_p]
         H.gpart_refine_set_engine.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int64]
         H.gpart_refine_keep_ratio.argtypes = [C.c_uint64, C.c_uint64]
+        H.gpart_spectral_set_queue.argtypes = [C.c_int32]
+        H.gpart_spectral_queue_set_tuning.argtypes = [C.c_int32]
         _hip = H
     return _hip
 
@@ -295,6 +297,20 @@ def set_refine_engine(max_vertices: int = 0, vertices_per_group: int = 512, max_
     not depend on it."""
     _check(hip_lib().gpart_refine_set_engine(_ctx(), int(max_vertices), int(vertices_per_group), int(max_grid),
                                              int(max_degree)))
+
+
+def set_spectral_queue(enabled: bool = True, own_blocks: int | None = None) -> None:
+    """Batched bisection through the persistent queue (default) or depth by
+    depth; own_blocks: 8-vertex blocks per queue workgroup in the team-size rule
+    (8..128). Process-wide; results do not depend on either."""
+    _check(hip_lib().gpart_spectral_set_queue(int(bool(enabled))))
+    if own_blocks is not None:
+        _check(hip_lib().gpart_spectral_queue_set_tuning(int(own_blocks)))
+
+
+def spectral_queue_settings() -> dict:
+    H = hip_lib()
+    return {"enabled": bool(H.gpart_spectral_queue_enabled()), "own_blocks": int(H.gpart_spectral_queue_tuning())}
 
 
 def refine_keep_ratio(current: int, best: int) -> bool:

@@ -114,6 +114,37 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
         const int precision = mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64;
         const auto max_task = gpart_spectral_depth_max_task();
         std::vector<gpart_bisect_task> depth{{0, n, 0, parts, total}};
+        if (gpart_spectral_queue_enabled()) {
+            // Tasks above the batched bound go through the plain form depth by depth;
+            // every smaller task, of any depth, is a root of one persistent launch.
+            std::vector<gpart_bisect_task> roots;
+            while (!depth.empty()) {
+                std::vector<gpart_bisect_task> next;
+                for (const auto& t : depth) {
+                    if (t.parts == 1) {
+                        if (t.weight > capacity) throw std::runtime_error("GPU initializer infeasible balance");
+                    } else if (t.count > max_task) {
+                        plain_task(graph, labels, order, vertex_weights, t, seed, capacity, x, y, degree, prefix, next);
+                    } else {
+                        roots.push_back(t);
+                    }
+                }
+                depth.swap(next);
+            }
+            gpart_queue_graph q{};
+            q.graph = view;
+            q.labels = labels.data();
+            q.order = order.data();
+            q.vertex_weights = vertex_weights.data();
+            q.coords = x.data();
+            q.capacity = capacity;
+            q.seed = seed;
+            const std::vector<std::int32_t> root_graph(roots.size(), 0);
+            GPART_CHECK(gpart_spectral_queue(ctx, &q, 1, roots.data(), root_graph.data(),
+                                             static_cast<std::int32_t>(roots.size()), precision, nullptr));
+            if (q.status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");
+            if (q.status == 2) throw std::runtime_error("GPU initializer infeasible balance");
+        }
         DeviceVector<gpart_bisect_task> device_next(64);
         while (!depth.empty()) {
             std::vector<gpart_bisect_task> batched, next;

@@ -26,6 +26,7 @@ def _bind():
     H.gpart_cut.argtypes = [vp, vp, vp, vp]
     H.gpart_cut_noalloc.argtypes = [vp, vp, vp, vp]
     H.gpart_cut_noalloc_max_entries.restype = i64
+    H.gpart_spectral_queue.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_int32, C.c_int, vp]
 
 
 _bind()
@@ -185,3 +186,48 @@ def cut(g: DeviceGraph, labels: DeviceArray, noalloc: bool = False) -> int:
 
 def cut_noalloc_max_entries() -> int:
     return int(hip_lib().gpart_cut_noalloc_max_entries())
+
+
+class BisectTask(C.Structure):
+    _fields_ = [("begin", C.c_int32), ("count", C.c_int32), ("first_part", C.c_int32), ("parts", C.c_int32),
+                ("weight", C.c_uint64)]
+
+
+class QueueGraph(C.Structure):
+    _fields_ = [("graph", CsrView), ("labels", C.c_void_p), ("order", C.c_void_p), ("vertex_weights", C.c_void_p),
+                ("coords", C.c_void_p), ("capacity", C.c_uint64), ("seed", C.c_uint32), ("status", C.c_int32)]
+
+
+class QueueReport(C.Structure):
+    _fields_ = [("tasks", C.c_int64), ("slots", C.c_int64), ("entries", C.c_int64), ("vertices", C.c_int64),
+                ("workgroups", C.c_int32), ("pad", C.c_int32)]
+
+
+def spectral_queue(graphs, parts, capacities, seeds, precision: str = "fp64", vertex_weights=None):
+    """gpart_spectral_queue: the recursive bisection of every graph (DeviceGraph)
+    into parts[i] parts in one persistent launch, each graph from one root task
+    over all its vertices. vertex_weights[i]: host weights (unit if None).
+    Returns ([(labels, order, coords, status)] per graph, report dict)."""
+    count = len(graphs)
+    keep, qgs = [], (QueueGraph * max(count, 1))()
+    roots = (BisectTask * max(count, 1))()
+    root_graph = (C.c_int32 * max(count, 1))()
+    for i, g in enumerate(graphs):
+        w = np.ones(g.n, np.uint64) if vertex_weights is None or vertex_weights[i] is None \
+            else np.asarray(vertex_weights[i], np.uint64)
+        labels = DeviceArray(np.zeros(g.n, np.int32), np.int32)
+        order = DeviceArray(np.arange(g.n, dtype=np.int32), np.int32)
+        vw = DeviceArray(w, np.uint64)
+        coords = DeviceArray(dtype=np.float64, count=g.n)
+        keep.append((labels, order, vw, coords))
+        qgs[i] = QueueGraph(g.view, labels.ptr, order.ptr, vw.ptr, coords.ptr, int(capacities[i]),
+                            int(seeds[i]) & 0xFFFFFFFF, 0)
+        roots[i] = BisectTask(0, g.n, 0, int(parts[i]), int(w.sum()))
+        root_graph[i] = i
+    rep = QueueReport()
+    prec = {"fp64": 0, "fp32": 1}[precision]
+    _check(hip_lib().gpart_spectral_queue(_ctx(), qgs, count, roots, root_graph, count, prec, C.byref(rep)))
+    out = [(labels.get(), order.get(), coords.get(), int(qgs[i].status))
+           for i, (labels, order, vw, coords) in enumerate(keep)]
+    return out, {"tasks": rep.tasks, "slots": rep.slots, "entries": rep.entries, "vertices": rep.vertices,
+                 "workgroups": rep.workgroups}

@@ -358,6 +358,46 @@ int gpart_spectral_depth(gpart_ctx* ctx, const gpart_csr_view* g, int32_t* label
 /* Largest task (vertices) the batched form accepts. */
 int64_t gpart_spectral_depth_max_task(void);
 
+/* Persistent form of the batched bisection, replacing the reference's host task
+ * loop (src/initial_partition_gpu.cu:198-255) for every task of at most
+ * gpart_spectral_depth_max_task() vertices: one launch serves the root tasks and
+ * every task they produce, at every depth, of every graph of the call, from one
+ * device work queue. A task of c vertices is served by ceil(ceil(c/8) / B)
+ * workgroups (B: gpart_spectral_queue_tuning()). Labels, order and coordinates are
+ * bit-identical to gpart_spectral_depth run depth by depth on the same roots.
+ * roots: host array; root_graph[r] names the graph of roots[r]; each root has at
+ * least two parts and its vertices carry the label first_part. Per graph status
+ * out: 0 ok, 1 infeasible split, 2 leaf above capacity (the graph's labels are then
+ * incomplete). A wait beyond its spin limit or a queue overflow is an error. */
+typedef struct gpart_queue_graph {
+    gpart_csr_view graph;            /* every graph of one call has the same width */
+    int32_t* labels;                 /* n, device */
+    int32_t* order;                  /* n, device: the roots' segments of vertex ids */
+    const uint64_t* vertex_weights;  /* n, device */
+    double* coords;                  /* n, device, or NULL: final coordinates */
+    uint64_t capacity;
+    uint32_t seed;
+    int32_t status;                  /* out */
+} gpart_queue_graph;
+typedef struct gpart_queue_report {
+    int64_t tasks;       /* tasks served */
+    int64_t slots;       /* team slots written (roots included) */
+    int64_t entries;     /* CSR entries of the served tasks' rows */
+    int64_t vertices;    /* vertices of the served tasks */
+    int32_t workgroups;  /* grid of the launch */
+    int32_t pad;
+} gpart_queue_report;
+int gpart_spectral_queue(gpart_ctx* ctx, gpart_queue_graph* graphs, int32_t graph_count,
+                         const gpart_bisect_task* roots, const int32_t* root_graph, int32_t root_count,
+                         int precision, gpart_queue_report* report);
+/* Whether the host drivers use the persistent form for batched bisection (default
+ * 1) or gpart_spectral_depth depth by depth (0). */
+int gpart_spectral_set_queue(int32_t enabled);
+int32_t gpart_spectral_queue_enabled(void);
+/* Blocks of 8 vertices per queue workgroup in the team-size rule (8..128). */
+int gpart_spectral_queue_set_tuning(int32_t own_blocks);
+int32_t gpart_spectral_queue_tuning(void);
+
 /* ---------------- refinement (src/refine.cu, src/refine_incremental.cuh) ---------------- */
 
 /* refine_cut_kernel + device_cut_value (src/refine.cu:52-87,117-142).
