@@ -245,14 +245,16 @@ def recorded_traffic(workload: str, family: str):
     return None
 
 
-def _mix_worker(rank, specs, order, mode, barrier, counter, queue, hw_queues):
+def _mix_worker(rank, specs, order, mode, barrier, counter, queue, hw_queues, group=1):
     """One host process of the concurrent mix. It generates every graph of the mix
     on the GPU (outside the timed region), partitions the three smallest once
     untimed (code objects, allocator), waits at a barrier shared with the other
-    processes, then claims graphs from a shared counter over `order` (largest
-    first) until none is left, and exits: an idle process that keeps its GPU
-    context alive slows the others down. Labels are checked after the clock stops.
-    hw_queues caps the process's hardware queues (read at HIP initialisation)."""
+    processes, then claims groups of `group` graphs from a shared counter over
+    `order` (largest first) until none is left, and exits: an idle process that
+    keeps its GPU context alive slows the others down. A group is one
+    partition_batch call, whose initial partitions share one persistent launch.
+    Labels are checked after the clock stops. hw_queues caps the process's
+    hardware queues (read at HIP initialisation)."""
     if hw_queues:
         os.environ["GPU_MAX_HW_QUEUES"] = str(hw_queues)
     try:
@@ -263,19 +265,21 @@ def _mix_worker(rank, specs, order, mode, barrier, counter, queue, hw_queues):
         for i in order[-3:]:
             graphs[i].partition(specs[i]["k"], specs[i]["ratio"], specs[i]["seed"], spectral_mode=mode,
                                 want_labels=False)
+        hx.set_batch_group(max(1, group))
         barrier.wait()
         t0 = time.perf_counter()
         done = []
         while True:
             with counter.get_lock():
                 slot = counter.value
-                counter.value += 1
+                counter.value += max(1, group)
             if slot >= len(order):
                 break
-            i = order[slot]
-            labels, rep = graphs[i].partition(specs[i]["k"], specs[i]["ratio"], specs[i]["seed"],
-                                              spectral_mode=mode, want_labels=True)
-            done.append((i, labels, rep))
+            ids = order[slot:slot + max(1, group)]
+            labels, reps, _ = hx.partition_batch([graphs[i] for i in ids], [specs[i]["k"] for i in ids],
+                                                 [specs[i]["ratio"] for i in ids], [specs[i]["seed"] for i in ids],
+                                                 spectral_mode=mode)
+            done.extend(zip(ids, labels, reps))
         t1 = time.perf_counter()
         for i, labels, rep in done:
             assert graphs[i].cut(labels) == rep["final_cut"], specs[i]
@@ -286,7 +290,7 @@ def _mix_worker(rank, specs, order, mode, barrier, counter, queue, hw_queues):
         queue.put((rank, None, None, repr(e)))
 
 
-def _mix_processes(specs, sizes, procs, mode, steps, hw_queues=0):
+def _mix_processes(specs, sizes, procs, mode, steps, hw_queues=0, group=1):
     """Partitions/sec of `procs` processes sharing the GPU, one fresh set of
     processes per step: per step, the wall from the earliest start to the latest
     finish. Returns (walls, {index: cut})."""
@@ -299,7 +303,8 @@ def _mix_processes(specs, sizes, procs, mode, steps, hw_queues=0):
         barrier = ctx.Barrier(procs)
         counter = ctx.Value("i", 0)
         queue = ctx.Queue()
-        workers = [ctx.Process(target=_mix_worker, args=(r, specs, order, mode, barrier, counter, queue, hw_queues))
+        workers = [ctx.Process(target=_mix_worker,
+                               args=(r, specs, order, mode, barrier, counter, queue, hw_queues, group))
                    for r in range(procs)]
         for w in workers:
             w.start()
@@ -343,7 +348,8 @@ def run_mix_processes(args) -> int:
             scale, ef = s["size"]
             sizes.append((1 << scale) * (1 + 2 * ef))
     t = time.perf_counter()
-    walls, cuts = _mix_processes(specs, sizes, args.mix_procs, "batched_fp32", args.steps, args.mix_hw_queues)
+    walls, cuts = _mix_processes(specs, sizes, args.mix_procs, "batched_fp32", args.steps, args.mix_hw_queues,
+                                 args.mix_group)
     for step, wall in enumerate(walls):
         print(f"[bench] mix step {step}: {len(specs)} partitions in {wall:.3f} s on {args.mix_procs} processes",
               file=sys.stderr, flush=True)
@@ -391,7 +397,7 @@ def run_mix_processes(args) -> int:
         "config": {"workload": f"mix{len(specs)}", "graphs": len(specs), "entries_and_vertices_total": int(sum(sizes)),
                    "parts": "k_i from {2,4,8,32}", "imbalance": 1.03, "width": "standard",
                    "processes": args.mix_procs, "hw_queues_per_process": args.mix_hw_queues or "default",
-                   "spectral": "batched_fp32"},
+                   "group": args.mix_group, "spectral": "batched_fp32"},
         "plain": {"partitions_per_sec": round(plain_value, 3), "processes": 1, "spectral": "plain_fp64",
                   "wall_s": round(plain_walls[0], 4)},
         "batched_over_plain": round(value / plain_value, 3),
@@ -424,6 +430,8 @@ def run_mix(args) -> int:
     parts = [s["k"] for s in specs]
     ratios = [s["ratio"] for s in specs]
     seeds = [s["seed"] for s in specs]
+
+    hx.set_batch_group(max(1, args.mix_group))
 
     def batch(mode, workers, want_labels):
         return hx.partition_batch(graphs, parts, ratios, seeds, spectral_mode=mode, workers=workers,
@@ -485,7 +493,7 @@ def run_mix(args) -> int:
         "config": {"workload": f"mix{len(graphs)}", "graphs": len(graphs),
                    "vertices_total": int(sum(n for n, _ in sizes)), "entries_total": int(sum(m for _, m in sizes)),
                    "parts": "k_i from {2,4,8,32}", "imbalance": 1.03, "width": "standard",
-                   "workers": args.mix_workers, "spectral": "batched_fp32"},
+                   "workers": args.mix_workers, "group": args.mix_group, "spectral": "batched_fp32"},
         "plain": {"partitions_per_sec": round(plain_value, 3), "workers": 1, "spectral": "plain_fp64",
                   "wall_s": round(plain_wall, 4)},
         "batched_over_plain": round(value / plain_value, 3),
@@ -513,6 +521,8 @@ def main() -> int:
                    help="host processes sharing the GPU in the batched mix (1: the in-process batch)")
     p.add_argument("--mix-hw-queues", type=int, default=1, choices=range(0, 5),
                    help="GPU_MAX_HW_QUEUES of each mix process (0: the runtime's default)")
+    p.add_argument("--mix-group", type=int, default=8,
+                   help="graphs per partition_batch group: their initial partitions share one persistent launch")
     p.add_argument("--mix-workers", type=int, default=1,
                    help="host threads (streams) of the batched mix; concurrent workers are switched off (1)")
     p.add_argument("--seed", type=int, default=0)

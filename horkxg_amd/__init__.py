@@ -26,6 +26,8 @@ from ._native import (  # noqa: F401
     host_lib,
     lib_dir,
     partition_batch,
+    set_batch_group,
+    batch_group,
     partition_capacity,
     profile_enable,
     profile_read,

@@ -137,7 +137,9 @@ def hip_lib():
         H.gpart_refine_set_engine.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int64]
         H.gpart_refine_keep_ratio.argtypes = [C.c_uint64, C.c_uint64]
         H.gpart_spectral_set_queue.argtypes = [C.c_int32]
-        H.gpart_spectral_queue_set_tuning.argtypes = [C.c_int32]
+        H.gpart_spectral_queue_set_tuning.argtypes = [C.c_int32, C.c_int32]
+        H.gpart_spectral_queue_tuning.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        H.gpart_spectral_queue_tuning.restype = None
         _hip = H
     return _hip
 
@@ -299,18 +301,24 @@ def set_refine_engine(max_vertices: int = 0, vertices_per_group: int = 512, max_
                                              int(max_degree)))
 
 
-def set_spectral_queue(enabled: bool = True, own_blocks: int | None = None) -> None:
-    """Batched bisection through the persistent queue (default) or depth by
-    depth; own_blocks: 8-vertex blocks per queue workgroup in the team-size rule
-    (8..128). Process-wide; results do not depend on either."""
-    _check(hip_lib().gpart_spectral_set_queue(int(bool(enabled))))
-    if own_blocks is not None:
-        _check(hip_lib().gpart_spectral_queue_set_tuning(int(own_blocks)))
+def set_spectral_queue(enabled: bool = True, own_blocks: int | None = None, max_task: int | None = None) -> None:
+    """Single-graph batched bisection through the persistent queue, or depth by
+    depth (the default); own_blocks: 8-vertex blocks per queue workgroup in the
+    team-size rule (8..128); max_task: largest task the single-graph driver gives
+    the queue (1..4096). Process-wide; results do not depend on them."""
+    H = hip_lib()
+    _check(H.gpart_spectral_set_queue(int(bool(enabled))))
+    if own_blocks is not None or max_task is not None:
+        cur = spectral_queue_settings()
+        _check(H.gpart_spectral_queue_set_tuning(int(own_blocks if own_blocks is not None else cur["own_blocks"]),
+                                                 int(max_task if max_task is not None else cur["max_task"])))
 
 
 def spectral_queue_settings() -> dict:
     H = hip_lib()
-    return {"enabled": bool(H.gpart_spectral_queue_enabled()), "own_blocks": int(H.gpart_spectral_queue_tuning())}
+    own, top = C.c_int32(), C.c_int32()
+    H.gpart_spectral_queue_tuning(C.byref(own), C.byref(top))
+    return {"enabled": bool(H.gpart_spectral_queue_enabled()), "own_blocks": own.value, "max_task": top.value}
 
 
 def refine_keep_ratio(current: int, best: int) -> bool:
@@ -451,11 +459,23 @@ class Hierarchy:
         return out, cut.value
 
 
+def set_batch_group(graphs: int = 16) -> None:
+    """Graphs per group of partition_batch: a group's initial partitions run in
+    one persistent launch (batched modes). 1: one graph at a time."""
+    _check(host_lib().gpart_batch_set_group(int(graphs)))
+
+
+def batch_group() -> int:
+    return int(host_lib().gpart_batch_group())
+
+
 def partition_batch(graphs, parts, ratios=None, seeds=None, spectral_mode: str = "batched_fp32",
                     workers: int = 1, want_labels: bool = True):
     """Partition many graphs, largest first, in the calling thread (workers must
-    be 1: the concurrent form is switched off). Returns (labels list or None, report dicts, wall
-    seconds from the first coarsening start to the last labels on the device)."""
+    be 1: the concurrent form is switched off), in groups of batch_group() graphs
+    of one width whose initial partitions share one launch. Returns (labels list or
+    None, report dicts, wall seconds from the first coarsening start to the last
+    labels on the device)."""
     count = len(graphs)
     parts = list(parts)
     ratios = [1.03] * count if ratios is None else list(ratios)

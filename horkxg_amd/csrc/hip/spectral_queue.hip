@@ -54,8 +54,9 @@ constexpr int kMemberBits = 20;
 
 static_assert(kBatchThreads == 1024, "one member is 32 half-waves");
 
-std::atomic<int> queue_own_blocks{64};  // the team-size rule's blocks per member
-std::atomic<int> queue_enabled{1};      // host drivers use the queue for batched bisection
+std::atomic<int> queue_own_blocks{8};               // the team-size rule's blocks per member
+std::atomic<int> queue_max_task{kMaxBatchedTask};  // the single-graph driver's largest queue task
+std::atomic<int> queue_enabled{0};                 // the single-graph driver uses the queue
 
 struct QueueGraph {
     const void* offsets;
@@ -130,8 +131,11 @@ struct QueueShared {
     int barrier_ok;
 };
 
-__device__ __forceinline__ unsigned long long atomic_load_agent(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Polls read relaxed (an agent-scope acquire invalidates the caches, which would
+// evict the data of every task running on the XCD at each poll); the waiter takes
+// one acquire fence once the wait is over.
+__device__ __forceinline__ unsigned long long poll_agent(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ bool aborted(QueueState* state) {
     return __hip_atomic_load(&state->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
@@ -161,7 +165,7 @@ __device__ bool team_barrier(QueueTask* rec, int team, unsigned long long epoch,
             __hip_atomic_store(&rec->release, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             unsigned long long spins = 0;
-            while (atomic_load_agent(&rec->release) < epoch) {
+            while (poll_agent(&rec->release) < epoch) {
                 if (++spins > kQueueSpinLimit || aborted(state)) {
                     raise_abort(state);
                     ok = 0;
@@ -412,9 +416,9 @@ __global__ void __launch_bounds__(kBatchThreads) spectral_queue_kernel(QueueArgs
             } else {
                 unsigned long long spins = 0;
                 for (;;) {
-                    word = atomic_load_agent(&a.slots[slot]);
+                    word = poll_agent(&a.slots[slot]);
                     if (word) break;
-                    if (atomic_load_agent(&a.state->outstanding) == 0 || aborted(a.state)) {
+                    if (poll_agent(&a.state->outstanding) == 0 || aborted(a.state)) {
                         done = 1;
                         break;
                     }
@@ -423,8 +427,9 @@ __global__ void __launch_bounds__(kBatchThreads) spectral_queue_kernel(QueueArgs
                         done = 1;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                    __builtin_amdgcn_s_sleep(2);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             sh.done = done;
             sh.word = word;
@@ -455,15 +460,21 @@ using namespace gpart_hip;
 
 extern "C" {
 
-int gpart_spectral_queue_set_tuning(int32_t own_blocks) {
+int gpart_spectral_queue_set_tuning(int32_t own_blocks, int32_t max_task) {
     return guarded([&] {
         if (own_blocks < 8 || own_blocks > kQueueMaxOwnBlocks)
             throw Error(GPART_ERR_INVALID_ARGUMENT, "blocks per queue member out of range");
+        if (max_task < 1 || max_task > kMaxBatchedTask)
+            throw Error(GPART_ERR_INVALID_ARGUMENT, "largest queue task out of range");
         queue_own_blocks = own_blocks;
+        queue_max_task = max_task;
     });
 }
 
-int32_t gpart_spectral_queue_tuning(void) { return queue_own_blocks.load(); }
+void gpart_spectral_queue_tuning(int32_t* own_blocks, int32_t* max_task) {
+    if (own_blocks) *own_blocks = queue_own_blocks.load();
+    if (max_task) *max_task = queue_max_task.load();
+}
 
 int gpart_spectral_set_queue(int32_t enabled) {
     return guarded([&] { queue_enabled = enabled != 0 ? 1 : 0; });
@@ -495,7 +506,7 @@ int gpart_spectral_queue(gpart_ctx* ctx, gpart_queue_graph* graphs, int32_t grap
             total_m += q.graph.m;
         }
         const int own_blocks = queue_own_blocks.load();
-        long long task_capacity = 0, slot_capacity = 0, max_team = 0;
+        long long task_capacity = 0, slot_capacity = 0, max_team = 0, concurrent = 0;
         for (int r = 0; r < root_count; ++r) {
             const auto& t = roots[r];
             const int gi = root_graph[r];
@@ -505,6 +516,9 @@ int gpart_spectral_queue(gpart_ctx* ctx, gpart_queue_graph* graphs, int32_t grap
                 throw Error(GPART_ERR_INVALID_ARGUMENT, "queue root task out of range");
             // a task of p parts has at most p - 1 tasks in its subtree (itself included)
             task_capacity += t.parts - 1;
+            // members serving the subtree at once: at most one team per part pair
+            // plus one member per own_blocks blocks of its vertices
+            concurrent += t.parts / 2 + (t.count + 8 * own_blocks - 1) / (8 * own_blocks);
         }
         max_team = (kMaxBatchedTask / 8 + own_blocks - 1) / own_blocks;
         slot_capacity = task_capacity * max_team;
@@ -589,7 +603,7 @@ int gpart_spectral_queue(gpart_ctx* ctx, gpart_queue_graph* graphs, int32_t grap
             // every workgroup of the grid can be resident at once; more than the
             // queue's slots would only poll
             const long long grid = std::max<long long>(
-                max_team, std::min<long long>(queue_grid(kernel, ctx->device), slot_capacity));
+                max_team, std::min<long long>({queue_grid(kernel, ctx->device), slot_capacity, concurrent}));
             hipLaunchKernelGGL((spectral_queue_kernel<O, W, T>), dim3(static_cast<unsigned>(grid)),
                                dim3(kBatchThreads), 0, ctx->stream, args);
             GPART_LAUNCH_CHECK();

@@ -97,21 +97,32 @@ void copy_level(const gpart::WeightedGraph<Types>& g, int64_t* offsets, int32_t*
     if (vw) std::copy(g.vertex_weights.begin(), g.vertex_weights.end(), vw);
 }
 
-// One partition with the CLI's production options. Returns the labels on the
-// device. The run ends, inside the timed scope, by returning every block of the
-// hierarchy to the context's cache (the reference's per-run scratch scope); the
-// cache keeps its device segments for the next run.
+// One partition with the CLI's production options, in three phases so that a
+// batch can run the initial partitions of several graphs in one launch. The run
+// ends, inside the timed scope, by returning every block of the hierarchy to the
+// context's cache (the reference's per-run scratch scope); the cache keeps its
+// device segments for the next run.
+using Clock = std::chrono::steady_clock;
+
 template <typename Types>
-gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify,
-                                               int mode, gpart_partition_report* report) {
-    using clock = std::chrono::steady_clock;
+struct PipelineRun {
+    gpart::DeviceHierarchy<Types> hierarchy;
+    std::uint64_t total = 0;
+    Clock::time_point t0, t1, t2;
+    std::uint64_t initial_cut = 0;
+    std::int64_t coarsest_vertices = 0;
+};
+
+template <typename Types>
+PipelineRun<Types> coarsen_phase(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify) {
     auto* ctx = gpart::detail::context();
-    std::uint64_t total = 0, heaviest = 0, lightest = 0;
+    PipelineRun<Types> run;
+    std::uint64_t heaviest = 0, lightest = 0;
     {
         const auto view = g.device.view();
-        GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &total, &heaviest, &lightest));
+        GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &run.total, &heaviest, &lightest));
     }
-    gpart::partition_capacity(total, parts, ratio);
+    gpart::partition_capacity(run.total, parts, ratio);
     gpart::CoarsenOptions co;
     co.parts = parts;
     co.seed = seed;
@@ -126,54 +137,101 @@ gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, dou
     input.edge_weights = decltype(input.edge_weights)::borrow(g.device.edge_weights);
     input.vertex_weights = decltype(input.vertex_weights)::borrow(g.device.vertex_weights);
     GPART_CHECK(gpart_ctx_synchronize(ctx));
-    const auto t0 = clock::now();
-    auto hierarchy = gpart::coarsen<Types>(std::move(input), co);
+    run.t0 = Clock::now();
+    run.hierarchy = gpart::coarsen<Types>(std::move(input), co);
     GPART_CHECK(gpart_ctx_synchronize(ctx));
-    const auto t1 = clock::now();
-    auto coarse = gpart::initial_partition_gpu<Types>(hierarchy.levels.back(), parts, ratio, seed,
-                                                      static_cast<gpart::SpectralMode>(mode));
-    const auto coarsest_vertices = hierarchy.levels.back().vertices();
-    std::uint64_t initial_cut = 0;
-    {
-        const auto view = hierarchy.levels.back().view();
-        GPART_CHECK(gpart_cut(ctx, &view, coarse.data(), &initial_cut));
-    }
+    run.t1 = Clock::now();
+    run.coarsest_vertices = run.hierarchy.levels.back().vertices();
+    return run;
+}
+
+// The initial partition's cut on the coarsest level and the end of its phase.
+template <typename Types>
+void note_initial(PipelineRun<Types>& run, const gpart::DeviceVector<std::int32_t>& coarse) {
+    auto* ctx = gpart::detail::context();
+    const auto view = run.hierarchy.levels.back().view();
+    GPART_CHECK(gpart_cut(ctx, &view, coarse.data(), &run.initial_cut));
     GPART_CHECK(gpart_ctx_synchronize(ctx));
-    const auto t2 = clock::now();
-    const int levels = static_cast<int>(hierarchy.levels.size());
+    run.t2 = Clock::now();
+}
+
+template <typename Types>
+gpart::DeviceVector<std::int32_t> uncoarsen_phase(PipelineRun<Types>& run, gpart::DeviceVector<std::int32_t> coarse,
+                                                  int parts, double ratio, int verify,
+                                                  gpart_partition_report* report) {
+    auto* ctx = gpart::detail::context();
+    const int levels = static_cast<int>(run.hierarchy.levels.size());
     gpart::RefineOptions ro;
     ro.parts = parts;
     ro.imbalance_ratio = ratio;
     ro.max_rounds = 32;
     ro.strict_verify = verify != 0;
     ro.refinement_growth_ratio = 2.0;
-    auto result = gpart::uncoarsen<Types>(hierarchy, std::move(coarse), ro);
+    auto result = gpart::uncoarsen<Types>(run.hierarchy, std::move(coarse), ro);
     GPART_CHECK(gpart_ctx_synchronize(ctx));
-    const auto t3 = clock::now();
-    hierarchy.levels.clear();
-    hierarchy.fine_to_coarse.clear();
+    const auto t3 = Clock::now();
+    run.hierarchy.levels.clear();
+    run.hierarchy.fine_to_coarse.clear();
     auto partition = std::move(result.device_partition);
-    const auto t4 = clock::now();
+    const auto t4 = Clock::now();
     if (report) {
         report->final_cut = result.final_cut;
         report->max_part_weight =
             *std::max_element(result.final_part_weights.begin(), result.final_part_weights.end());
-        report->capacity = gpart::partition_capacity(total, parts, ratio);
-        report->initial_cut = initial_cut;
-        report->coarsest_vertices = coarsest_vertices;
+        report->capacity = gpart::partition_capacity(run.total, parts, ratio);
+        report->initial_cut = run.initial_cut;
+        report->coarsest_vertices = run.coarsest_vertices;
         report->levels = levels;
         int refined = 0;
         for (const auto& l : result.levels) refined += l.refinement.rounds > 0 ? 1 : 0;
         report->refined_levels = refined;
         auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
-        report->coarsen_seconds = secs(t0, t1);
-        report->initial_seconds = secs(t1, t2);
-        report->uncoarsen_seconds = secs(t2, t3);
+        report->coarsen_seconds = secs(run.t0, run.t1);
+        report->initial_seconds = secs(run.t1, run.t2);
+        report->uncoarsen_seconds = secs(run.t2, t3);
         report->free_seconds = secs(t3, t4);
-        report->algorithm_seconds = secs(t0, t4);
+        report->algorithm_seconds = secs(run.t0, t4);
     }
     return partition;
 }
+
+template <typename Types>
+gpart::DeviceVector<std::int32_t> run_pipeline(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify,
+                                               int mode, gpart_partition_report* report) {
+    auto run = coarsen_phase(g, parts, ratio, seed, verify);
+    auto coarse = gpart::initial_partition_gpu<Types>(run.hierarchy.levels.back(), parts, ratio, seed,
+                                                      static_cast<gpart::SpectralMode>(mode));
+    note_initial(run, coarse);
+    return uncoarsen_phase(run, std::move(coarse), parts, ratio, verify, report);
+}
+
+// A group of graphs of one width: every graph is coarsened, the initial
+// partitions of all of them run in one persistent launch, then each graph is
+// uncoarsened. Each graph's labels are those run_pipeline gives it; its report
+// carries its own coarsening and uncoarsening time and the group's initial time
+// from its coarsening end (the hierarchies of the group are alive together).
+template <typename Types>
+std::vector<gpart::DeviceVector<std::int32_t>> run_group(const std::vector<GraphOf<Types>*>& graphs,
+                                                         const std::vector<int>& parts,
+                                                         const std::vector<double>& ratios,
+                                                         const std::vector<std::uint32_t>& seeds, int mode,
+                                                         const std::vector<gpart_partition_report*>& reports) {
+    std::vector<PipelineRun<Types>> runs;
+    runs.reserve(graphs.size());
+    for (std::size_t i = 0; i < graphs.size(); ++i)
+        runs.push_back(coarsen_phase(*graphs[i], parts[i], ratios[i], seeds[i], 0));
+    std::vector<const gpart::DeviceWeightedGraph<Types>*> coarsest;
+    for (auto& r : runs) coarsest.push_back(&r.hierarchy.levels.back());
+    auto coarse = gpart::initial_partition_group<Types>(coarsest, parts, ratios, seeds,
+                                                        static_cast<gpart::SpectralMode>(mode));
+    for (std::size_t i = 0; i < runs.size(); ++i) note_initial(runs[i], coarse[i]);
+    std::vector<gpart::DeviceVector<std::int32_t>> out;
+    for (std::size_t i = 0; i < runs.size(); ++i)
+        out.push_back(uncoarsen_phase(runs[i], std::move(coarse[i]), parts[i], ratios[i], 0, reports[i]));
+    return out;
+}
+
+std::atomic<int> batch_group{16};  // graphs per group of gpart_partition_batch
 
 template <typename Types>
 void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify, int mode, int32_t* labels,
@@ -466,18 +524,50 @@ int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* part
             std::vector<std::pair<int, gpart::DeviceVector<std::int32_t>>> done;
             try {
                 auto* ctx = gpart::detail::context();
+                const int group = spectral_mode == 0 ? 1 : std::max(1, batch_group.load());
                 for (;;) {
                     if (failed.load()) break;
+                    // up to `group` consecutive graphs of one width
                     const int slot = next.fetch_add(1);
                     if (slot >= count) break;
-                    const int i = order[static_cast<std::size_t>(slot)];
-                    auto part = std::visit(
-                        [&](auto& x) {
-                            return run_pipeline(x, parts[i], ratios[i], seeds[i], 0, spectral_mode,
-                                                reports ? reports + i : nullptr);
+                    std::vector<int> claim{order[static_cast<std::size_t>(slot)]};
+                    const int width = graphs[claim[0]]->width;
+                    while (static_cast<int>(claim.size()) < group) {
+                        int j = next.load();
+                        if (j >= count || graphs[order[static_cast<std::size_t>(j)]]->width != width) break;
+                        if (next.compare_exchange_strong(j, j + 1)) claim.push_back(order[static_cast<std::size_t>(j)]);
+                    }
+                    if (claim.size() == 1) {
+                        const int i = claim[0];
+                        auto part = std::visit(
+                            [&](auto& x) {
+                                return run_pipeline(x, parts[i], ratios[i], seeds[i], 0, spectral_mode,
+                                                    reports ? reports + i : nullptr);
+                            },
+                            graphs[i]->g);
+                        done.emplace_back(i, std::move(part));
+                        continue;
+                    }
+                    std::visit(
+                        [&](auto& first) {
+                            using G = std::decay_t<decltype(first)>;
+                            std::vector<G*> members;
+                            std::vector<int> p;
+                            std::vector<double> r;
+                            std::vector<std::uint32_t> sd;
+                            std::vector<gpart_partition_report*> rep;
+                            for (const int i : claim) {
+                                members.push_back(&std::get<G>(graphs[i]->g));
+                                p.push_back(parts[i]);
+                                r.push_back(ratios[i]);
+                                sd.push_back(seeds[i]);
+                                rep.push_back(reports ? reports + i : nullptr);
+                            }
+                            auto labels_of = run_group(members, p, r, sd, spectral_mode, rep);
+                            for (std::size_t c = 0; c < claim.size(); ++c)
+                                done.emplace_back(claim[c], std::move(labels_of[c]));
                         },
-                        graphs[i]->g);
-                    done.emplace_back(i, std::move(part));
+                        graphs[claim[0]]->g);
                 }
                 GPART_CHECK(gpart_ctx_synchronize(ctx));
             } catch (...) {
@@ -537,6 +627,15 @@ int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* part
         if (wall_seconds) *wall_seconds = std::chrono::duration<double>(wall_end - wall_start).count();
     });
 }
+
+int gpart_batch_set_group(int graphs) {
+    return guard([&] {
+        if (graphs < 1 || graphs > 4096) throw std::invalid_argument("batch group size out of range");
+        batch_group = graphs;
+    });
+}
+
+int gpart_batch_group(void) { return batch_group.load(); }
 
 int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ratio, int max_levels,
                         int cluster_size_factor, int pack_isolated, int verify, gpart_hierarchy** out) {

@@ -68,19 +68,39 @@ void plain_task(const DeviceWeightedGraph<Types>& graph, DeviceVector<std::int32
     children.push_back({task.begin, boundary, task.first_part, left_parts, left_weight});
 }
 
-}  // namespace
+// One graph's recursive bisection up to the persistent launch: validation, the
+// device arrays, and every task the queue does not take (queue_max: the largest
+// queue task, 0 for none). The tasks left for the queue are in `roots`.
+template <typename Types>
+struct Bisection {
+    const DeviceWeightedGraph<Types>* graph = nullptr;
+    int n = 0, parts = 0;
+    SpectralMode mode = SpectralMode::BatchedFp64;
+    std::uint32_t seed = 0;
+    U64 total = 0, capacity = 0;
+    DeviceVector<std::int32_t> labels, order;
+    DeviceVector<U64> vertex_weights;
+    DeviceVector<double> x, y, degree;
+    DeviceVector<U64> prefix;
+    std::vector<gpart_bisect_task> roots;
+    std::chrono::steady_clock::time_point start;
+};
 
 template <typename Types>
-DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeightedGraph<Types>& graph, int parts,
-                                                            double imbalance, std::uint32_t seed, SpectralMode mode,
-                                                            DeviceVector<double>* coords) {
-    const auto start = std::chrono::steady_clock::now();
+void bisect_outside_queue(Bisection<Types>& s, const DeviceWeightedGraph<Types>& graph, int parts, double imbalance,
+                          std::uint32_t seed, SpectralMode mode, int queue_max) {
+    s.start = std::chrono::steady_clock::now();
     auto* ctx = detail::context();
     const auto n64 = graph.vertices();
     if (n64 <= 0 || n64 > std::numeric_limits<int>::max() - 255 || parts < 2 || parts > n64 || parts > 32 ||
         !std::isfinite(imbalance) || imbalance < 1.0)
         throw std::invalid_argument("invalid GPU initial partition input");
     const int n = static_cast<int>(n64);
+    s.graph = &graph;
+    s.n = n;
+    s.parts = parts;
+    s.mode = mode;
+    s.seed = seed;
     const auto view = graph.view();
     U64 total = 0, max_weight = 0, min_weight = 0, edge_total = 0, edge_limited = 0;
     GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &total, &max_weight, &min_weight));
@@ -89,15 +109,25 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
         throw std::overflow_error("GPU initializer weight accumulation overflow");
     const U64 capacity = partition_capacity(total, parts, imbalance);
     if (max_weight > capacity) throw std::invalid_argument("a vertex exceeds partition capacity");
+    s.total = total;
+    s.capacity = capacity;
 
-    DeviceVector<std::int32_t> labels(static_cast<std::size_t>(n), 0);
-    DeviceVector<std::int32_t> order(static_cast<std::size_t>(n));
+    auto& labels = s.labels;
+    auto& order = s.order;
+    auto& vertex_weights = s.vertex_weights;
+    labels = DeviceVector<std::int32_t>(static_cast<std::size_t>(n), 0);
+    order = DeviceVector<std::int32_t>(static_cast<std::size_t>(n));
     GPART_CHECK(gpart_sequence_i32(ctx, order.data(), n));
-    DeviceVector<U64> vertex_weights(static_cast<std::size_t>(n));
+    vertex_weights = DeviceVector<U64>(static_cast<std::size_t>(n));
     GPART_CHECK(gpart_widen_weights(ctx, graph.vertex_weights.data(), Types::kWidth, n, vertex_weights.data()));
-    DeviceVector<double> x(static_cast<std::size_t>(n)), y(static_cast<std::size_t>(n)),
-        degree(static_cast<std::size_t>(n));
-    DeviceVector<U64> prefix(static_cast<std::size_t>(n));
+    s.x = DeviceVector<double>(static_cast<std::size_t>(n));
+    s.y = DeviceVector<double>(static_cast<std::size_t>(n));
+    s.degree = DeviceVector<double>(static_cast<std::size_t>(n));
+    s.prefix = DeviceVector<U64>(static_cast<std::size_t>(n));
+    auto& x = s.x;
+    auto& y = s.y;
+    auto& degree = s.degree;
+    auto& prefix = s.prefix;
 
     if (mode == SpectralMode::PlainFp64) {
         std::vector<gpart_bisect_task> tasks{{0, n, 0, parts, total}};
@@ -110,90 +140,140 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
             }
             plain_task(graph, labels, order, vertex_weights, task, seed, capacity, x, y, degree, prefix, tasks);
         }
-    } else {
-        const int precision = mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64;
-        const auto max_task = gpart_spectral_depth_max_task();
-        std::vector<gpart_bisect_task> depth{{0, n, 0, parts, total}};
-        if (gpart_spectral_queue_enabled()) {
-            // Tasks above the batched bound go through the plain form depth by depth;
-            // every smaller task, of any depth, is a root of one persistent launch.
-            std::vector<gpart_bisect_task> roots;
-            while (!depth.empty()) {
-                std::vector<gpart_bisect_task> next;
-                for (const auto& t : depth) {
-                    if (t.parts == 1) {
-                        if (t.weight > capacity) throw std::runtime_error("GPU initializer infeasible balance");
-                    } else if (t.count > max_task) {
-                        plain_task(graph, labels, order, vertex_weights, t, seed, capacity, x, y, degree, prefix, next);
-                    } else {
-                        roots.push_back(t);
-                    }
-                }
-                depth.swap(next);
-            }
-            gpart_queue_graph q{};
-            q.graph = view;
-            q.labels = labels.data();
-            q.order = order.data();
-            q.vertex_weights = vertex_weights.data();
-            q.coords = x.data();
-            q.capacity = capacity;
-            q.seed = seed;
-            const std::vector<std::int32_t> root_graph(roots.size(), 0);
-            GPART_CHECK(gpart_spectral_queue(ctx, &q, 1, roots.data(), root_graph.data(),
-                                             static_cast<std::int32_t>(roots.size()), precision, nullptr));
-            if (q.status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");
-            if (q.status == 2) throw std::runtime_error("GPU initializer infeasible balance");
-        }
-        DeviceVector<gpart_bisect_task> device_next(64);
-        while (!depth.empty()) {
-            std::vector<gpart_bisect_task> batched, next;
-            for (const auto& t : depth) {
-                if (t.parts == 1) {
-                    if (t.weight > capacity) throw std::runtime_error("GPU initializer infeasible balance");
-                } else if (t.count > max_task) {
-                    std::vector<gpart_bisect_task> children;
-                    plain_task(graph, labels, order, vertex_weights, t, seed, capacity, x, y, degree, prefix, children);
-                    next.insert(next.end(), children.begin(), children.end());
-                } else {
-                    batched.push_back(t);
-                }
-            }
-            if (!batched.empty()) {
-                std::int32_t produced = 0, status = 0;
-                GPART_CHECK(gpart_spectral_depth(ctx, &view, labels.data(), order.data(), vertex_weights.data(),
-                                                 batched.data(), static_cast<std::int32_t>(batched.size()), seed,
-                                                 capacity, precision, x.data(), device_next.data(), &produced,
-                                                 &status));
-                if (status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");
-                if (status == 2) throw std::runtime_error("GPU initializer infeasible balance");
-                std::vector<gpart_bisect_task> produced_tasks(static_cast<std::size_t>(produced));
-                if (produced)
-                    GPART_CHECK(gpart_memcpy_dtoh(ctx, produced_tasks.data(), device_next.data(),
-                                                  produced_tasks.size() * sizeof(gpart_bisect_task)));
-                next.insert(next.end(), produced_tasks.begin(), produced_tasks.end());
-            }
-            depth.swap(next);
-        }
+        return;
     }
+    const int precision = mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64;
+    const auto max_task = gpart_spectral_depth_max_task();
+    std::vector<gpart_bisect_task> depth{{0, n, 0, parts, total}};
+    DeviceVector<gpart_bisect_task> device_next(64);
+    while (!depth.empty()) {
+        std::vector<gpart_bisect_task> batched, next;
+        for (const auto& t : depth) {
+            if (t.parts == 1) {
+                if (t.weight > capacity) throw std::runtime_error("GPU initializer infeasible balance");
+            } else if (t.count > max_task) {
+                std::vector<gpart_bisect_task> children;
+                plain_task(graph, labels, order, vertex_weights, t, seed, capacity, x, y, degree, prefix, children);
+                next.insert(next.end(), children.begin(), children.end());
+            } else if (t.count <= queue_max) {
+                s.roots.push_back(t);
+            } else {
+                batched.push_back(t);
+            }
+        }
+        if (!batched.empty()) {
+            std::int32_t produced = 0, status = 0;
+            GPART_CHECK(gpart_spectral_depth(ctx, &view, labels.data(), order.data(), vertex_weights.data(),
+                                             batched.data(), static_cast<std::int32_t>(batched.size()), seed,
+                                             capacity, precision, x.data(), device_next.data(), &produced, &status));
+            if (status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");
+            if (status == 2) throw std::runtime_error("GPU initializer infeasible balance");
+            std::vector<gpart_bisect_task> produced_tasks(static_cast<std::size_t>(produced));
+            if (produced)
+                GPART_CHECK(gpart_memcpy_dtoh(ctx, produced_tasks.data(), device_next.data(),
+                                              produced_tasks.size() * sizeof(gpart_bisect_task)));
+            next.insert(next.end(), produced_tasks.begin(), produced_tasks.end());
+        }
+        depth.swap(next);
+    }
+}
+
+// One persistent launch over the roots of every graph of the group.
+template <typename Types>
+void run_queue(std::vector<Bisection<Types>>& group) {
+    std::vector<gpart_queue_graph> graphs;
+    std::vector<gpart_bisect_task> roots;
+    std::vector<std::int32_t> root_graph;
+    std::vector<std::size_t> member;
+    for (std::size_t i = 0; i < group.size(); ++i) {
+        auto& s = group[i];
+        if (s.roots.empty()) continue;
+        gpart_queue_graph q{};
+        q.graph = s.graph->view();
+        q.labels = s.labels.data();
+        q.order = s.order.data();
+        q.vertex_weights = s.vertex_weights.data();
+        q.coords = s.x.data();
+        q.capacity = s.capacity;
+        q.seed = s.seed;
+        for (const auto& t : s.roots) {
+            roots.push_back(t);
+            root_graph.push_back(static_cast<std::int32_t>(graphs.size()));
+        }
+        graphs.push_back(q);
+        member.push_back(i);
+    }
+    if (roots.empty()) return;
+    const int precision = group[member[0]].mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64;
+    for (std::size_t j = 0; j < member.size(); ++j)
+        if ((group[member[j]].mode == SpectralMode::BatchedFp32 ? GPART_SPECTRAL_FP32 : GPART_SPECTRAL_FP64) != precision)
+            throw std::invalid_argument("one queue launch runs one precision");
+    GPART_CHECK(gpart_spectral_queue(detail::context(), graphs.data(), static_cast<std::int32_t>(graphs.size()),
+                                     roots.data(), root_graph.data(), static_cast<std::int32_t>(roots.size()), precision,
+                                     nullptr));
+    for (const auto& q : graphs) {
+        if (q.status == 1) throw std::runtime_error("GPU spectral split cannot satisfy capacity");
+        if (q.status == 2) throw std::runtime_error("GPU initializer infeasible balance");
+    }
+}
+
+// Validation of the loads, the cut and the report lines.
+template <typename Types>
+DeviceVector<std::int32_t> finish_bisection(Bisection<Types>& s, DeviceVector<double>* coords) {
+    auto* ctx = detail::context();
+    const auto view = s.graph->view();
     // plain tasks leave their final coordinates in x; batched tasks write them there
-    if (coords) *coords = x;
-    DeviceVector<U64> loads(static_cast<std::size_t>(parts));
-    GPART_CHECK(gpart_part_weights(ctx, &view, labels.data(), parts, loads.data()));
+    if (coords) *coords = s.x;
+    DeviceVector<U64> loads(static_cast<std::size_t>(s.parts));
+    GPART_CHECK(gpart_part_weights(ctx, &view, s.labels.data(), s.parts, loads.data()));
     const auto host_loads = loads.to_host();
-    if (*std::max_element(host_loads.begin(), host_loads.end()) > capacity)
+    if (*std::max_element(host_loads.begin(), host_loads.end()) > s.capacity)
         throw std::runtime_error("GPU initializer balance validation failed");
     U64 cut = 0;
-    GPART_CHECK(gpart_cut(ctx, &view, labels.data(), &cut));
+    GPART_CHECK(gpart_cut(ctx, &view, s.labels.data(), &cut));
     static const char* names[] = {"gpu_spectral_plain_fp64", "gpu_spectral_batched_fp64", "gpu_spectral_batched_fp32"};
-    gpart::detail::log() << "initial_partition_backend=" << names[static_cast<int>(mode)] << '\n'
-              << "coarsest_vertices=" << n << '\n'
-              << "coarsest_edge_entries=" << graph.edges() << '\n'
-              << "initial_partition_parts=" << parts << '\n'
-              << "initial_partition_edgecut=" << cut << '\n'
-              << "initial_partition_seconds="
-              << std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count() << '\n';
-    return labels;
+    gpart::detail::log() << "initial_partition_backend=" << names[static_cast<int>(s.mode)] << '\n'
+                         << "coarsest_vertices=" << s.n << '\n'
+                         << "coarsest_edge_entries=" << s.graph->edges() << '\n'
+                         << "initial_partition_parts=" << s.parts << '\n'
+                         << "initial_partition_edgecut=" << cut << '\n'
+                         << "initial_partition_seconds="
+                         << std::chrono::duration<double>(std::chrono::steady_clock::now() - s.start).count() << '\n';
+    return std::move(s.labels);
+}
+
+}  // namespace
+
+template <typename Types>
+DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeightedGraph<Types>& graph, int parts,
+                                                            double imbalance, std::uint32_t seed, SpectralMode mode,
+                                                            DeviceVector<double>* coords) {
+    // Tasks of at most queue_max vertices, of any depth, are roots of one
+    // persistent launch; larger batched tasks run depth by depth.
+    std::int32_t own_blocks = 0, queue_max = 0;
+    gpart_spectral_queue_tuning(&own_blocks, &queue_max);
+    if (!gpart_spectral_queue_enabled()) queue_max = 0;
+    std::vector<Bisection<Types>> one(1);
+    bisect_outside_queue(one[0], graph, parts, imbalance, seed, mode, queue_max);
+    run_queue(one);
+    return finish_bisection(one[0], coords);
+}
+
+template <typename Types>
+std::vector<DeviceVector<typename Types::VertexT>> initial_partition_group(
+    const std::vector<const DeviceWeightedGraph<Types>*>& graphs, const std::vector<int>& parts,
+    const std::vector<double>& imbalance, const std::vector<std::uint32_t>& seeds, SpectralMode mode) {
+    if (parts.size() != graphs.size() || imbalance.size() != graphs.size() || seeds.size() != graphs.size())
+        throw std::invalid_argument("initial partition group arrays of different lengths");
+    const int queue_max = mode == SpectralMode::PlainFp64 ? 0 : static_cast<int>(gpart_spectral_depth_max_task());
+    std::vector<Bisection<Types>> group(graphs.size());
+    for (std::size_t i = 0; i < graphs.size(); ++i)
+        bisect_outside_queue(group[i], *graphs[i], parts[i], imbalance[i], seeds[i], mode, queue_max);
+    run_queue(group);
+    std::vector<DeviceVector<typename Types::VertexT>> out;
+    out.reserve(graphs.size());
+    for (auto& s : group) out.push_back(finish_bisection(s, nullptr));
+    return out;
 }
 
 template <typename Types>
@@ -301,6 +381,13 @@ template SpectralComparison compare_spectral_precision<StandardTypes>(const Devi
                                                                       double, std::uint32_t);
 template SpectralComparison compare_spectral_precision<BigTypes>(const DeviceWeightedGraph<BigTypes>&, int, double,
                                                                  std::uint32_t);
+
+template std::vector<DeviceVector<std::int32_t>> initial_partition_group<StandardTypes>(
+    const std::vector<const DeviceWeightedGraph<StandardTypes>*>&, const std::vector<int>&, const std::vector<double>&,
+    const std::vector<std::uint32_t>&, SpectralMode);
+template std::vector<DeviceVector<std::int32_t>> initial_partition_group<BigTypes>(
+    const std::vector<const DeviceWeightedGraph<BigTypes>*>&, const std::vector<int>&, const std::vector<double>&,
+    const std::vector<std::uint32_t>&, SpectralMode);
 
 template DeviceVector<std::int32_t> initial_partition_gpu<StandardTypes>(const DeviceWeightedGraph<StandardTypes>&, int,
                                                                          double, std::uint32_t, SpectralMode,

@@ -158,12 +158,19 @@ int gpart_graph_partition(gpart_graph* g, int parts, double ratio, uint32_t seed
  * allocator), are implemented but switched off until a host crash seen with four
  * of them on the full mix is explained. Each graph gets gpart_graph_partition's pipeline and options with
  * parts[i], ratios[i], seeds[i] and spectral_mode; a worker keeps its allocator
- * cache from one graph to the next. labels[i] (host, n_i entries) and reports may
- * be NULL. *wall_seconds runs from the first coarsening start to the moment the
- * last graph's labels are on the device (label copies come after). */
+ * cache from one graph to the next. With a batched spectral_mode, consecutive
+ * graphs of one width are taken in groups (gpart_batch_set_group, default 16):
+ * the group is coarsened, the initial partitions of all its graphs run in one
+ * persistent launch (gpart_spectral_queue), then each graph is uncoarsened; each
+ * graph's labels are those it gets alone. labels[i] (host, n_i entries) and
+ * reports may be NULL. *wall_seconds runs from the first coarsening start to the
+ * moment the last graph's labels are on the device (label copies come after). */
 int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* parts, const double* ratios,
                           const uint32_t* seeds, int spectral_mode, int workers, int32_t* const* labels,
                           gpart_partition_report* reports, double* wall_seconds);
+/* Graphs per group of gpart_partition_batch (1..4096; 1: one graph at a time). */
+int gpart_batch_set_group(int graphs);
+int gpart_batch_group(void);
 
 /* Coarsening hierarchy (host copy). */
 int gpart_graph_coarsen(gpart_graph* g, int parts, uint32_t seed, double stop_ratio, int max_levels,

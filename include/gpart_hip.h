@@ -363,7 +363,7 @@ int64_t gpart_spectral_depth_max_task(void);
  * gpart_spectral_depth_max_task() vertices: one launch serves the root tasks and
  * every task they produce, at every depth, of every graph of the call, from one
  * device work queue. A task of c vertices is served by ceil(ceil(c/8) / B)
- * workgroups (B: gpart_spectral_queue_tuning()). Labels, order and coordinates are
+ * workgroups (B: own_blocks of gpart_spectral_queue_tuning). Labels, order and coordinates are
  * bit-identical to gpart_spectral_depth run depth by depth on the same roots.
  * roots: host array; root_graph[r] names the graph of roots[r]; each root has at
  * least two parts and its vertices carry the label first_part. Per graph status
@@ -390,13 +390,16 @@ typedef struct gpart_queue_report {
 int gpart_spectral_queue(gpart_ctx* ctx, gpart_queue_graph* graphs, int32_t graph_count,
                          const gpart_bisect_task* roots, const int32_t* root_graph, int32_t root_count,
                          int precision, gpart_queue_report* report);
-/* Whether the host drivers use the persistent form for batched bisection (default
- * 1) or gpart_spectral_depth depth by depth (0). */
+/* Whether the single-graph host driver uses the persistent form for batched
+ * bisection (1) or gpart_spectral_depth depth by depth (0, the default: faster for
+ * one graph). Groups of graphs (gpart::initial_partition_group) always use it. */
 int gpart_spectral_set_queue(int32_t enabled);
 int32_t gpart_spectral_queue_enabled(void);
-/* Blocks of 8 vertices per queue workgroup in the team-size rule (8..128). */
-int gpart_spectral_queue_set_tuning(int32_t own_blocks);
-int32_t gpart_spectral_queue_tuning(void);
+/* own_blocks: blocks of 8 vertices per queue workgroup in the team-size rule
+ * (8..128, default 8); max_task: the single-graph driver's largest queue task
+ * (1..4096, default 4096), larger batched tasks run depth by depth. */
+int gpart_spectral_queue_set_tuning(int32_t own_blocks, int32_t max_task);
+void gpart_spectral_queue_tuning(int32_t* own_blocks, int32_t* max_task);
 
 /* ---------------- refinement (src/refine.cu, src/refine_incremental.cuh) ---------------- */
 

@@ -22,6 +22,16 @@ DeviceVector<typename Types::VertexT> initial_partition_gpu(const DeviceWeighted
                                                             SpectralMode mode = SpectralMode::BatchedFp64,
                                                             DeviceVector<double>* coords = nullptr);
 
+// The batched bisection of several graphs (one width, one mode): every task of
+// at most gpart_spectral_depth_max_task() vertices of every graph runs in one
+// persistent launch (gpart_spectral_queue); each graph's labels are those
+// initial_partition_gpu gives it alone. PlainFp64 runs the graphs one by one.
+template <typename Types>
+std::vector<DeviceVector<typename Types::VertexT>> initial_partition_group(
+    const std::vector<const DeviceWeightedGraph<Types>*>& graphs, const std::vector<int>& parts,
+    const std::vector<double>& imbalance_ratios, const std::vector<std::uint32_t>& seeds,
+    SpectralMode mode = SpectralMode::BatchedFp64);
+
 // FP32 against FP64 diffusion on one graph (no reference counterpart: the
 // reference has only FP64). Each recursion depth runs both precisions on the same
 // tasks, from the FP64 state of the depth before; the FP64 result carries on.

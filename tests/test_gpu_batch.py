@@ -83,3 +83,24 @@ def test_batch_error_is_reported(hx, small_mix):
     # the library stays usable
     _, reports, _ = hx.partition_batch(graphs[:2], [s["k"] for s in specs[:2]], workers=1, want_labels=False)
     assert len(reports) == 2
+
+
+def test_batch_groups_equal_one_at_a_time(hx, small_mix):
+    """Groups of graphs whose initial partitions share one persistent launch give
+    every graph the labels it gets alone, in both batched precisions."""
+    specs, graphs, _ = small_mix
+    args = ([s["k"] for s in specs], [s["ratio"] for s in specs], [s["seed"] for s in specs])
+    try:
+        for mode in ("batched_fp64", "batched_fp32"):
+            hx.set_batch_group(1)
+            say(f"case batch groups mode={mode} group=1")
+            alone, arep, _ = hx.partition_batch(graphs, *args, spectral_mode=mode)
+            for group in (5, 16):
+                hx.set_batch_group(group)
+                say(f"case batch groups mode={mode} group={group}")
+                together, trep, _ = hx.partition_batch(graphs, *args, spectral_mode=mode)
+                for s, a, t, ra, rt in zip(specs, alone, together, arep, trep):
+                    assert np.array_equal(a, t), (s, group)
+                    assert ra["final_cut"] == rt["final_cut"] and ra["initial_cut"] == rt["initial_cut"]
+    finally:
+        hx.set_batch_group(16)

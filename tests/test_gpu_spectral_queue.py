@@ -36,7 +36,7 @@ INPUTS = {
 @pytest.fixture
 def restore_queue(hx):
     yield
-    hx.set_spectral_queue(True, 64)
+    hx.set_spectral_queue(False, 8, 4096)
 
 
 @pytest.mark.parametrize("name", list(INPUTS))
@@ -49,11 +49,12 @@ def test_queue_matches_depth_form(hx, oracle, restore_queue, name, mode):
         say(f"case queue graph={name} mode={mode} k={k}")
         hx.set_spectral_queue(False)
         ref_labels, ref_coords = g.initial_partition(k, 1.03, 3, mode, with_coords=True)
-        for own_blocks in (64, 8, 128):
-            hx.set_spectral_queue(True, own_blocks)
+        # team sizes 1 to 64, and a queue that takes only the deeper tasks
+        for own_blocks, max_task in ((64, 4096), (8, 4096), (128, 4096), (16, 600)):
+            hx.set_spectral_queue(True, own_blocks, max_task)
             labels, coords = g.initial_partition(k, 1.03, 3, mode, with_coords=True)
-            assert np.array_equal(labels, ref_labels), (k, own_blocks)
-            assert np.array_equal(coords.view(np.uint64), ref_coords.view(np.uint64)), (k, own_blocks)
+            assert np.array_equal(labels, ref_labels), (k, own_blocks, max_task)
+            assert np.array_equal(coords.view(np.uint64), ref_coords.view(np.uint64)), (k, own_blocks, max_task)
         if mode == "batched_fp64":
             ol, ocoords, _ = oracle.initial_partition(og, k, 1.03, 3, with_coords=True)
             assert np.array_equal(ref_labels, ol), k
