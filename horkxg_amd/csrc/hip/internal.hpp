@@ -2,6 +2,7 @@
 #pragma once
 
 #include "../../../include/gpart_hip.h"
+#include "../tuned_gfx950.hpp"
 #include "block_cache.hpp"
 #include "common.hpp"
 
@@ -45,8 +46,8 @@ struct gpart_ctx {
     // Refinement rounds rebuild the affinity cache instead of updating it when the
     // level has at least rebuild_min_entries entries and the round's movers hold
     // more than 1/rebuild_share of them (share <= 0: never).
-    std::int64_t rebuild_min_entries = std::int64_t{1} << 23;
-    std::int64_t rebuild_share = 16;
+    std::int64_t rebuild_min_entries = gpart_tuned::kRebuildMinEntries;
+    std::int64_t rebuild_share = gpart_tuned::kRebuildShare;
     // The rebalance's bucket-counter replicas, kept between rounds: the bucket
     // scans zero every counter they read, so a completed round leaves them zero.
     void* rebalance_counters = nullptr;

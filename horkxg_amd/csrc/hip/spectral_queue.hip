@@ -54,9 +54,11 @@ constexpr int kMemberBits = 20;
 
 static_assert(kBatchThreads == 1024, "one member is 32 half-waves");
 
-std::atomic<int> queue_own_blocks{8};               // the team-size rule's blocks per member
-std::atomic<int> queue_max_task{kMaxBatchedTask};  // the single-graph driver's largest queue task
-std::atomic<int> queue_enabled{0};                 // the single-graph driver uses the queue
+// defaults from the autotuner's choice
+std::atomic<int> queue_own_blocks{gpart_tuned::kQueueOwnBlocks};  // the team-size rule's blocks per member
+std::atomic<int> queue_max_task{gpart_tuned::kQueueSingleMaxTask > 0 ? gpart_tuned::kQueueSingleMaxTask
+                                                                     : kMaxBatchedTask};
+std::atomic<int> queue_enabled{gpart_tuned::kQueueSingleMaxTask > 0 ? 1 : 0};  // single-graph driver uses it
 
 struct QueueGraph {
     const void* offsets;

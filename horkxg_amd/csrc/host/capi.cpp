@@ -11,6 +11,7 @@
 #include "partition_capacity.hpp"
 #include "quality.hpp"
 #include "uncoarsen.hpp"
+#include "../tuned_gfx950.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -231,7 +232,7 @@ std::vector<gpart::DeviceVector<std::int32_t>> run_group(const std::vector<Graph
     return out;
 }
 
-std::atomic<int> batch_group{16};  // graphs per group of gpart_partition_batch
+std::atomic<int> batch_group{gpart_tuned::kBatchGroup};  // graphs per group of gpart_partition_batch
 
 template <typename Types>
 void partition_impl(GraphOf<Types>& g, int parts, double ratio, uint32_t seed, int verify, int mode, int32_t* labels,

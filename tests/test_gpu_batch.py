@@ -90,6 +90,7 @@ def test_batch_groups_equal_one_at_a_time(hx, small_mix):
     every graph the labels it gets alone, in both batched precisions."""
     specs, graphs, _ = small_mix
     args = ([s["k"] for s in specs], [s["ratio"] for s in specs], [s["seed"] for s in specs])
+    saved = hx.batch_group()
     try:
         for mode in ("batched_fp64", "batched_fp32"):
             hx.set_batch_group(1)
@@ -103,4 +104,4 @@ def test_batch_groups_equal_one_at_a_time(hx, small_mix):
                     assert np.array_equal(a, t), (s, group)
                     assert ra["final_cut"] == rt["final_cut"] and ra["initial_cut"] == rt["initial_cut"]
     finally:
-        hx.set_batch_group(16)
+        hx.set_batch_group(saved)

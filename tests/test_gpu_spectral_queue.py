@@ -35,8 +35,9 @@ INPUTS = {
 
 @pytest.fixture
 def restore_queue(hx):
+    saved = hx.spectral_queue_settings()
     yield
-    hx.set_spectral_queue(False, 8, 4096)
+    hx.set_spectral_queue(saved["enabled"], saved["own_blocks"], saved["max_task"])
 
 
 @pytest.mark.parametrize("name", list(INPUTS))
