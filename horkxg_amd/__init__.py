@@ -39,6 +39,8 @@ from ._native import (  # noqa: F401
     set_log,
     set_refine_rebuild,
     set_refine_engine,
+    set_refine_graphs,
+    refine_graph_stats,
     set_spectral_queue,
     spectral_queue_settings,
     refine_keep_ratio,

@@ -137,6 +137,9 @@ def hip_lib():
         H.gpart_refine_set_engine.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int64]
         H.gpart_refine_keep_ratio.argtypes = [C.c_uint64, C.c_uint64]
         H.gpart_spectral_set_queue.argtypes = [C.c_int32]
+        H.gpart_refine_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        H.gpart_refine_graph_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                               C.POINTER(C.c_int64)]
         H.gpart_spectral_queue_set_tuning.argtypes = [C.c_int32, C.c_int32]
         H.gpart_spectral_queue_tuning.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         H.gpart_spectral_queue_tuning.restype = None
@@ -319,6 +322,18 @@ def spectral_queue_settings() -> dict:
     own, top = C.c_int32(), C.c_int32()
     H.gpart_spectral_queue_tuning(C.byref(own), C.byref(top))
     return {"enabled": bool(H.gpart_spectral_queue_enabled()), "own_blocks": own.value, "max_task": top.value}
+
+
+def set_refine_graphs(enabled: bool = True) -> None:
+    """The calling thread's refinement rounds recorded as hipGraphs and replayed
+    (default) or launched kernel by kernel. Results do not depend on it."""
+    _check(hip_lib().gpart_refine_set_graphs(_ctx(), int(bool(enabled))))
+
+
+def refine_graph_stats() -> dict:
+    plans, captures, replays = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(hip_lib().gpart_refine_graph_stats(_ctx(), C.byref(plans), C.byref(captures), C.byref(replays)))
+    return {"plans": plans.value, "captures": captures.value, "replays": replays.value}
 
 
 def refine_keep_ratio(current: int, best: int) -> bool:

@@ -16,6 +16,18 @@
 #include <string>
 #include <vector>
 
+namespace gpart_hip {
+// Scratch of a recorded launch sequence (the refinement's round graphs): while a
+// recorder is set on a context, ctx_alloc serves requests from it and ctx_free
+// leaves its blocks alone, so a replayed sequence finds its buffers at the
+// addresses it was captured with.
+struct ScratchRecorder {
+    virtual ~ScratchRecorder() = default;
+    virtual void* take(gpart_ctx* ctx, std::size_t bytes) = 0;
+    virtual bool owns(const void* p) const = 0;
+};
+}  // namespace gpart_hip
+
 struct gpart_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -33,7 +45,20 @@ struct gpart_ctx {
     // status. Saves the copy engine and the blocking wait of a read-back per round.
     std::uint64_t* status_host = nullptr;
     std::uint64_t* status_device = nullptr;  // the same memory, device address
+    // The status sequence: the kernels that publish a status increment a device
+    // word and publish its value (so a captured launch needs no new argument per
+    // call); the host counts the same publications in status_seq.
     std::uint64_t status_seq = 0;
+    unsigned long long* status_counter = nullptr;  // device word
+    gpart_hip::ScratchRecorder* recorder = nullptr;
+    // The refinement's recorded round graphs (refine.hip), dropped before the
+    // cache releases memory or the context ends; round_graphs switches them.
+    std::shared_ptr<void> round_plans;
+    bool round_graphs = true;
+    // The round graphs' scratch, apart from `cache` so that recording a round does
+    // not move the addresses the partition's own buffers get (a later partition of
+    // the same graph then finds its rounds recorded).
+    std::unique_ptr<gpart_hip::BlockCache> plan_cache;
     // Profiling per family.
     struct Profile {
         bool enabled = false;

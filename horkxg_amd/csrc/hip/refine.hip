@@ -25,6 +25,8 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <list>
+#include <unordered_set>
 
 namespace gpart_hip {
 namespace {
@@ -532,7 +534,7 @@ __global__ void __launch_bounds__(kBlock) commit_kernel(const RoundState* st, co
 // One wave: lane p < k holds part p's weight; every lane writes one status word.
 __global__ void __launch_bounds__(kWave) round_status_kernel(RoundState* st, const unsigned long long* part_weights,
                                                              int k, unsigned long long* status,
-                                                             unsigned long long seq) {
+                                                             unsigned long long* counter) {
     constexpr int kHead = static_cast<int>(offsetof(RoundState, pw) / 8);  // words before pw
     const int lane = threadIdx.x;
     const unsigned long long w = lane < k ? part_weights[lane] : 0;
@@ -549,7 +551,11 @@ __global__ void __launch_bounds__(kWave) round_status_kernel(RoundState* st, con
     if (lane < kHead + k) status[1 + lane] = words[lane];
     __threadfence_system();
     __syncthreads();
-    if (lane == 0) __hip_atomic_store(status, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (lane == 0) {
+        const unsigned long long seq = *counter + 1;
+        *counter = seq;
+        __hip_atomic_store(status, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 static_assert(sizeof(RoundState) % 8 == 0 && sizeof(RoundState) / 8 + 1 <= kStatusWords, "status words");
 static_assert(offsetof(RoundState, pw) / 8 + 32 <= static_cast<std::size_t>(kWave), "one wave writes the status");
@@ -1026,6 +1032,231 @@ __global__ void __launch_bounds__(kBlock) balance_destination_ranks_kernel(
     }
 }
 
+// ---------------------------------------------------------------- round graphs
+//
+// A refinement round (an LP round or a rebalance iteration) is a fixed sequence
+// of launches for a given level and set of arrays, repeated tens of times per
+// level, each followed by a host wait for its status. The first call of a
+// sequence captures its launches into hipGraphs; later calls with the same key
+// (every pointer, size and setting the launches depend on) replay them, which
+// submits a round at once instead of launch by launch (on MI355X 2.0-2.6 us per
+// kernel instead of 2.8-4.3 us with a host wait every 20 kernels). The round's
+// scratch comes from its plan, whose blocks keep their addresses from call to
+// call. A family timed with events runs outside the graphs (events recorded in a
+// capture carry no timing), which splits the sequence into segments.
+constexpr std::size_t kMaxRoundPlans = 64;
+constexpr std::int64_t kMaxGraphVertices = std::int64_t{1} << 23;  // larger levels are not launch-bound
+
+BlockCache* plan_cache(gpart_ctx* ctx) {
+    if (!ctx->plan_cache) {
+        SegmentSource source;
+        source.allocate = [](std::size_t bytes) -> void* {
+            void* q = nullptr;
+            const hipError_t e = hipMalloc(&q, bytes);
+            if (e == hipErrorOutOfMemory) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+            GPART_HIP_CHECK(e);
+            return q;
+        };
+        source.release = [](void* q) { GPART_HIP_CHECK(hipFree(q)); };
+        source.drain = [ctx] { GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream)); };
+        ctx->plan_cache = std::make_unique<BlockCache>(std::move(source));
+    }
+    return ctx->plan_cache.get();
+}
+
+struct RoundPlan final : ScratchRecorder {
+    explicit RoundPlan(gpart_ctx* c, std::vector<std::uint64_t> k) : ctx(c), key(std::move(k)) {}
+    ~RoundPlan() override {
+        for (auto g : segments) (void)hipGraphExecDestroy(g);
+        for (auto& list : blocks)
+            for (auto& b : list) ctx->plan_cache->free(b.first);
+    }
+    void* take(gpart_ctx*, std::size_t bytes) override {
+        if (blocks.size() <= static_cast<std::size_t>(phase)) {
+            blocks.resize(phase + 1);
+            cursor.resize(phase + 1, 0);
+        }
+        auto& list = blocks[phase];
+        std::size_t& at = cursor[phase];
+        if (!recorded) {
+            void* p = nullptr;
+            try {
+                p = plan_cache(ctx)->allocate(bytes);
+            } catch (const std::runtime_error& e) {
+                if (dynamic_cast<const Error*>(&e)) throw;
+                throw Error(GPART_ERR_RUNTIME, e.what());
+            }
+            list.emplace_back(p, bytes);
+            owned.insert(p);
+            ++at;
+            return p;
+        }
+        if (at >= list.size() || list[at].second != bytes)
+            throw Error(GPART_ERR_RUNTIME, "a recorded refinement round asked for different scratch");
+        return list[at++].first;
+    }
+    bool owns(const void* p) const override { return owned.count(p) != 0; }
+
+    gpart_ctx* ctx;
+    std::vector<std::uint64_t> key;
+    std::vector<std::vector<std::pair<void*, std::size_t>>> blocks;  // per phase; 0: the round's body
+    std::vector<std::size_t> cursor;
+    std::unordered_set<const void*> owned;
+    std::vector<hipGraphExec_t> segments;
+    bool recorded = false;  // a first call has captured the segments
+    bool broken = false;    // a capture failed: the plan is dropped
+    int phase = 0;
+    long long calls = 0, captures = 0, replays = 0;
+};
+
+struct RoundPlans {
+    gpart_ctx* ctx = nullptr;
+    std::list<std::unique_ptr<RoundPlan>> plans;  // most recently used first
+    long long captures = 0, replays = 0;
+    ~RoundPlans() {
+        if (ctx && !plans.empty()) (void)hipStreamSynchronize(ctx->stream);
+    }
+};
+
+RoundPlans& round_plans(gpart_ctx* ctx) {
+    if (!ctx->round_plans) {
+        auto* set = new RoundPlans;
+        set->ctx = ctx;
+        ctx->round_plans = std::shared_ptr<void>(set, [](void* q) { delete static_cast<RoundPlans*>(q); });
+    }
+    return *static_cast<RoundPlans*>(ctx->round_plans.get());
+}
+
+// The plan of a round with this key, or nullptr (launch directly) when the
+// context does not use round graphs, the level is large, or the key is new: a
+// round is recorded on its second call, so a sequence that runs once never pays
+// for a capture.
+RoundPlan* round_plan(gpart_ctx* ctx, std::int64_t n, std::vector<std::uint64_t> key) {
+    if (!ctx->round_graphs || n > kMaxGraphVertices) return nullptr;
+    for (int f = 0; f < GPART_PROFILE_FAMILIES; ++f) key.push_back(ctx->profile[f].enabled ? 1 : 0);
+    auto& set = round_plans(ctx);
+    for (auto it = set.plans.begin(); it != set.plans.end(); ++it) {
+        if ((*it)->key != key) continue;
+        if ((*it)->broken) {
+            set.plans.erase(it);
+            break;
+        }
+        set.plans.splice(set.plans.begin(), set.plans, it);
+        RoundPlan* plan = set.plans.front().get();
+        return ++plan->calls >= 2 ? plan : nullptr;
+    }
+    if (set.plans.size() >= kMaxRoundPlans) {
+        GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // its graphs may still run
+        set.plans.pop_back();
+    }
+    set.plans.push_front(std::make_unique<RoundPlan>(ctx, std::move(key)));
+    set.plans.front()->calls = 1;
+    return nullptr;
+}
+
+// One call of a round: phases run their launches into the plan's graphs (first
+// call) or replay them (later calls). Scratch of the round's body is taken
+// before, between or after phases in the same order every call.
+class RoundRun {
+public:
+    RoundRun(gpart_ctx* ctx, RoundPlan* plan) : ctx_(ctx), plan_(plan) {
+        if (!plan_) return;
+        ctx_->recorder = plan_;
+        plan_->phase = 0;
+        std::fill(plan_->cursor.begin(), plan_->cursor.end(), 0);
+        if (plan_->recorded) {
+            ++plan_->replays;
+            ++round_plans(ctx_).replays;
+        } else {
+            ++plan_->captures;
+            ++round_plans(ctx_).captures;
+        }
+    }
+    ~RoundRun() {
+        if (!plan_) return;
+        if (capturing_) {  // an exception left a capture open
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx_->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            plan_->broken = true;
+        }
+        ctx_->recorder = nullptr;
+    }
+    RoundRun(const RoundRun&) = delete;
+    RoundRun& operator=(const RoundRun&) = delete;
+
+    // The launches of one phase; family >= 0: the profile family they report to
+    // (run outside the graphs while that family is timed).
+    template <typename F>
+    void phase(int family, F&& launches) {
+        if (!plan_) {
+            launches();
+            return;
+        }
+        plan_->phase = ++phases_;
+        if (plan_->cursor.size() > static_cast<std::size_t>(phases_)) plan_->cursor[phases_] = 0;
+        const bool direct = family >= 0 && ctx_->profile[family].enabled;
+        if (direct) {
+            close_segment();
+            in_segment_ = false;
+            launches();
+        } else if (!plan_->recorded) {
+            if (!capturing_) {
+                GPART_HIP_CHECK(hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeRelaxed));
+                capturing_ = true;
+            }
+            launches();
+        } else if (!in_segment_) {
+            if (segment_ >= plan_->segments.size())
+                throw Error(GPART_ERR_RUNTIME, "a recorded refinement round has fewer segments than its calls");
+            GPART_HIP_CHECK(hipGraphLaunch(plan_->segments[segment_++], ctx_->stream));
+            in_segment_ = true;
+        }
+        plan_->phase = 0;
+    }
+    // After the last phase.
+    void end() {
+        if (!plan_) return;
+        close_segment();
+        plan_->recorded = true;
+    }
+
+private:
+    // First call: the open capture becomes a segment and runs now.
+    void close_segment() {
+        if (!capturing_) return;
+        hipGraph_t graph = nullptr;
+        capturing_ = false;
+        GPART_HIP_CHECK(hipStreamEndCapture(ctx_->stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            plan_->broken = true;
+            GPART_HIP_CHECK(e);
+        }
+        plan_->segments.push_back(exec);
+        GPART_HIP_CHECK(hipGraphLaunch(exec, ctx_->stream));
+    }
+
+    gpart_ctx* ctx_;
+    RoundPlan* plan_;
+    int phases_ = 0;
+    bool capturing_ = false;
+    bool in_segment_ = false;
+    std::size_t segment_ = 0;
+};
+
+inline std::uint64_t key_of(const void* p) { return reinterpret_cast<std::uintptr_t>(p); }
+inline std::uint64_t key_of(double v) {
+    std::uint64_t b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+}
+
 // ---------------------------------------------------------------- drivers
 
 template <typename O, typename W, typename C = unsigned long long>
@@ -1143,8 +1374,8 @@ struct Ops {
     // and a full rebuild of the cache after the commit (one gather per entry, no
     // atomics, same cache); the cut delta is summed from the movers' entries
     // either way.
-    static void apply(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_degree_classes* c, RoundState* st,
-                      const std::int32_t* light, const std::int32_t* heavy, std::int32_t* labels,
+    static void apply(gpart_ctx* ctx, RoundRun& run, const gpart_csr_view* g, const gpart_degree_classes* c,
+                      RoundState* st, const std::int32_t* light, const std::int32_t* heavy, std::int32_t* labels,
                       const std::int32_t* targets, const std::int32_t* flags, int k, C* cache,
                       unsigned long long* part_weights, std::int32_t* locks, bool lp) {
         const std::int64_t n = g->n;
@@ -1152,6 +1383,7 @@ struct Ops {
         const auto* w = static_cast<const W*>(g->edge_weights);
         const long long rebuild_above = rebuild_threshold(ctx, g);
         const auto light_blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * 8), 4 * kRoundBlocks));
+        run.phase(GPART_PROFILE_CACHE_UPDATE, [&] {
         profile_begin(ctx, GPART_PROFILE_CACHE_UPDATE);
         hipLaunchKernelGGL((update_cache_kernel<O, W, 8, C>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
                            &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta,
@@ -1173,12 +1405,16 @@ struct Ops {
                                flags, k, cache, &st->delta, rebuild_above);
         }
         profile_end(ctx, GPART_PROFILE_CACHE_UPDATE, 0.0);
-        hipLaunchKernelGGL(commit_kernel<W>,
-                           dim3(static_cast<unsigned>(std::min<std::int64_t>(grid_for(n), kRoundBlocks))),
-                           dim3(kBlock), 0, ctx->stream, st, light, heavy, labels, targets,
-                           static_cast<const W*>(g->vertex_weights), part_weights, locks, lp, k);
-        GPART_LAUNCH_CHECK();
-        if (rebuild_above >= 0) build_cache(ctx, g, c, labels, k, cache, &st->edges, rebuild_above, &st->cut_directed);
+        });
+        run.phase(-1, [&] {
+            hipLaunchKernelGGL(commit_kernel<W>,
+                               dim3(static_cast<unsigned>(std::min<std::int64_t>(grid_for(n), kRoundBlocks))),
+                               dim3(kBlock), 0, ctx->stream, st, light, heavy, labels, targets,
+                               static_cast<const W*>(g->vertex_weights), part_weights, locks, lp, k);
+            GPART_LAUNCH_CHECK();
+            if (rebuild_above >= 0)
+                build_cache(ctx, g, c, labels, k, cache, &st->edges, rebuild_above, &st->cut_directed);
+        });
     }
 
     // Mover entries above which a round rebuilds the cache; -1: never.
@@ -1186,13 +1422,17 @@ struct Ops {
         return ctx->rebuild_share > 0 && g->m >= ctx->rebuild_min_entries ? g->m / ctx->rebuild_share : -1;
     }
 
-    static void finish(gpart_ctx* ctx, const gpart_csr_view* g, RoundState* st, const unsigned long long* part_weights,
-                       int k,
-                       gpart_refine_status* out) {
+    // The round's status into the fine-grained status words (the last phase), then
+    // the host's wait and reading of it.
+    static void finish(gpart_ctx* ctx, RoundRun& run, const gpart_csr_view* g, RoundState* st,
+                       const unsigned long long* part_weights, int k, gpart_refine_status* out) {
+        run.phase(-1, [&] {
+            hipLaunchKernelGGL(round_status_kernel, dim3(1), dim3(kWave), 0, ctx->stream, st, part_weights, k,
+                               reinterpret_cast<unsigned long long*>(ctx->status_device), ctx->status_counter);
+            GPART_LAUNCH_CHECK();
+        });
+        run.end();
         const std::uint64_t seq = ++ctx->status_seq;
-        hipLaunchKernelGGL(round_status_kernel, dim3(1), dim3(kWave), 0, ctx->stream, st, part_weights, k,
-                           reinterpret_cast<unsigned long long*>(ctx->status_device), seq);
-        GPART_LAUNCH_CHECK();
         wait_status(ctx, seq);
         RoundState h{};
         std::memcpy(&h, ctx->status_host + 1, offsetof(RoundState, pw) + 8 * static_cast<std::size_t>(k));
@@ -1227,26 +1467,52 @@ struct Ops {
                          unsigned long long* part_weights, gpart_refine_status* out) {
         const std::int64_t n = g->n;
         const auto* off = static_cast<const O*>(g->offsets);
+        RoundRun run(ctx, round_plan(ctx, n, round_key(ctx, g, c, cache, labels, k, targets, flags, part_weights,
+                                                       {0, key_of(filter), key_of(locks), key_of(gains)})));
         // the proposals kernel zeroes the round state
         Scratch<RoundState> st(ctx, 1);
-        const int G = group_for_k(k);
-        profile_begin(ctx, GPART_PROFILE_REFINE_PROPOSALS);
-        const unsigned blocks = std::max<unsigned>(1, grid_for(n * G));
-        switch (G) {
-            case 1: hipLaunchKernelGGL((lp_proposals_kernel<1, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
-            case 2: hipLaunchKernelGGL((lp_proposals_kernel<2, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
-            default: hipLaunchKernelGGL((lp_proposals_kernel<4, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
-        }
-        GPART_LAUNCH_CHECK();
-        profile_end(ctx, GPART_PROFILE_REFINE_PROPOSALS, static_cast<double>(n) * (20.0 + sizeof(C) * k));
-        filter_movers(ctx, g, c, labels, targets, gains, flags);
         Scratch<std::int32_t> light(ctx, n), heavy(ctx, n);
-        device_scan<MoverPair, MoverPairOp>(ctx, MoverLoad<O>{flags, off}, MoverStore{light.get(), heavy.get(), n,
-                                                                                      st.get()},
-                                            n, MoverPairOp{});
-        apply(ctx, g, c, st.get(), light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, locks,
-              true);
-        finish(ctx, g, st.get(), part_weights, k, out);
+        const int G = group_for_k(k);
+        run.phase(GPART_PROFILE_REFINE_PROPOSALS, [&] {
+            profile_begin(ctx, GPART_PROFILE_REFINE_PROPOSALS);
+            const unsigned blocks = std::max<unsigned>(1, grid_for(n * G));
+            switch (G) {
+                case 1: hipLaunchKernelGGL((lp_proposals_kernel<1, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
+                case 2: hipLaunchKernelGGL((lp_proposals_kernel<2, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
+                default: hipLaunchKernelGGL((lp_proposals_kernel<4, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, n, cache, labels, k, filter, locks, targets, gains, st.get()); break;
+            }
+            GPART_LAUNCH_CHECK();
+            profile_end(ctx, GPART_PROFILE_REFINE_PROPOSALS, static_cast<double>(n) * (20.0 + sizeof(C) * k));
+        });
+        run.phase(GPART_PROFILE_REFINE_FILTER, [&] { filter_movers(ctx, g, c, labels, targets, gains, flags); });
+        run.phase(-1, [&] {
+            device_scan<MoverPair, MoverPairOp>(ctx, MoverLoad<O>{flags, off},
+                                                MoverStore{light.get(), heavy.get(), n, st.get()}, n, MoverPairOp{});
+        });
+        apply(ctx, run, g, c, st.get(), light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights,
+              locks, true);
+        finish(ctx, run, g, st.get(), part_weights, k, out);
+    }
+
+    // Everything a round's launches depend on besides its own scratch.
+    static std::vector<std::uint64_t> round_key(const gpart_ctx* ctx, const gpart_csr_view* g,
+                                                const gpart_degree_classes* c, const C* cache,
+                                                const std::int32_t* labels, int k, const std::int32_t* targets,
+                                                const std::int32_t* flags, const unsigned long long* part_weights,
+                                                std::initializer_list<std::uint64_t> extra) {
+        std::vector<std::uint64_t> key{sizeof(O), sizeof(C), static_cast<std::uint64_t>(g->n),
+                                       static_cast<std::uint64_t>(g->m), key_of(g->offsets), key_of(g->neighbors),
+                                       key_of(g->edge_weights), key_of(g->vertex_weights), key_of(c->low),
+                                       key_of(c->medium), key_of(c->high), static_cast<std::uint64_t>(c->low_count),
+                                       static_cast<std::uint64_t>(c->medium_count),
+                                       static_cast<std::uint64_t>(c->high_count),
+                                       static_cast<std::uint64_t>(c->high_edges),
+                                       static_cast<std::uint64_t>(c->max_degree), key_of(cache), key_of(labels),
+                                       static_cast<std::uint64_t>(k), key_of(targets), key_of(flags),
+                                       key_of(part_weights), static_cast<std::uint64_t>(rebuild_threshold(ctx, g)),
+                                       key_of(ctx->status_device), key_of(ctx->status_counter)};
+        key.insert(key.end(), extra.begin(), extra.end());
+        return key;
     }
 
     // One rebalance iteration: proposals, source and destination bucket quotas,
@@ -1274,6 +1540,8 @@ struct Ops {
         }
         if (!ctx->rebalance_counters_zero) fill_bytes(ctx, ctx->rebalance_counters, 0, ctx->rebalance_counter_bytes);
         ctx->rebalance_counters_zero = false;  // until this round completes
+        RoundRun run(ctx, round_plan(ctx, n, round_key(ctx, g, c, cache, labels, k, targets, flags, part_weights,
+                                                       {1, capacity, key_of(ctx->rebalance_counters)})));
         Scratch<RoundState> st(ctx, 1);
         Scratch<unsigned long long> counts(ctx, 2 * width), offsets(ctx, 2 * width), thresholds(ctx, 64);
         auto* st_ptr = st.get();
@@ -1290,11 +1558,14 @@ struct Ops {
         Scratch<unsigned long long> sums(ctx, static_cast<std::int64_t>(slices) * 32),
             prefix(ctx, static_cast<std::int64_t>(slices) * 32);
 
-        device_scan<long long, SumOp<long long>>(ctx, OverloadedLoad{labels, part_weights, capacity},
-                                                 CandidateStore{cand.get(), flags, n, st_ptr}, n,
-                                                 SumOp<long long>{});
+        run.phase(-1, [&] {
+            device_scan<long long, SumOp<long long>>(ctx, OverloadedLoad{labels, part_weights, capacity},
+                                                     CandidateStore{cand.get(), flags, n, st_ptr}, n,
+                                                     SumOp<long long>{});
+        });
         const int G = group_for_k(k);
         const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), 4 * kRoundBlocks));
+        run.phase(GPART_PROFILE_REBALANCE, [&] {
         profile_begin(ctx, GPART_PROFILE_REBALANCE);
 #define GPART_BALANCE_PROPOSALS(GG)                                                                              \
     hipLaunchKernelGGL((balance_proposals_kernel<W, GG, C>), dim3(blocks), dim3(kBlock), 0, ctx->stream, st_ptr,   \
@@ -1308,6 +1579,8 @@ struct Ops {
 #undef GPART_BALANCE_PROPOSALS
         GPART_LAUNCH_CHECK();
         profile_end(ctx, GPART_PROFILE_REBALANCE, 0.0);
+        });
+        run.phase(-1, [&] {
         hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, src_replicas, k,
                            src_counts, src_offsets, part_weights, capacity, false, src_thresholds);
         hipLaunchKernelGGL(balance_sources_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st_ptr,
@@ -1329,9 +1602,10 @@ struct Ops {
                            st_ptr, cand.get(), dbucket.get(), vw, off, dst_offsets, dst_thresholds, prefix.get(),
                            light.get(), heavy.get(), flags);
         GPART_LAUNCH_CHECK();
-        apply(ctx, g, c, st_ptr, light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights, nullptr,
-              false);
-        finish(ctx, g, st_ptr, part_weights, k, out);
+        });
+        apply(ctx, run, g, c, st_ptr, light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights,
+              nullptr, false);
+        finish(ctx, run, g, st_ptr, part_weights, k, out);
         ctx->rebalance_counters_zero = true;
         // per candidate: id, label, weight, cache row, target and bucket writes
         profile_add_bytes(ctx, GPART_PROFILE_REBALANCE,
@@ -1500,6 +1774,25 @@ int gpart_refine_lp_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_d
                              OpsT::lp_round(ctx, g, classes, static_cast<CacheT*>(cache), labels, k, filter, locks,
                                             targets, reinterpret_cast<long long*>(gains), flags,
                                             reinterpret_cast<unsigned long long*>(part_weights), status));
+    });
+}
+
+int gpart_refine_set_graphs(gpart_ctx* ctx, int enabled) {
+    return guarded([&] {
+        GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->round_plans.reset();
+        ctx->round_graphs = enabled != 0;
+    });
+}
+
+int gpart_refine_graphs_enabled(gpart_ctx* ctx) { return ctx && ctx->round_graphs ? 1 : 0; }
+
+int gpart_refine_graph_stats(gpart_ctx* ctx, int64_t* plans, int64_t* captures, int64_t* replays) {
+    return guarded([&] {
+        const RoundPlans* set = ctx->round_plans ? static_cast<const RoundPlans*>(ctx->round_plans.get()) : nullptr;
+        if (plans) *plans = set ? static_cast<int64_t>(set->plans.size()) : 0;
+        if (captures) *captures = set ? set->captures : 0;
+        if (replays) *replays = set ? set->replays : 0;
     });
 }
 

@@ -56,7 +56,7 @@ __global__ void fill_u8_kernel(std::uint8_t* dst, std::uint8_t value, std::int64
 
 // Copies bytes into status words 1.. (host-visible), then stores seq in word 0.
 __global__ void __launch_bounds__(kWave) status_copy_kernel(const std::uint8_t* src, std::int64_t bytes,
-                                                            unsigned long long* status, unsigned long long seq) {
+                                                            unsigned long long* status, unsigned long long* counter) {
     auto* dst = reinterpret_cast<std::uint8_t*>(status + 1);
     if ((reinterpret_cast<std::uintptr_t>(src) & 7) == 0) {
         const auto* s8 = reinterpret_cast<const unsigned long long*>(src);
@@ -68,7 +68,11 @@ __global__ void __launch_bounds__(kWave) status_copy_kernel(const std::uint8_t* 
     }
     __threadfence_system();
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(status, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0) {
+        const unsigned long long seq = *counter + 1;
+        *counter = seq;
+        __hip_atomic_store(status, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 __global__ void sequence_kernel(std::int32_t* out, std::int64_t n) {
@@ -98,8 +102,10 @@ void release_blocks(gpart_ctx* ctx) {
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
     GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->round_plans.reset();  // their blocks become free and releasable
     const auto t1 = clock::now();
     ctx->released_blocks = ctx->cache->release_free_segments();
+    if (ctx->plan_cache) ctx->released_blocks += ctx->plan_cache->release_free_segments();
     const auto t2 = clock::now();
     ctx->release_seconds[0] = 0.0;
     ctx->release_seconds[1] = std::chrono::duration<double>(t1 - t0).count();
@@ -107,6 +113,7 @@ void release_blocks(gpart_ctx* ctx) {
 }
 
 void* ctx_alloc(gpart_ctx* ctx, std::size_t bytes) {
+    if (ctx->recorder) return ctx->recorder->take(ctx, bytes);
     try {
         return ctx->cache->allocate(bytes);
     } catch (const std::runtime_error& e) {
@@ -117,6 +124,7 @@ void* ctx_alloc(gpart_ctx* ctx, std::size_t bytes) {
 
 void ctx_free(gpart_ctx* ctx, void* p) {
     if (!p) return;
+    if (ctx->recorder && ctx->recorder->owns(p)) return;
     try {
         ctx->cache->free(p);
     } catch (const std::invalid_argument& e) {
@@ -147,7 +155,7 @@ void read_back(gpart_ctx* ctx, void* host, const void* device, std::size_t bytes
         const std::uint64_t seq = ++ctx->status_seq;
         hipLaunchKernelGGL(status_copy_kernel, dim3(1), dim3(kWave), 0, ctx->stream,
                            static_cast<const std::uint8_t*>(device), static_cast<std::int64_t>(bytes),
-                           reinterpret_cast<unsigned long long*>(ctx->status_device), seq);
+                           reinterpret_cast<unsigned long long*>(ctx->status_device), ctx->status_counter);
         GPART_LAUNCH_CHECK();
         wait_status(ctx, seq);
         std::memcpy(host, ctx->status_host + 1, bytes);
@@ -259,6 +267,8 @@ int gpart_ctx_create(int device, gpart_ctx** out) {
                                           hipHostMallocCoherent | hipHostMallocMapped));
             GPART_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->status_device), ctx->status_host, 0));
             std::memset(ctx->status_host, 0, kStatusWords * 8);
+            GPART_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&ctx->status_counter), sizeof(unsigned long long)));
+            GPART_HIP_CHECK(hipMemset(ctx->status_counter, 0, sizeof(unsigned long long)));
             SegmentSource source;
             source.allocate = [ctx](std::size_t bytes) -> void* {
                 void* p = nullptr;
@@ -276,6 +286,7 @@ int gpart_ctx_create(int device, gpart_ctx** out) {
         } catch (...) {
             if (ctx->pinned) (void)hipHostFree(ctx->pinned);
             if (ctx->status_host) (void)hipHostFree(ctx->status_host);
+            if (ctx->status_counter) (void)hipFree(ctx->status_counter);
             if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
             delete ctx;
             throw;
@@ -289,9 +300,14 @@ int gpart_ctx_destroy(gpart_ctx* ctx) {
     return guarded([&] {
         GPART_HIP_CHECK(hipSetDevice(ctx->device));
         GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->round_plans.reset();
         // Every segment goes, including those of blocks the caller never freed.
         for (const auto& kv : ctx->cache->segments()) GPART_HIP_CHECK(hipFree(kv.first));
         ctx->cache.reset();
+        if (ctx->plan_cache) {
+            for (const auto& kv : ctx->plan_cache->segments()) GPART_HIP_CHECK(hipFree(kv.first));
+            ctx->plan_cache.reset();
+        }
         for (auto& p : ctx->profile)
             for (auto& e : p.events) {
                 GPART_HIP_CHECK(hipEventDestroy(e.first));
@@ -300,6 +316,7 @@ int gpart_ctx_destroy(gpart_ctx* ctx) {
         for (auto e : ctx->spare_events) GPART_HIP_CHECK(hipEventDestroy(e));
         if (ctx->pinned) GPART_HIP_CHECK(hipHostFree(ctx->pinned));
         if (ctx->status_host) GPART_HIP_CHECK(hipHostFree(ctx->status_host));
+        if (ctx->status_counter) GPART_HIP_CHECK(hipFree(ctx->status_counter));
         GPART_HIP_CHECK(hipStreamDestroy(ctx->stream));
         delete ctx;
     });
@@ -326,6 +343,14 @@ int gpart_cache_info(gpart_ctx* ctx, gpart_cache_info_t* out) {
         out->free_bytes = ctx->cache->free_bytes();
         out->peak_reserved_bytes = st.peak_reserved_bytes;
         out->peak_live_bytes = st.peak_live_bytes;
+        if (ctx->plan_cache) {  // the round graphs' scratch (their peaks add as an upper bound)
+            const auto ps = ctx->plan_cache->stats();
+            out->reserved_bytes += ps.reserved_bytes;
+            out->live_bytes += ps.live_bytes;
+            out->free_bytes += ctx->plan_cache->free_bytes();
+            out->peak_reserved_bytes += ps.peak_reserved_bytes;
+            out->peak_live_bytes += ps.peak_live_bytes;
+        }
         out->segments = st.segments;
         out->live_blocks = st.live_blocks;
         out->free_blocks = ctx->cache->free_blocks();

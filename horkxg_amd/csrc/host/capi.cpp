@@ -507,6 +507,16 @@ int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* part
                    gpart_graph_edges(graphs[b]) + gpart_graph_vertices(graphs[b]);
         });
         GPART_CHECK(gpart_ctx_synchronize(gpart::detail::context()));  // uploads complete
+        // Round graphs pay for a recording per level of every graph and are replayed
+        // only within that graph here: the batch launches rounds directly.
+        auto* caller_ctx = gpart::detail::context();
+        const int graphs_were = gpart_refine_graphs_enabled(caller_ctx);
+        GPART_CHECK(gpart_refine_set_graphs(caller_ctx, 0));
+        struct RestoreGraphs {
+            gpart_ctx* ctx;
+            int enabled;
+            ~RestoreGraphs() { (void)gpart_refine_set_graphs(ctx, enabled); }
+        } restore_graphs{caller_ctx, graphs_were};
         const int threads = std::max(1, std::min(workers, count));
         std::atomic<int> next{0};
         std::atomic<bool> failed{false};
@@ -613,6 +623,7 @@ int gpart_partition_batch(int count, gpart_graph* const* graphs, const int* part
                 return;
             }
             gpart::detail::bind_context(ctx);
+            (void)gpart_refine_set_graphs(ctx, 0);
             run();
             gpart::detail::bind_context(nullptr);
             gpart_ctx_destroy(ctx);

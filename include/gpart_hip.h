@@ -465,6 +465,19 @@ int gpart_refine_rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g,
                                  const gpart_degree_classes* classes, void* cache, int cache_bytes,
                                  int32_t* labels, int k, uint64_t* part_weights, uint64_t capacity,
                                  int32_t* targets, int32_t* flags, gpart_refine_status* status);
+/* Round graphs (no reference counterpart: the reference launches every kernel of a
+ * round from the host). gpart_refine_lp_round and gpart_refine_rebalance_round
+ * record the launches of a round the first time they see its level, arrays and
+ * settings, as hipGraphs, and replay them on later calls; the results are those
+ * of the launches themselves. A profiled family's kernels run outside the graphs.
+ * A round is recorded on the second call of its key. Levels above 2^23 vertices are
+ * not recorded. gpart_partition_batch launches directly (its graphs do not repeat
+ * a key often enough to repay recording). enabled = 0 drops the recordings
+ * and launches every kernel directly. The stats count the plans held, the calls
+ * that recorded and the calls that replayed since the plans were last dropped. */
+int gpart_refine_set_graphs(gpart_ctx* ctx, int enabled);
+int gpart_refine_graphs_enabled(gpart_ctx* ctx);
+int gpart_refine_graph_stats(gpart_ctx* ctx, int64_t* plans, int64_t* captures, int64_t* replays);
 /* Outcome of gpart_refine_level. */
 typedef struct gpart_refine_level_stats {
     int32_t rounds;               /* LP rounds run */
