@@ -17,7 +17,10 @@
 // neighbours of its segment, so a fully freed segment is one block again.
 //
 // Host-only code: the segment source supplies and returns raw device memory.
-// Not thread-safe; one context is driven by one thread at a time.
+// Not thread-safe: the first thread that allocates owns the cache, and a call from
+// any other thread is refused with an exception instead of racing on the free
+// lists (rebind_owner() hands the cache over, as a context's teardown on another
+// thread does).
 #pragma once
 
 #include <chrono>
@@ -28,6 +31,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 namespace gpart_hip {
@@ -72,7 +76,10 @@ public:
         return (bytes + kAlign - 1) / kAlign * kAlign;
     }
 
+    void rebind_owner() { owner_ = std::this_thread::get_id(); }
+
     void* allocate(std::size_t bytes) {
+        check_owner();
         const std::size_t size = rounded(bytes);
         const bool small = size <= kSmallLimit;
         auto& pool = small ? free_small_ : free_large_;
@@ -103,6 +110,7 @@ public:
     // Returns the block to the cache; the caller's stream order makes it safe for
     // the next allocation on that stream.
     void free(void* p) {
+        check_owner();
         auto found = blocks_.find(static_cast<char*>(p));
         if (found == blocks_.end() || found->second->free)
             throw std::invalid_argument("block cache: free of an unknown device block");
@@ -136,6 +144,7 @@ public:
     // Drains the stream and returns every segment without live blocks to the
     // source. Returns the number of segments released.
     std::int64_t release_free_segments() {
+        check_owner();
         source_.drain();
         std::int64_t released = 0;
         for (auto* pool : {&free_small_, &free_large_}) {
@@ -177,6 +186,13 @@ public:
     }
 
 private:
+    void check_owner() {
+        const auto me = std::this_thread::get_id();
+        if (owner_ == std::thread::id()) owner_ = me;
+        else if (owner_ != me) throw std::logic_error("block cache: used by a thread that does not own its context");
+    }
+    std::thread::id owner_{};
+
     struct Block {
         char* ptr;
         std::size_t size;

@@ -300,6 +300,10 @@ int gpart_ctx_destroy(gpart_ctx* ctx) {
     return guarded([&] {
         GPART_HIP_CHECK(hipSetDevice(ctx->device));
         GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // teardown may run on another thread (a thread that ended leaves its
+        // context to the next one that creates a context)
+        ctx->cache->rebind_owner();
+        if (ctx->plan_cache) ctx->plan_cache->rebind_owner();
         ctx->round_plans.reset();
         // Every segment goes, including those of blocks the caller never freed.
         for (const auto& kv : ctx->cache->segments()) GPART_HIP_CHECK(hipFree(kv.first));

@@ -8,6 +8,8 @@
 #include <cstdlib>
 #include <map>
 #include <random>
+#include <stdexcept>
+#include <thread>
 #include <vector>
 
 using gpart_hip::BlockCache;
@@ -186,10 +188,48 @@ void test_out_of_memory() {
     REQUIRE(dev.used == 0);
 }
 
+// The first thread that allocates owns the cache: another thread's allocate,
+// free or release is refused with an exception and leaves the cache intact;
+// rebind_owner() hands it over.
+void test_thread_ownership() {
+    FakeDevice dev{std::size_t{8} << 30};
+    BlockCache cache(dev.source());
+    void* a = cache.allocate(1000);
+    int refused = 0;
+    std::thread other([&] {
+        try {
+            cache.allocate(64);
+        } catch (const std::logic_error&) {
+            ++refused;
+        }
+        try {
+            cache.free(a);
+        } catch (const std::logic_error&) {
+            ++refused;
+        }
+        try {
+            cache.release_free_segments();
+        } catch (const std::logic_error&) {
+            ++refused;
+        }
+    });
+    other.join();
+    REQUIRE(refused == 3);
+    REQUIRE(cache.stats().live_blocks == 1);
+    std::thread taker([&] {
+        cache.rebind_owner();
+        cache.free(a);
+        cache.release_free_segments();
+    });
+    taker.join();
+    REQUIRE(cache.stats().live_blocks == 0 && dev.used == 0);
+}
+
 int main() {
     test_best_fit_and_coalescing();
     test_out_of_memory();
     test_random_sequences();
+    test_thread_ownership();
     std::puts("block cache checks passed");
     return 0;
 }
