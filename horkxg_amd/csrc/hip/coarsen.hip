@@ -1494,25 +1494,29 @@ int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const 
             hipLaunchKernelGGL(admit_decide_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
                                ordered_targets.get(), prefix.get(), cw, capacity, accept.get());
             GPART_LAUNCH_CHECK();
-            Scratch<unsigned long long> acc(ctx, 2);
-            fill_bytes(ctx, acc.get(), 0, 16);
+            Scratch<unsigned long long> acc(ctx, 3);  // accepted, predicted gain, non-empty clusters
+            fill_bytes(ctx, acc.get(), 0, 24);
             profile_begin(ctx, GPART_PROFILE_SCLP_COMMIT);
             hipLaunchKernelGGL(admit_apply_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
                                ordered_targets.get(), ordered_weights.get(), accept.get(),
                                reinterpret_cast<const unsigned long long*>(gains), clusters, cw, acc.get());
             GPART_LAUNCH_CHECK();
             profile_end(ctx, GPART_PROFILE_SCLP_COMMIT, 0.0);
-            unsigned long long h[2];
+            // the non-empty cluster count comes back with the commit's counters
+            device_reduce<unsigned long long>(ctx, NonzeroLoad{cw}, n, SumU64{}, acc.get() + 2);
+            unsigned long long h[3];
             read_back(ctx, h, acc.get(), sizeof(h));
             *accepted = static_cast<std::int64_t>(h[0]);
+            *nonempty = static_cast<std::int64_t>(h[2]);
             // per proposal its order, target, weight and decision; per accepted move
             // its gain, cluster read and write and two cluster-weight atomics
             profile_add_bytes(ctx, GPART_PROFILE_SCLP_COMMIT,
                               20.0 * static_cast<double>(p) + 48.0 * static_cast<double>(h[0]));
             profile_add_units(ctx, GPART_PROFILE_SCLP_COMMIT, 2.0 * static_cast<double>(h[0]));
             if (predicted_gain) *predicted_gain = h[1];
+        } else {
+            *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));
         }
-        *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));
     });
 }
 
@@ -1538,8 +1542,8 @@ int gpart_sclp_isolated(gpart_ctx* ctx, const gpart_csr_view* g, const uint64_t*
             device_scan<std::uint64_t, SumOp<std::uint64_t>>(ctx, GatherWeightLoad{list.get(), vertex_weights},
                                                               InclusiveStore<std::uint64_t>{prefix.get()}, count,
                                                               SumOp<std::uint64_t>{});
-            const std::uint64_t total = read_scalar(ctx, prefix.get() + count - 1);
-            const std::int64_t bins = static_cast<std::int64_t>(total / capacity + (total % capacity != 0));
+            // every bin holds a vertex, so count bounds the bins (no read-back of the total)
+            const std::int64_t bins = count;
             Scratch<std::int32_t> reps(ctx, bins);
             Scratch<std::uint64_t> run_base(ctx, bins);
             hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(bins)), dim3(kBlock), 0, ctx->stream, reps.get(),
