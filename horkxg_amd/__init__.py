@@ -40,6 +40,7 @@ from ._native import (  # noqa: F401
     set_refine_rebuild,
     set_refine_engine,
     set_refine_graphs,
+    set_refine_balance_blocks,
     refine_graph_stats,
     set_spectral_queue,
     spectral_queue_settings,

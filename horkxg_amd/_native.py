@@ -138,6 +138,7 @@ def hip_lib():
         H.gpart_refine_keep_ratio.argtypes = [C.c_uint64, C.c_uint64]
         H.gpart_spectral_set_queue.argtypes = [C.c_int32]
         H.gpart_refine_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        H.gpart_refine_set_balance_blocks.argtypes = [C.c_void_p, C.c_int64]
         H.gpart_refine_graph_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64)]
         H.gpart_spectral_queue_set_tuning.argtypes = [C.c_int32, C.c_int32]
@@ -328,6 +329,11 @@ def set_refine_graphs(enabled: bool = True) -> None:
     """The calling thread's refinement rounds recorded as hipGraphs and replayed
     (default) or launched kernel by kernel. Results do not depend on it."""
     _check(hip_lib().gpart_refine_set_graphs(_ctx(), int(bool(enabled))))
+
+
+def set_refine_balance_blocks(blocks: int) -> None:
+    """The calling thread's cap on the rebalance proposals grid (result-neutral)."""
+    _check(hip_lib().gpart_refine_set_balance_blocks(_ctx(), int(blocks)))
 
 
 def refine_graph_stats() -> dict:

@@ -18,6 +18,8 @@ Tunables (defaults compiled into the library from tuned_gfx950.hpp):
   queue_single_max_task               largest task the single-graph driver gives the
                                       queue (0: per-depth launches only)
   batch_group                         graphs per group of partition_batch
+  balance_blocks                      workgroups of the rebalance proposals launch at
+                                      most (each flushes its bucket table once)
 
 Run on the GPU from the repository root:
     python -m horkxg_amd.autotune --write
@@ -44,7 +46,7 @@ HEADER_PATH = PACKAGE / "csrc" / "tuned_gfx950.hpp"
 
 # The library's built-in defaults (the values tuned_gfx950.hpp held before tuning).
 DEFAULTS = {"rebuild_share": 16, "rebuild_min_entries": 1 << 23, "queue_own_blocks": 8,
-            "queue_single_max_task": 0, "batch_group": 16}
+            "queue_single_max_task": 0, "batch_group": 16, "balance_blocks": 4096}
 
 
 def say(*args):
@@ -94,6 +96,7 @@ def apply(hx, setting):
     max_task = setting["queue_single_max_task"]
     hx.set_spectral_queue(max_task > 0, setting["queue_own_blocks"], max(1, max_task) if max_task else 4096)
     hx.set_batch_group(setting["batch_group"])
+    hx.set_refine_balance_blocks(setting["balance_blocks"])
 
 
 def sweep(hx, workloads, base, name, candidates, repeats, reference, margin):
@@ -138,6 +141,7 @@ constexpr std::int64_t kRebuildMinEntries = {chosen['rebuild_min_entries']};
 constexpr int kQueueOwnBlocks = {chosen['queue_own_blocks']};
 constexpr int kQueueSingleMaxTask = {chosen['queue_single_max_task']};  // 0: single graphs run depth by depth
 constexpr int kBatchGroup = {chosen['batch_group']};
+constexpr std::int64_t kBalanceBlocks = {chosen['balance_blocks']};
 }}  // namespace gpart_tuned
 """
     path.write_text(text)
@@ -187,6 +191,7 @@ def main(argv=None) -> int:
         ("queue_own_blocks", [8, 16, 32, 64, 128], single + [mix]),
         ("queue_single_max_task", [0, 256, 512, 1024, 4096], single),
         ("batch_group", [1, 4, 8, 16, 32], [mix]),
+        ("balance_blocks", [4096, 512, 1024, 2048, 8192], single),
     ]
     for name, candidates, on in plan:
         say(f"[autotune] {name}")

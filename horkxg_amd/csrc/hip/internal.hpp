@@ -73,6 +73,9 @@ struct gpart_ctx {
     // more than 1/rebuild_share of them (share <= 0: never).
     std::int64_t rebuild_min_entries = gpart_tuned::kRebuildMinEntries;
     std::int64_t rebuild_share = gpart_tuned::kRebuildShare;
+    // Workgroups of the rebalance proposals launch at most (each flushes its
+    // bucket table into the global counters once).
+    std::int64_t balance_blocks = gpart_tuned::kBalanceBlocks;
     // The rebalance's bucket-counter replicas, kept between rounds: the bucket
     // scans zero every counter they read, so a completed round leaves them zero.
     void* rebalance_counters = nullptr;

@@ -1564,7 +1564,7 @@ struct Ops {
                                                      SumOp<long long>{});
         });
         const int G = group_for_k(k);
-        const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), 4 * kRoundBlocks));
+        const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), ctx->balance_blocks));
         run.phase(GPART_PROFILE_REBALANCE, [&] {
         profile_begin(ctx, GPART_PROFILE_REBALANCE);
 #define GPART_BALANCE_PROPOSALS(GG)                                                                              \
@@ -1774,6 +1774,13 @@ int gpart_refine_lp_round(gpart_ctx* ctx, const gpart_csr_view* g, const gpart_d
                              OpsT::lp_round(ctx, g, classes, static_cast<CacheT*>(cache), labels, k, filter, locks,
                                             targets, reinterpret_cast<long long*>(gains), flags,
                                             reinterpret_cast<unsigned long long*>(part_weights), status));
+    });
+}
+
+int gpart_refine_set_balance_blocks(gpart_ctx* ctx, int64_t blocks) {
+    return guarded([&] {
+        if (blocks < 1 || blocks > (int64_t{1} << 20)) throw Error(GPART_ERR_INVALID_ARGUMENT, "block count out of range");
+        ctx->balance_blocks = blocks;
     });
 }
 

@@ -8,7 +8,8 @@
 namespace gpart_tuned {
 constexpr std::int64_t kRebuildShare = 16;
 constexpr std::int64_t kRebuildMinEntries = 8388608;
-constexpr int kQueueOwnBlocks = 16;
+constexpr int kQueueOwnBlocks = 8;
 constexpr int kQueueSingleMaxTask = 0;  // 0: single graphs run depth by depth
 constexpr int kBatchGroup = 16;
+constexpr std::int64_t kBalanceBlocks = 4096;
 }  // namespace gpart_tuned

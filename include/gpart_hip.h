@@ -476,6 +476,9 @@ int gpart_refine_rebalance_round(gpart_ctx* ctx, const gpart_csr_view* g,
  * and launches every kernel directly. The stats count the plans held, the calls
  * that recorded and the calls that replayed since the plans were last dropped. */
 int gpart_refine_set_graphs(gpart_ctx* ctx, int enabled);
+/* Workgroups of the rebalance proposals launch at most (default from the tuning
+ * header); results do not depend on it. */
+int gpart_refine_set_balance_blocks(gpart_ctx* ctx, int64_t blocks);
 int gpart_refine_graphs_enabled(gpart_ctx* ctx);
 int gpart_refine_graph_stats(gpart_ctx* ctx, int64_t* plans, int64_t* captures, int64_t* replays);
 /* Outcome of gpart_refine_level. */

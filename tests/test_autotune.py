@@ -11,7 +11,7 @@ from horkxg_amd import autotune
 def header_values(text: str) -> dict:
     names = {"kRebuildShare": "rebuild_share", "kRebuildMinEntries": "rebuild_min_entries",
              "kQueueOwnBlocks": "queue_own_blocks", "kQueueSingleMaxTask": "queue_single_max_task",
-             "kBatchGroup": "batch_group"}
+             "kBatchGroup": "batch_group", "kBalanceBlocks": "balance_blocks"}
     out = {}
     for const, key in names.items():
         m = re.search(rf"constexpr [\w:]+ {const} = (-?\d+);", text)
@@ -22,7 +22,7 @@ def header_values(text: str) -> dict:
 
 def test_header_round_trip(tmp_path):
     chosen = {"rebuild_share": 8, "rebuild_min_entries": 1 << 21, "queue_own_blocks": 16,
-              "queue_single_max_task": 512, "batch_group": 4}
+              "queue_single_max_task": 512, "batch_group": 4, "balance_blocks": 1024}
     path = tmp_path / "tuned.hpp"
     autotune.write_header(chosen, path)
     assert header_values(path.read_text()) == chosen
