@@ -51,6 +51,11 @@ struct RoundState {
 };
 
 constexpr int kHeavyDegree = 64;
+// Levels whose largest degree is at most this update their heavy movers' cache rows
+// a wave per mover (one launch); above it the movers' entries are flattened into
+// tiles first (a scan and a tile map, three more launches) so that hubs spread
+// over many workgroups.
+constexpr int kWaveHeavyMaxDegree = 4096;
 constexpr int kRoundBlocks = 1024;  // cap of grid-stride launches whose count lives on the device
 constexpr int kCacheBlocks = 4096;  // cap of the cache build launches (grid-stride)
 
@@ -1388,7 +1393,14 @@ struct Ops {
         hipLaunchKernelGGL((update_cache_kernel<O, W, 8, C>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
                            &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta,
                            &st->edges, rebuild_above);
-        if (c->max_degree > kHeavyDegree) {
+        if (c->max_degree > kHeavyDegree && c->max_degree <= kWaveHeavyMaxDegree) {
+            const std::int64_t heavy_bound = c->medium_count + c->high_count;
+            const auto heavy_blocks = static_cast<unsigned>(std::max<std::int64_t>(
+                1, std::min<std::int64_t>(grid_for(heavy_bound * kWave), 4 * kRoundBlocks)));
+            hipLaunchKernelGGL((update_cache_kernel<O, W, kWave, C>), dim3(heavy_blocks), dim3(kBlock), 0,
+                               ctx->stream, &st->heavy, heavy, off, g->neighbors, w, labels, targets, flags, k, cache,
+                               &st->delta, &st->edges, rebuild_above);
+        } else if (c->max_degree > kHeavyDegree) {
             const std::int64_t heavy_bound = c->medium_count + c->high_count;
             Scratch<long long> heavy_start(ctx, heavy_bound + 1);
             Scratch<std::int32_t> tile_mover(ctx, tile_count(g->m) + 1);
