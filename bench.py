@@ -534,6 +534,9 @@ def main() -> int:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
     p.add_argument("--no-profile", action="store_true", help="skip the live per-family event timing")
+    p.add_argument("--no-round-graphs", action="store_true",
+                   help="launch the refinement rounds kernel by kernel instead of replaying their hipGraphs "
+                        "(for kernel-trace profiles: the kernels are the same)")
     p.add_argument("--no-precision", action="store_true",
                    help="skip the FP32-against-FP64 comparison (a second coarsening and partition)")
     p.add_argument("--copy-calibration", type=int, metavar="MIB", help="only run the copy calibration")
@@ -571,6 +574,8 @@ def main() -> int:
         return 1
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.no_round_graphs:
+        hx.set_refine_graphs(False)
     spec = WORKLOADS[args.workload]
     k = spec["k"]
     width = spec.get("width", "standard")
@@ -784,7 +789,9 @@ def main() -> int:
         "cold_step_seconds": (round(float(cold_seconds), 6) if cold_seconds is not None else None),
         "round_graphs": {"plans": graph_stats["plans"], "recorded_rounds": graph_stats["captures"],
                          "replayed_rounds": graph_stats["replays"],
-                         "note": "refinement rounds replayed from hipGraphs recorded on the first partition"},
+                         "note": ("refinement rounds launched kernel by kernel (--no-round-graphs)"
+                                  if args.no_round_graphs else
+                                  "refinement rounds replayed from hipGraphs recorded on the first partition")},
         "device_memory": {
             "peak_reserved_gib": round(alloc_after["peak_reserved_bytes"] / 2**30, 3),
             "peak_live_gib": round(alloc_after["peak_live_bytes"] / 2**30, 3),

@@ -11,6 +11,8 @@
 //      ballots give the lanes sharing a digit), waves are combined in LDS, the
 //      tile is staged in LDS in digit order and written out so that consecutive
 //      threads store consecutive addresses of one digit run.
+// Inputs of at most 8192 keys (four tiles) are sorted by one workgroup instead:
+// three or more launches per 8-bit pass cost more than the whole sort in LDS.
 #include "internal.hpp"
 #include "scan.hpp"
 
@@ -134,12 +136,68 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const std::
     }
 }
 
+// Stable sort of at most kLdsSortMax keys by their bits [shift, shift + width) in
+// one workgroup: a bitonic network in LDS over (digits, input position) pairs.
+// Positions are distinct, so the order is total and equal digits keep their
+// input order, as in the multi-pass sort. The m - n padding entries carry the
+// largest digits and positions and end up past the n real ones.
+constexpr int kLdsSortMax = 4 * kSortTile;
+constexpr int kLdsSortThreads = 1024;
+static_assert(kLdsSortMax <= 65536, "positions are 16-bit");
+
+template <typename V, bool HasValues>
+__global__ void __launch_bounds__(kLdsSortThreads) lds_sort_kernel(const std::uint64_t* keys_in,
+                                                                   std::uint64_t* keys_out, const V* values_in,
+                                                                   V* values_out, int n, int m, int shift,
+                                                                   std::uint64_t mask) {
+    __shared__ std::uint64_t digit[kLdsSortMax];
+    __shared__ std::uint16_t position[kLdsSortMax];
+    for (int i = threadIdx.x; i < m; i += kLdsSortThreads) {
+        digit[i] = i < n ? (keys_in[i] >> shift) & mask : ~0ULL;
+        position[i] = static_cast<std::uint16_t>(i);
+    }
+    __syncthreads();
+    for (int size = 2; size <= m; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = threadIdx.x; q < (m >> 1); q += kLdsSortThreads) {
+                const int i = ((q & ~(stride - 1)) << 1) | (q & (stride - 1));
+                const int j = i + stride;
+                const std::uint64_t di = digit[i], dj = digit[j];
+                const std::uint16_t pi = position[i], pj = position[j];
+                const bool greater = di > dj || (di == dj && pi > pj);
+                if (greater == ((i & size) == 0)) {  // ascending runs where bit `size` of i is clear
+                    digit[i] = dj;
+                    digit[j] = di;
+                    position[i] = pj;
+                    position[j] = pi;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int r = threadIdx.x; r < n; r += kLdsSortThreads) {
+        const int from = position[r];
+        keys_out[r] = keys_in[from];
+        if constexpr (HasValues) values_out[r] = values_in[from];
+    }
+}
+
 template <typename V, bool HasValues>
 bool radix_sort_impl(gpart_ctx* ctx, std::uint64_t* keys, std::uint64_t* keys_alt, V* values, V* values_alt,
                      std::int64_t n, int begin_bit, int end_bit) {
     if (n <= 1 || end_bit <= begin_bit) return false;
     if (n > (std::int64_t{1} << 32) - kSortTile)
         throw Error(GPART_ERR_INVALID_ARGUMENT, "radix sort supports fewer than 2^32 keys per call");
+    if (n <= kLdsSortMax) {
+        int m = 2;
+        while (m < n) m <<= 1;
+        const int width = end_bit - begin_bit;
+        const std::uint64_t mask = width >= 64 ? ~0ULL : ((1ULL << width) - 1);
+        hipLaunchKernelGGL((lds_sort_kernel<V, HasValues>), dim3(1), dim3(kLdsSortThreads), 0, ctx->stream, keys,
+                           keys_alt, values, values_alt, static_cast<int>(n), m, begin_bit, mask);
+        GPART_LAUNCH_CHECK();
+        return true;
+    }
     const std::int64_t tiles = (n + kSortTile - 1) / kSortTile;
     Scratch<std::uint32_t> hist(ctx, tiles * kDigits);
     std::uint64_t* src_k = keys;

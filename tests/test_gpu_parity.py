@@ -57,19 +57,30 @@ GRAPHS = {
 }
 
 
-@pytest.mark.parametrize("n", [1, 7, 2048, 2049, 100_000, 1_000_003])
+# sizes on both sides of the one-workgroup sort (at most 8192 keys) and of a tile
+@pytest.mark.parametrize("n", [1, 2, 7, 2048, 2049, 5000, 8192, 8193, 100_000, 1_000_003])
 def test_sort_pairs_stable(hx, n):
     say(f"case sort n={n}")
     rng = np.random.default_rng(n)
-    for begin, end in [(0, 64), (0, 17), (20, 44)]:
-        keys = rng.integers(0, 2**63, n, dtype=np.uint64) if n > 8 else rng.integers(0, 4, n).astype(np.uint64)
-        vals = np.arange(n, dtype=np.uint32)
-        k, v = hx.device.sort_pairs(keys, vals, begin, end)
-        mask = np.uint64(((1 << (end - begin)) - 1) if end - begin < 64 else 2**64 - 1)
-        digit = (keys >> np.uint64(begin)) & mask
-        order = np.argsort(digit, kind="stable")
-        assert np.array_equal(v, vals[order])
-        assert np.array_equal(k, keys[order])
+    for begin, end in [(0, 64), (0, 17), (20, 44), (60, 64)]:
+        for spread in ("wide", "few"):
+            # "few": a handful of distinct digits, so stability decides most of the order
+            if spread == "wide" and n > 8:
+                keys = rng.integers(0, 2**64, n, dtype=np.uint64)
+            else:
+                keys = rng.integers(0, 4, n).astype(np.uint64) << np.uint64(begin)
+                keys |= rng.integers(0, 2**16, n).astype(np.uint64) << np.uint64(end % 64) if end < 64 else 0
+            mask = np.uint64(((1 << (end - begin)) - 1) if end - begin < 64 else 2**64 - 1)
+            digit = (keys >> np.uint64(begin)) & mask
+            order = np.argsort(digit, kind="stable")
+            vals = np.arange(n, dtype=np.uint32)
+            k, v = hx.device.sort_pairs(keys, vals, begin, end)
+            assert np.array_equal(v, vals[order]), (begin, end, spread)
+            assert np.array_equal(k, keys[order]), (begin, end, spread)
+            vals64 = rng.integers(0, 2**64, n, dtype=np.uint64)
+            k, v = hx.device.sort_pairs(keys, vals64, begin, end)
+            assert np.array_equal(v, vals64[order]), (begin, end, spread)
+            assert np.array_equal(hx.device.sort_pairs(keys, None, begin, end), keys[order]), (begin, end, spread)
 
 
 @pytest.mark.parametrize("n", [1, 5, 2048, 4097, 3_000_001])
