@@ -3,9 +3,10 @@
 // A tile is 256 threads x 8 items in blocked arrangement. Pass 1 reduces each
 // tile; pass 2 scans the tile totals (recursively); pass 3 rescans each tile
 // with its carried-in prefix and hands (index, exclusive prefix, item) to a
-// store functor. Load/store functors let callers fuse their producers and
-// consumers (flags -> compaction, (value, head) -> segmented prefix) into the
-// scan instead of materialising temporaries.
+// store functor. Up to 1024 tiles pass 3 reduces the totals below its tile
+// itself and pass 2 is not launched. Load/store functors let callers fuse
+// their producers and consumers (flags -> compaction, (value, head) ->
+// segmented prefix) into the scan instead of materialising temporaries.
 #pragma once
 
 #include "internal.hpp"
@@ -15,6 +16,7 @@ namespace gpart_hip {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr std::int64_t kScanInlineTiles = 1024;  // up to here two launches (the tiles' totals reduced inline)
 
 template <typename T>
 struct SumOp {
@@ -94,9 +96,13 @@ __global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(Load load, st
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
+// partial_prefix: null (one tile), the tiles' exclusive prefixes, or (tile_totals)
+// the tiles' totals, which each workgroup then reduces up to its own tile, in
+// order, itself: at most kScanInlineTiles tiles, so at most that many L2 reads
+// per workgroup, and the launch that scans the totals is saved.
 template <typename T, typename Op, typename Load, typename Store>
 __global__ void __launch_bounds__(kScanThreads) scan_downsweep_kernel(Load load, Store store, std::int64_t n, Op op,
-                                                                      const T* partial_prefix) {
+                                                                      const T* partial_prefix, int tile_totals) {
     __shared__ T shared_waves[kScanThreads / kWave + 1];
     const std::int64_t base = static_cast<std::int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
     T items[kScanItems];
@@ -107,9 +113,24 @@ __global__ void __launch_bounds__(kScanThreads) scan_downsweep_kernel(Load load,
         items[i] = idx < n ? load(idx) : op.identity();
         acc = op(acc, items[i]);
     }
+    T carry = op.identity();
+    if (partial_prefix && tile_totals) {
+        // thread t reduces totals [t * per, (t + 1) * per) below this tile; the
+        // block scan's total combines the threads in order
+        const int below = static_cast<int>(blockIdx.x);
+        const int per = (below + kScanThreads - 1) / kScanThreads;
+        T part = op.identity();
+        for (int i = 0; i < per; ++i) {
+            const int idx = static_cast<int>(threadIdx.x) * per + i;
+            if (idx < below) part = op(part, partial_prefix[idx]);
+        }
+        (void)block_exclusive_scan(part, op, &carry, shared_waves);
+    } else if (partial_prefix) {
+        carry = partial_prefix[blockIdx.x];
+    }
     T total;
     T prefix = block_exclusive_scan(acc, op, &total, shared_waves);
-    if (partial_prefix) prefix = op(partial_prefix[blockIdx.x], prefix);
+    if (partial_prefix) prefix = op(carry, prefix);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         const std::int64_t idx = base + i;
@@ -146,7 +167,7 @@ void device_scan(gpart_ctx* ctx, Load load, Store store, std::int64_t n, Op op) 
     const std::int64_t tiles = (n + kScanTile - 1) / kScanTile;
     if (tiles == 1) {
         hipLaunchKernelGGL((scan_downsweep_kernel<T, Op, Load, Store>), dim3(1), dim3(kScanThreads), 0,
-                           ctx->stream, load, store, n, op, static_cast<const T*>(nullptr));
+                           ctx->stream, load, store, n, op, static_cast<const T*>(nullptr), 0);
         GPART_LAUNCH_CHECK();
         return;
     }
@@ -154,11 +175,13 @@ void device_scan(gpart_ctx* ctx, Load load, Store store, std::int64_t n, Op op) 
     hipLaunchKernelGGL((scan_reduce_kernel<T, Op, Load>), dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0,
                        ctx->stream, load, n, op, partials.get());
     GPART_LAUNCH_CHECK();
-    // Exclusive scan of the tile totals, in place.
-    device_scan<T, Op>(ctx, ArrayLoad<T>{partials.get()}, ExclusiveStore<T>{partials.get()}, tiles, op);
+    const bool inline_totals = tiles <= kScanInlineTiles;
+    // Otherwise: exclusive scan of the tile totals, in place.
+    if (!inline_totals)
+        device_scan<T, Op>(ctx, ArrayLoad<T>{partials.get()}, ExclusiveStore<T>{partials.get()}, tiles, op);
     hipLaunchKernelGGL((scan_downsweep_kernel<T, Op, Load, Store>), dim3(static_cast<unsigned>(tiles)),
                        dim3(kScanThreads), 0, ctx->stream, load, store, n, op,
-                       static_cast<const T*>(partials.get()));
+                       static_cast<const T*>(partials.get()), inline_totals ? 1 : 0);
     GPART_LAUNCH_CHECK();
 }
 

@@ -83,7 +83,8 @@ def test_sort_pairs_stable(hx, n):
             assert np.array_equal(hx.device.sort_pairs(keys, None, begin, end), keys[order]), (begin, end, spread)
 
 
-@pytest.mark.parametrize("n", [1, 5, 2048, 4097, 3_000_001])
+# one tile, tiles whose totals each workgroup reduces itself (up to 1024), and more
+@pytest.mark.parametrize("n", [1, 5, 2048, 4097, 600_001, 2_097_152, 2_097_153, 3_000_001])
 def test_scan_and_compact(hx, n):
     say(f"case scan n={n}")
     rng = np.random.default_rng(n)
