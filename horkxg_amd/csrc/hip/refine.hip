@@ -917,6 +917,35 @@ __global__ void __launch_bounds__(kBucketScanThreads) column_scan_kernel(const u
     }
 }
 
+// The exclusive prefix over the workgroups before this one of each of the 32
+// columns of block_sums (a row of 32 per workgroup) into running[]: read from
+// column_prefix when the column scan ran, else summed here from the rows, which
+// grids of at most kInlineColumnBlocks workgroups do instead of that launch
+// (thread t sums column t % 32 over the rows t / 32, t / 32 + 8, ...).
+constexpr int kInlineColumnBlocks = 256;
+__device__ __forceinline__ void column_running(const unsigned long long* column_prefix,
+                                               const unsigned long long* block_sums, unsigned long long* running,
+                                               unsigned long long (*groups)[32]) {
+    static_assert(kBlock == 256, "8 row groups of 32 columns");
+    if (column_prefix) {
+        if (threadIdx.x < 32)
+            running[threadIdx.x] = column_prefix[static_cast<long long>(blockIdx.x) * 32 + threadIdx.x];
+    } else {
+        const int column = threadIdx.x & 31, group = threadIdx.x >> 5;
+        unsigned long long sum = 0;
+        for (int row = group; row < static_cast<int>(blockIdx.x); row += kBlock / 32)
+            sum += block_sums[static_cast<long long>(row) * 32 + column];
+        groups[group][column] = sum;
+        __syncthreads();
+        if (threadIdx.x < 32) {
+            unsigned long long total = 0;
+#pragma unroll
+            for (int g = 0; g < kBlock / 32; ++g) total += groups[g][threadIdx.x];
+            running[threadIdx.x] = total;
+        }
+    }
+}
+
 // Source boundary members: taken while offset + rank < overload.
 template <typename W>
 __global__ void __launch_bounds__(kBlock) balance_source_ranks_kernel(const RoundState* st, const std::int32_t* cand,
@@ -927,14 +956,16 @@ __global__ void __launch_bounds__(kBlock) balance_source_ranks_kernel(const Roun
                                                                       const unsigned long long* offsets,
                                                                       const unsigned long long* thresholds,
                                                                       const unsigned long long* column_prefix,
+                                                                      const unsigned long long* block_sums,
                                                                       unsigned long long* dst_replicas,
                                                                       long long width) {
     __shared__ BucketTable table;
     unsigned long long* dst_counts = dst_replicas + static_cast<long long>(blockIdx.x % kReplicas) * width;
     __shared__ unsigned long long wave_total[kBlock / kWave][32];
     __shared__ unsigned long long running[32];
+    __shared__ unsigned long long groups[kBlock / 32][32];
     table_clear(table);
-    if (threadIdx.x < 32) running[threadIdx.x] = column_prefix[static_cast<long long>(blockIdx.x) * 32 + threadIdx.x];
+    column_running(column_prefix, block_sums, running, groups);
     __syncthreads();
     long long begin, end;
     block_slice(st->candidates, begin, end);
@@ -1008,12 +1039,14 @@ template <typename O, typename W>
 __global__ void __launch_bounds__(kBlock) balance_destination_ranks_kernel(
     RoundState* st, const std::int32_t* cand, const std::int32_t* dbucket, const W* vertex_weights,
     const O* offsets_csr, const unsigned long long* offsets, const unsigned long long* thresholds,
-    const unsigned long long* column_prefix, std::int32_t* light, std::int32_t* heavy, std::int32_t* flags) {
+    const unsigned long long* column_prefix, const unsigned long long* block_sums, std::int32_t* light,
+    std::int32_t* heavy, std::int32_t* flags) {
     __shared__ unsigned long long wave_total[kBlock / kWave][32];
     __shared__ unsigned long long running[32];
+    __shared__ unsigned long long groups[kBlock / 32][32];
     __shared__ unsigned counts_lds[2 * (kBlock / kWave)];
     __shared__ unsigned long long base_lds[2];
-    if (threadIdx.x < 32) running[threadIdx.x] = column_prefix[static_cast<long long>(blockIdx.x) * 32 + threadIdx.x];
+    column_running(column_prefix, block_sums, running, groups);
     __syncthreads();
     long long begin, end;
     block_slice(st->candidates, begin, end);
@@ -1569,6 +1602,9 @@ struct Ops {
         const int slices = static_cast<int>(std::max<std::int64_t>(1, std::min<std::int64_t>(grid_for(n), kRoundBlocks)));
         Scratch<unsigned long long> sums(ctx, static_cast<std::int64_t>(slices) * 32),
             prefix(ctx, static_cast<std::int64_t>(slices) * 32);
+        // small grids sum the per-workgroup boundary weights in the ranks kernels
+        const bool inline_columns = slices <= kInlineColumnBlocks;
+        const unsigned long long* column_prefix = inline_columns ? nullptr : prefix.get();
 
         run.phase(-1, [&] {
             device_scan<long long, SumOp<long long>>(ctx, OverloadedLoad{labels, part_weights, capacity},
@@ -1598,21 +1634,23 @@ struct Ops {
         hipLaunchKernelGGL(balance_sources_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st_ptr,
                            cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_counts, src_offsets,
                            src_thresholds, sums.get(), dst_replicas, width);
-        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, sums.get(), slices,
-                           prefix.get());
+        if (!inline_columns)
+            hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, sums.get(),
+                               slices, prefix.get());
         hipLaunchKernelGGL(balance_source_ranks_kernel<W>, dim3(slices), dim3(kBlock), 0, ctx->stream, st_ptr,
                            cand.get(), cbucket.get(), dbucket.get(), targets, vw, src_offsets, src_thresholds,
-                           prefix.get(), dst_replicas, width);
+                           column_prefix, sums.get(), dst_replicas, width);
         hipLaunchKernelGGL(balance_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, dst_replicas, k,
                            dst_counts, dst_offsets, part_weights, capacity, true, dst_thresholds);
         hipLaunchKernelGGL((balance_destinations_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
                            st_ptr, cand.get(), dbucket.get(), vw, off, dst_counts, dst_offsets, dst_thresholds,
                            sums.get(), light.get(), heavy.get(), flags);
-        hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, sums.get(), slices,
-                           prefix.get());
+        if (!inline_columns)
+            hipLaunchKernelGGL(column_scan_kernel, dim3(k), dim3(kBucketScanThreads), 0, ctx->stream, sums.get(),
+                               slices, prefix.get());
         hipLaunchKernelGGL((balance_destination_ranks_kernel<O, W>), dim3(slices), dim3(kBlock), 0, ctx->stream,
-                           st_ptr, cand.get(), dbucket.get(), vw, off, dst_offsets, dst_thresholds, prefix.get(),
-                           light.get(), heavy.get(), flags);
+                           st_ptr, cand.get(), dbucket.get(), vw, off, dst_offsets, dst_thresholds, column_prefix,
+                           sums.get(), light.get(), heavy.get(), flags);
         GPART_LAUNCH_CHECK();
         });
         apply(ctx, run, g, c, st_ptr, light.get(), heavy.get(), labels, targets, flags, k, cache, part_weights,
