@@ -266,7 +266,7 @@ def _mix_worker(rank, specs, order, mode, barrier, counter, queue, hw_queues, gr
             graphs[i].partition(specs[i]["k"], specs[i]["ratio"], specs[i]["seed"], spectral_mode=mode,
                                 want_labels=False)
         hx.set_batch_group(max(1, group))
-        barrier.wait()
+        barrier.wait(timeout=MIX_BARRIER_SECONDS)
         t0 = time.perf_counter()
         done = []
         while True:
@@ -290,6 +290,34 @@ def _mix_worker(rank, specs, order, mode, barrier, counter, queue, hw_queues, gr
         queue.put((rank, None, None, repr(e)))
 
 
+MIX_BARRIER_SECONDS = 600  # every worker has its graphs and code objects by then
+
+
+def _collect_mix_results(queue, workers, poll_seconds=1.0):
+    """One result per worker from `queue`. A worker that ends without posting one
+    (a hard crash, an out-of-memory kill) is found by its exit code: the remaining
+    workers are terminated and the bench fails with that code instead of waiting."""
+    import queue as queue_mod
+
+    results = []
+    while len(results) < len(workers):
+        try:
+            results.append(queue.get(timeout=poll_seconds))
+            continue
+        except queue_mod.Empty:
+            pass
+        dead = [w for w in workers if w.exitcode not in (None, 0)]
+        if dead:
+            for w in workers:
+                if w.exitcode is None:
+                    w.terminate()
+            for w in workers:
+                w.join()
+            raise RuntimeError(f"mix worker died with exit code {dead[0].exitcode} "
+                               f"({len(results)} of {len(workers)} results in)")
+    return results
+
+
 def _mix_processes(specs, sizes, procs, mode, steps, hw_queues=0, group=1):
     """Partitions/sec of `procs` processes sharing the GPU, one fresh set of
     processes per step: per step, the wall from the earliest start to the latest
@@ -308,7 +336,7 @@ def _mix_processes(specs, sizes, procs, mode, steps, hw_queues=0, group=1):
                    for r in range(procs)]
         for w in workers:
             w.start()
-        results = [queue.get() for _ in workers]
+        results = _collect_mix_results(queue, workers)
         for w in workers:
             w.join()
         errors = [e for _, _, _, e in results if e]
