@@ -549,9 +549,9 @@ struct PartLoad {
 // i, entries [begin + p * len, ...) with len = ceil(degree / P)), so no workgroup
 // walks a whole row: sclp_split_count_kernel stores each entry's cluster in row
 // order (-1 for a self loop) and counts, per chunk, the entries of each group
-// q = mix32(cluster) & (P - 1); sclp_split_scatter_kernel places chunk p's
-// entries of group q after those of the groups before q and of the chunks before
-// p. Ranks within a chunk come from one LDS atomic per entry (measured faster
+// q = mix32(cluster) & (P - 1); sclp_split_offsets_kernel turns the counts into
+// offsets and sclp_split_scatter_kernel places chunk p's entries of group q
+// after those of the groups before q and of the chunks before p. Ranks within a chunk come from one LDS atomic per entry (measured faster
 // than a ballot per group and wave, from 2 groups up); the LDS tables that read a
 // group sum exactly, in any order.
 constexpr int kMaxParts = static_cast<int>(kPartMaxDegree / kPartKeys);
@@ -644,41 +644,61 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_count_kernel(
     }
 }
 
+// Once per split row (in the workgroup that meets its chunk 0): each group's
+// size over the row's chunks, the groups' places in the row's region
+// (group_begin / group_end) and, in place of the chunk counts, the offset in the
+// region at which each chunk's entries of each group start (the groups before
+// it, then the same group's entries of the chunks before). Thread g walks the
+// chunks of group g: P * P loads per row, where every chunk item redoing the sums
+// would take P * P per item.
+__global__ void __launch_bounds__(kPartThreads) sclp_split_offsets_kernel(
+    const std::int64_t* split_total, const std::int32_t* split_row, const std::int64_t* split_start,
+    const std::int64_t* region, const std::int64_t* item_start, std::uint32_t* chunk_counts,
+    std::int64_t* group_begin, std::int64_t* group_end) {
+    __shared__ unsigned int sizes[kMaxParts];
+    const std::int64_t total = *split_total;
+    for (std::int64_t j = blockIdx.x; j < total; j += gridDim.x) {
+        const std::int32_t i = split_row[j];
+        const std::int64_t first = split_start[i];
+        if (j != first) continue;  // the same j in every thread of the workgroup
+        const int parts = static_cast<int>(split_start[i + 1] - first);
+        if (static_cast<int>(threadIdx.x) < parts) {
+            unsigned int size = 0;
+            for (int p = 0; p < parts; ++p) size += chunk_counts[(first + p) * kMaxParts + threadIdx.x];
+            sizes[threadIdx.x] = size;
+        }
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < parts) {
+            unsigned int at = 0;
+            for (int g = 0; g < static_cast<int>(threadIdx.x); ++g) at += sizes[g];
+            const std::int64_t base = region[i];
+            group_begin[item_start[i] + threadIdx.x] = base + at;
+            group_end[item_start[i] + threadIdx.x] = base + at + sizes[threadIdx.x];
+            for (int p = 0; p < parts; ++p) {
+                std::uint32_t* slot = chunk_counts + (first + p) * kMaxParts + threadIdx.x;
+                const unsigned int n = *slot;
+                *slot = at;
+                at += n;
+            }
+        }
+        __syncthreads();  // sizes is reused by the next row
+    }
+}
+
 template <typename O, typename W>
 __global__ void __launch_bounds__(kPartThreads) sclp_split_scatter_kernel(
     const std::int64_t* split_total, const std::int32_t* split_row, const std::int64_t* split_start,
     const std::int32_t* list, const O* offsets, const W* weights, const std::int64_t* region,
-    const std::uint32_t* chunk_counts, const std::int32_t* row_clusters, const std::int64_t* item_start,
-    std::int32_t* group_keys, W* group_weights, std::int64_t* group_begin, std::int64_t* group_end) {
+    const std::uint32_t* chunk_offsets, const std::int32_t* row_clusters, std::int32_t* group_keys,
+    W* group_weights) {
     __shared__ unsigned long long cursor[kMaxParts];
-    __shared__ unsigned int sizes[kMaxParts];
     const std::int64_t total = *split_total;
     for (std::int64_t j = blockIdx.x; j < total; j += gridDim.x) {
         const SplitChunk c = split_chunk(j, split_row, split_start, list, offsets);
-        const std::int64_t first_chunk = j - c.p;
         const std::int64_t base = region[c.i];
         const auto pmask = static_cast<std::uint32_t>(c.parts - 1);
-        // thread g: group g's size over the row's chunks, and its entries in chunks before p
-        unsigned int before = 0;
-        if (threadIdx.x < c.parts) {
-            unsigned int size = 0;
-            for (int p = 0; p < c.parts; ++p) {
-                const unsigned int n = chunk_counts[(first_chunk + p) * kMaxParts + threadIdx.x];
-                size += n;
-                if (p < c.p) before += n;
-            }
-            sizes[threadIdx.x] = size;
-        }
-        __syncthreads();
-        if (threadIdx.x < c.parts) {
-            std::int64_t at = base;
-            for (int g = 0; g < static_cast<int>(threadIdx.x); ++g) at += sizes[g];
-            cursor[threadIdx.x] = static_cast<unsigned long long>(at + before);
-            if (c.p == 0) {
-                group_begin[item_start[c.i] + threadIdx.x] = at;
-                group_end[item_start[c.i] + threadIdx.x] = at + sizes[threadIdx.x];
-            }
-        }
+        if (static_cast<int>(threadIdx.x) < c.parts)
+            cursor[threadIdx.x] = static_cast<unsigned long long>(base + chunk_offsets[j * kMaxParts + threadIdx.x]);
         __syncthreads();
         for (std::int64_t e0 = c.begin; e0 < c.end; e0 += kPartThreads) {
             const std::int64_t e = e0 + threadIdx.x;
@@ -1300,11 +1320,13 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
             hipLaunchKernelGGL((sclp_split_count_kernel<O>), dim3(split_grid), dim3(kPartThreads), 0, ctx->stream,
                                split_items.total(), split_items.hub.get(), split_items.start.get(), c->high, off,
                                g->neighbors, clusters, region.get(), row_clusters.get(), chunk_counts.get());
+            hipLaunchKernelGGL(sclp_split_offsets_kernel, dim3(split_grid), dim3(kPartThreads), 0, ctx->stream,
+                               split_items.total(), split_items.hub.get(), split_items.start.get(), region.get(),
+                               items.start.get(), chunk_counts.get(), group_begin.get(), group_end.get());
             hipLaunchKernelGGL((sclp_split_scatter_kernel<O, W>), dim3(split_grid), dim3(kPartThreads), 0,
                                ctx->stream, split_items.total(), split_items.hub.get(), split_items.start.get(),
                                c->high, off, w, region.get(), chunk_counts.get(), row_clusters.get(),
-                               items.start.get(), group_keys.get(), group_weights.get(), group_begin.get(),
-                               group_end.get());
+                               group_keys.get(), group_weights.get());
         }
         const unsigned grid = static_cast<unsigned>(std::min<std::int64_t>(items.bound, kPartLaunchCap));
         // connection sums fit 32 bits when every row weight does
