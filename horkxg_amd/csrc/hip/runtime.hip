@@ -137,8 +137,9 @@ void wait_status(gpart_ctx* ctx, std::uint64_t seq) {
     const auto start = clock::now();
     for (unsigned spin = 0;; ++spin) {
         if (__atomic_load_n(ctx->status_host, __ATOMIC_ACQUIRE) == seq) return;
-        // poll for up to 200 us: a round's kernels usually finish within that
-        if ((spin & 255) == 255 && clock::now() - start > std::chrono::microseconds(200)) break;
+        // poll for up to 20 ms (rounds of the large levels run for hundreds of us; a
+        // blocking wait on the stream wakes up later than the poll sees the word)
+        if ((spin & 255) == 255 && clock::now() - start > std::chrono::milliseconds(20)) break;
         __builtin_ia32_pause();
     }
     GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream));
