@@ -366,8 +366,18 @@ int gpart_cache_info(gpart_ctx* ctx, gpart_cache_info_t* out) {
     });
 }
 
+// Waits for the context's stream by a status word written after the queued work
+// (polled), falling back to the blocking stream wait, which also reports a failed
+// kernel, when the word does not come.
 int gpart_ctx_synchronize(gpart_ctx* ctx) {
-    return guarded([&] { GPART_HIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+    return guarded([&] {
+        const std::uint64_t seq = ++ctx->status_seq;
+        hipLaunchKernelGGL(status_copy_kernel, dim3(1), dim3(kWave), 0, ctx->stream,
+                           static_cast<const std::uint8_t*>(nullptr), std::int64_t{0},
+                           reinterpret_cast<unsigned long long*>(ctx->status_device), ctx->status_counter);
+        GPART_LAUNCH_CHECK();
+        wait_status(ctx, seq);
+    });
 }
 
 void* gpart_ctx_stream(gpart_ctx* ctx) { return ctx->stream; }
