@@ -911,6 +911,16 @@ struct SegOneLoad {
         return SegValue{1, static_cast<unsigned>(i == 0 || targets[i] != targets[i - 1])};
     }
 };
+// Exclusive prefix per item, and the total after the last item.
+struct ExclusiveTotalStore {
+    std::int64_t* out;
+    std::int64_t n;
+    std::int64_t* total;
+    __device__ __forceinline__ void operator()(std::int64_t i, std::int64_t prefix, std::int64_t item) const {
+        out[i] = prefix;
+        if (i == n - 1) *total = prefix + item;
+    }
+};
 struct SegInclusiveStore {
     std::uint64_t* out;
     __device__ __forceinline__ void operator()(std::int64_t i, const SegValue& prefix, const SegValue& item) const {
@@ -1614,20 +1624,26 @@ int gpart_sclp_two_hop(gpart_ctx* ctx, int64_t n, const int32_t* proposals, cons
             hipLaunchKernelGGL(pair_flags_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
                                ordered_weights.get(), ranks.get(), capacity, pair_flags.get());
             GPART_LAUNCH_CHECK();
-            exclusive_scan<std::int64_t>(ctx, pair_flags.get(), positions.get(), p);
-            const std::int64_t last_pos = read_scalar(ctx, positions.get() + p - 1);
-            const std::int64_t last_flag = read_scalar(ctx, pair_flags.get() + p - 1);
-            const std::int64_t pair_count = last_pos + last_flag;
+            // positions and, in one read-back, the pair count
+            device_scan<std::int64_t, SumOp<std::int64_t>>(ctx, ArrayLoad<std::int64_t>{pair_flags.get()},
+                                                           ExclusiveTotalStore{positions.get(), p, d_count.get()},
+                                                           p, SumOp<std::int64_t>{});
+            const std::int64_t pair_count = read_scalar(ctx, d_count.get());
             *pairable = pair_count;
             const auto budget = static_cast<std::uint64_t>(std::max<std::int64_t>(0, merge_budget));
             if (pair_count > 0 && budget > 0) {
-                Scratch<unsigned long long> m(ctx, 1);
-                fill_bytes(ctx, m.get(), 0, 8);
+                Scratch<unsigned long long> m(ctx, 2);  // merged pairs, non-empty clusters
+                fill_bytes(ctx, m.get(), 0, 16);
                 hipLaunchKernelGGL(two_hop_commit_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
                                    pair_flags.get(), positions.get(), budget, ordered_weights.get(), clusters, cw,
                                    m.get());
                 GPART_LAUNCH_CHECK();
-                *merged = static_cast<std::int64_t>(read_scalar(ctx, m.get()));
+                device_reduce<unsigned long long>(ctx, NonzeroLoad{cw}, n, SumU64{}, m.get() + 1);
+                unsigned long long h[2];
+                read_back(ctx, h, m.get(), sizeof(h));
+                *merged = static_cast<std::int64_t>(h[0]);
+                *nonempty = static_cast<std::int64_t>(h[1]);
+                return;
             }
         }
         *nonempty = static_cast<std::int64_t>(count_nonempty(ctx, n, cw));

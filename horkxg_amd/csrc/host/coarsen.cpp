@@ -22,6 +22,13 @@ template <typename Types>
 struct SclpWorkspace<Types>::Impl {
     bool record_bound_known = false;
     bool sorted_unique_rows = false;
+    // Vertex-weight statistics of the next graph to aggregate when it is the last
+    // contraction's output: the total is the finer level's (contraction keeps it),
+    // the maximum is the largest cluster weight, and every weight is a sum of
+    // positive weights, so the graph needs no reduction pass for them.
+    bool weight_stats_known = false;
+    std::uint64_t known_total_weight = 0;
+    std::uint64_t known_maximum_weight = 0;
     std::uint64_t maximum_row_weight = 0;
     DeviceVector<std::uint64_t> row_bounds;
     DeviceVector<std::uint64_t> next_row_bounds;
@@ -113,7 +120,15 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
     constexpr double two_hop_threshold = 0.60;
     const auto view = graph.view();
     std::uint64_t total_weight = 0, maximum_vertex = 0, minimum_vertex = 0;
-    GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &total_weight, &maximum_vertex, &minimum_vertex));
+    if (ws.weight_stats_known) {
+        total_weight = ws.known_total_weight;
+        maximum_vertex = ws.known_maximum_weight;
+        minimum_vertex = 1;  // a lower bound: sums of positive weights
+        ws.weight_stats_known = false;
+    } else {
+        GPART_CHECK(gpart_vertex_weight_stats(ctx, &view, &total_weight, &maximum_vertex, &minimum_vertex));
+    }
+    ws.known_total_weight = total_weight;
     if (minimum_vertex == 0)
         throw std::invalid_argument("SCLP coarsening requires positive vertex weights");
     const auto denominator = static_cast<std::uint64_t>(cluster_size_factor) * static_cast<std::uint64_t>(parts);
@@ -290,6 +305,8 @@ DeviceWeightedGraph<Types> contract_impl(const DeviceWeightedGraph<Types>& fine,
         ws.row_bounds.swap(ws.next_row_bounds);
     }
     ws.forget_classes();
+    ws.known_maximum_weight = agg.maximum_weight;
+    ws.weight_stats_known = true;
     timings.total_seconds = since(start);
     return coarse;
 }
