@@ -100,33 +100,37 @@ __device__ __forceinline__ void decode(std::int32_t v, std::int32_t source, unsi
 
 // ---------------------------------------------------------------- classes
 
-// Three-way stable split of vertex ids by degree class, as one scan.
-struct Class3 {
+// Three-way stable split of vertex ids by degree class, as one scan that also
+// carries the level's isolated-vertex count, the high rows' entries and the
+// largest degree (prefix sums and a prefix maximum).
+struct DegreeScan {
     long long c[3];
+    long long isolated;
+    long long high_edges;
+    long long max_degree;
 };
-struct Class3Op {
-    __device__ __forceinline__ Class3 identity() const { return Class3{{0, 0, 0}}; }
-    __device__ __forceinline__ Class3 operator()(const Class3& a, const Class3& b) const {
-        return Class3{{a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]}};
+struct DegreeScanOp {
+    __device__ __forceinline__ DegreeScan identity() const { return DegreeScan{{0, 0, 0}, 0, 0, 0}; }
+    __device__ __forceinline__ DegreeScan operator()(const DegreeScan& a, const DegreeScan& b) const {
+        return DegreeScan{{a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]},
+                          a.isolated + b.isolated,
+                          a.high_edges + b.high_edges,
+                          a.max_degree > b.max_degree ? a.max_degree : b.max_degree};
     }
 };
-
-template <typename O>
-__device__ __forceinline__ int degree_class(const O* offsets, std::int64_t v) {
-    const std::int64_t deg = static_cast<std::int64_t>(offsets[v + 1] - offsets[v]);
-    return deg <= kLowDegree ? 0 : deg <= kMediumDegree ? 1 : 2;
-}
 
 template <typename O>
 struct ClassLoad {
     const O* offsets;
-    __device__ __forceinline__ Class3 operator()(std::int64_t v) const {
-        Class3 r{{0, 0, 0}};
-        r.c[degree_class(offsets, v)] = 1;
+    __device__ __forceinline__ DegreeScan operator()(std::int64_t v) const {
+        const auto deg = static_cast<long long>(offsets[v + 1] - offsets[v]);
+        DegreeScan r{{0, 0, 0}, deg == 0 ? 1 : 0, deg > kMediumDegree ? deg : 0, deg};
+        r.c[deg <= kLowDegree ? 0 : deg <= kMediumDegree ? 1 : 2] = 1;
         return r;
     }
 };
 
+// counts: low, medium, high, high entries, largest degree, isolated vertices.
 template <typename O>
 struct ClassStore {
     const O* offsets;
@@ -135,29 +139,18 @@ struct ClassStore {
     std::int32_t* high;
     std::int64_t n;
     unsigned long long* counts;
-    __device__ __forceinline__ void operator()(std::int64_t v, const Class3& prefix, const Class3& item) const {
+    __device__ __forceinline__ void operator()(std::int64_t v, const DegreeScan& prefix,
+                                               const DegreeScan& item) const {
         if (item.c[0]) low[prefix.c[0]] = static_cast<std::int32_t>(v);
         else if (item.c[1]) medium[prefix.c[1]] = static_cast<std::int32_t>(v);
         else high[prefix.c[2]] = static_cast<std::int32_t>(v);
-        if (v == n - 1)
-            for (int j = 0; j < 3; ++j) counts[j] = static_cast<unsigned long long>(prefix.c[j] + item.c[j]);
-    }
-};
-
-template <typename O>
-struct DegreeLoad {
-    const O* offsets;
-    __device__ __forceinline__ unsigned long long operator()(std::int64_t v) const {
-        return static_cast<unsigned long long>(offsets[v + 1] - offsets[v]);
-    }
-};
-
-template <typename O>
-struct HubEdgeLoad {
-    const O* offsets;
-    __device__ __forceinline__ unsigned long long operator()(std::int64_t v) const {
-        const auto d = static_cast<unsigned long long>(offsets[v + 1] - offsets[v]);
-        return d > static_cast<unsigned long long>(kMediumDegree) ? d : 0;
+        if (v == n - 1) {
+            const DegreeScan total = DegreeScanOp{}(prefix, item);
+            for (int j = 0; j < 3; ++j) counts[j] = static_cast<unsigned long long>(total.c[j]);
+            counts[3] = static_cast<unsigned long long>(total.high_edges);
+            counts[4] = static_cast<unsigned long long>(total.max_degree);
+            counts[5] = static_cast<unsigned long long>(total.isolated);
+        }
     }
 };
 
@@ -1383,23 +1376,22 @@ void launch_proposals(gpart_ctx* ctx, const gpart_csr_view* g, const std::int32_
 
 template <typename O, typename W>
 void build_classes(gpart_ctx* ctx, const gpart_csr_view* g, gpart_degree_classes* c) {
-    Scratch<unsigned long long> counters(ctx, 5);
-    fill_bytes(ctx, counters.get(), 0, 40);
+    Scratch<unsigned long long> counters(ctx, 6);
+    fill_bytes(ctx, counters.get(), 0, 48);
     if (g->n > 0) {
         const auto* off = static_cast<const O*>(g->offsets);
-        device_scan<Class3, Class3Op>(ctx, ClassLoad<O>{off},
-                                      ClassStore<O>{off, c->low, c->medium, c->high, g->n, counters.get()}, g->n,
-                                      Class3Op{});
-        device_reduce<unsigned long long>(ctx, HubEdgeLoad<O>{off}, g->n, SumU64{}, counters.get() + 3);
-        device_reduce<unsigned long long>(ctx, DegreeLoad<O>{off}, g->n, MaxU64{}, counters.get() + 4);
+        device_scan<DegreeScan, DegreeScanOp>(
+            ctx, ClassLoad<O>{off}, ClassStore<O>{off, c->low, c->medium, c->high, g->n, counters.get()}, g->n,
+            DegreeScanOp{});
     }
-    unsigned long long h[5];
+    unsigned long long h[6];
     read_back(ctx, h, counters.get(), sizeof(h));
     c->low_count = static_cast<std::int64_t>(h[0]);
     c->medium_count = static_cast<std::int64_t>(h[1]);
     c->high_count = static_cast<std::int64_t>(h[2]);
     c->high_edges = static_cast<std::int64_t>(h[3]);
     c->max_degree = static_cast<std::int64_t>(h[4]);
+    c->isolated_count = static_cast<std::int64_t>(h[5]);
 }
 
 template <typename O, typename W>
@@ -1409,7 +1401,7 @@ void record_bounds_impl(gpart_ctx* ctx, const gpart_csr_view* g, std::uint64_t* 
     fill_bytes(ctx, meta.get(), 0, 16);
     if (g->n > 0) {
         Scratch<std::int32_t> low(ctx, g->n), medium(ctx, g->n), high(ctx, g->n);
-        gpart_degree_classes c{low.get(), medium.get(), high.get(), 0, 0, 0, 0, 0};
+        gpart_degree_classes c{low.get(), medium.get(), high.get(), 0, 0, 0, 0, 0, 0};
         build_classes<O, W>(ctx, g, &c);
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);

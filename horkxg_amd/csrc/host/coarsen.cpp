@@ -44,7 +44,7 @@ struct SclpWorkspace<Types>::Impl {
             low.resize_discard(static_cast<std::size_t>(std::max<std::int64_t>(n, 1)));
             medium.resize_discard(static_cast<std::size_t>(std::max<std::int64_t>(n, 1)));
             high.resize_discard(static_cast<std::size_t>(std::max<std::int64_t>(n, 1)));
-            classes = gpart_degree_classes{low.data(), medium.data(), high.data(), 0, 0, 0, 0, 0};
+            classes = gpart_degree_classes{low.data(), medium.data(), high.data(), 0, 0, 0, 0, 0, 0};
             const auto view = g.view();
             GPART_CHECK(gpart_degree_classes_build(detail::context(), &view, &classes));
             classes_for = g.offsets.data();
@@ -197,7 +197,8 @@ DeviceAggregateResult<Types> aggregate(const DeviceWeightedGraph<Types>& graph, 
         if (static_cast<double>(cluster_count) <= 0.60 * n) break;
     }
 
-    if (pack_isolated) {
+    // a level without two isolated vertices has nothing to pack
+    if (pack_isolated && classes.isolated_count > 1) {
         std::int64_t isolated = 0, nonempty = cluster_count;
         GPART_CHECK(gpart_sclp_isolated(ctx, &view, vertex_weights64.data(), stats.capacity, clusters.data(),
                                         cluster_weights.data(), &isolated, &nonempty));

@@ -86,7 +86,7 @@ RefineStats run_refinement(Level<Types>& level, const RefineOptions& options, in
     const auto N = static_cast<std::size_t>(n);
 
     DeviceVector<std::int32_t> low(N), medium(N), high(N);
-    gpart_degree_classes classes{low.data(), medium.data(), high.data(), 0, 0, 0, 0, 0};
+    gpart_degree_classes classes{low.data(), medium.data(), high.data(), 0, 0, 0, 0, 0, 0};
     GPART_CHECK(gpart_degree_classes_build(ctx, &view, &classes));
     // Cache entries are 4 bytes when the level's total edge weight, a bound on
     // every row sum, fits 32 bits; otherwise 8.

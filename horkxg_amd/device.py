@@ -69,7 +69,7 @@ class CsrView(C.Structure):
 class DegreeClasses(C.Structure):
     _fields_ = [("low", C.c_void_p), ("medium", C.c_void_p), ("high", C.c_void_p), ("low_count", C.c_int64),
                 ("medium_count", C.c_int64), ("high_count", C.c_int64), ("high_edges", C.c_int64),
-                ("max_degree", C.c_int64)]
+                ("max_degree", C.c_int64), ("isolated_count", C.c_int64)]
 
 
 class SclpRound(C.Structure):
@@ -93,7 +93,7 @@ class DeviceGraph:
         self.view = CsrView(n, m, WIDTHS[width], self.off.ptr, self.adj.ptr, self.ew.ptr, self.vw.ptr)
         self.n, self.m = n, m
         self.lists = [DeviceArray(dtype=np.int32, count=max(n, 1)) for _ in range(3)]
-        self.classes = DegreeClasses(self.lists[0].ptr, self.lists[1].ptr, self.lists[2].ptr, 0, 0, 0, 0, 0)
+        self.classes = DegreeClasses(self.lists[0].ptr, self.lists[1].ptr, self.lists[2].ptr, 0, 0, 0, 0, 0, 0)
         _check(hip_lib().gpart_degree_classes_build(_ctx(), C.byref(self.view), C.byref(self.classes)))
 
 

@@ -161,6 +161,7 @@ typedef struct gpart_degree_classes {
     int64_t high_count;
     int64_t high_edges;  /* sum of degrees of high vertices */
     int64_t max_degree;
+    int64_t isolated_count;  /* vertices of degree 0 */
 } gpart_degree_classes;
 
 /* host out: counts. */
