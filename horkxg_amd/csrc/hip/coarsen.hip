@@ -655,9 +655,10 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_offsets_kernel(
         const std::int64_t first = split_start[i];
         if (j != first) continue;  // the same j in every thread of the workgroup
         const int parts = static_cast<int>(split_start[i + 1] - first);
+        std::uint32_t* column = chunk_counts + first * kMaxParts + threadIdx.x;  // group g's count in chunk 0
         if (static_cast<int>(threadIdx.x) < parts) {
             unsigned int size = 0;
-            for (int p = 0; p < parts; ++p) size += chunk_counts[(first + p) * kMaxParts + threadIdx.x];
+            for (int p = 0; p < parts; ++p) size += column[static_cast<std::int64_t>(p) * kMaxParts];
             sizes[threadIdx.x] = size;
         }
         __syncthreads();
@@ -667,11 +668,18 @@ __global__ void __launch_bounds__(kPartThreads) sclp_split_offsets_kernel(
             const std::int64_t base = region[i];
             group_begin[item_start[i] + threadIdx.x] = base + at;
             group_end[item_start[i] + threadIdx.x] = base + at + sizes[threadIdx.x];
-            for (int p = 0; p < parts; ++p) {
-                std::uint32_t* slot = chunk_counts + (first + p) * kMaxParts + threadIdx.x;
-                const unsigned int n = *slot;
-                *slot = at;
-                at += n;
+            // eight chunks' counts loaded before any is overwritten, so the loads overlap
+            constexpr int kBatch = 8;
+            for (int p0 = 0; p0 < parts; p0 += kBatch) {
+                unsigned int n[kBatch];
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q)
+                    n[q] = p0 + q < parts ? column[static_cast<std::int64_t>(p0 + q) * kMaxParts] : 0;
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) {
+                    if (p0 + q < parts) column[static_cast<std::int64_t>(p0 + q) * kMaxParts] = at;
+                    at += n[q];
+                }
             }
         }
         __syncthreads();  // sizes is reused by the next row
