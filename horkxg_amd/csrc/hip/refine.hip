@@ -379,24 +379,19 @@ __global__ void priority_filter_hub_finish_kernel(std::int64_t count, const std:
     sums[i] = 0;
 }
 
-// incremental_update_cache_and_cut (:69-95) over a mover list whose length is on the device.
+// incremental_update_cache_and_cut (:69-95) over a mover list whose length is on
+// the device, G lanes per mover, by the workgroups [0, blocks) numbered `block`;
+// the block's gain sum goes to *delta. Called by every thread of the workgroup.
 template <typename O, typename W, int G, typename C>
-__global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* count_ptr, const std::int32_t* movers,
-                                                              const O* offsets, const std::int32_t* neighbors,
-                                                              const W* weights, const std::int32_t* labels,
-                                                              const std::int32_t* targets,
-                                                              const std::int32_t* flags, int k,
-                                                              C* cache, long long* delta,
-                                                              const long long* edges_ptr, long long rebuild_above) {
-    __shared__ long long block_sum[kBlock / kWave];
-    // a rebuild round recomputes cache and cut from the CSR after the commit
-    if (rebuild_above >= 0 && *edges_ptr > rebuild_above) return;
-    const long long count = *count_ptr;
+__device__ __forceinline__ void update_cache_movers(long long count, const std::int32_t* movers, const O* offsets,
+                                                    const std::int32_t* neighbors, const W* weights,
+                                                    const std::int32_t* labels, const std::int32_t* targets,
+                                                    const std::int32_t* flags, int k, C* cache, long long* delta,
+                                                    long long block, long long blocks, long long* block_sum) {
     const int sub = threadIdx.x & (G - 1);
-    const long long groups = static_cast<long long>(gridDim.x) * (kBlock / G);
+    const long long groups = blocks * (kBlock / G);
     long long gain = 0;
-    for (long long group = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) / G; group < count;
-         group += groups) {
+    for (long long group = (block * blockDim.x + threadIdx.x) / G; group < count; group += groups) {
         const std::int32_t v = movers[group];
         const std::int32_t source = labels[v];
         const std::int32_t target = targets[v];
@@ -423,6 +418,28 @@ __global__ void __launch_bounds__(kBlock) update_cache_kernel(const long long* c
         for (int w = 0; w < kBlock / kWave; ++w) total += block_sum[w];
         if (total) atomicAdd(reinterpret_cast<unsigned long long*>(delta), static_cast<unsigned long long>(total));
     }
+}
+
+// The light movers (8 lanes each) on workgroups [0, light_blocks), and, when
+// heavy is given, the heavy movers (a wave each) on the rest of the grid.
+template <typename O, typename W, typename C>
+__global__ void __launch_bounds__(kBlock) update_cache_kernel(const RoundState* st, const std::int32_t* light,
+                                                              const std::int32_t* heavy, unsigned light_blocks,
+                                                              const O* offsets, const std::int32_t* neighbors,
+                                                              const W* weights, const std::int32_t* labels,
+                                                              const std::int32_t* targets,
+                                                              const std::int32_t* flags, int k,
+                                                              C* cache, long long* delta, long long rebuild_above) {
+    __shared__ long long block_sum[kBlock / kWave];
+    // a rebuild round recomputes cache and cut from the CSR after the commit
+    if (rebuild_above >= 0 && st->edges > rebuild_above) return;
+    if (blockIdx.x < light_blocks)
+        update_cache_movers<O, W, 8, C>(st->light, light, offsets, neighbors, weights, labels, targets, flags, k,
+                                        cache, delta, blockIdx.x, light_blocks, block_sum);
+    else
+        update_cache_movers<O, W, kWave, C>(st->heavy, heavy, offsets, neighbors, weights, labels, targets, flags,
+                                            k, cache, delta, blockIdx.x - light_blocks, gridDim.x - light_blocks,
+                                            block_sum);
 }
 
 // Heavy movers (degree > kHeavyDegree) as one flattened entry range: heavy_start
@@ -1423,18 +1440,18 @@ struct Ops {
         const auto light_blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * 8), 4 * kRoundBlocks));
         run.phase(GPART_PROFILE_CACHE_UPDATE, [&] {
         profile_begin(ctx, GPART_PROFILE_CACHE_UPDATE);
-        hipLaunchKernelGGL((update_cache_kernel<O, W, 8, C>), dim3(light_blocks), dim3(kBlock), 0, ctx->stream,
-                           &st->light, light, off, g->neighbors, w, labels, targets, flags, k, cache, &st->delta,
-                           &st->edges, rebuild_above);
-        if (c->max_degree > kHeavyDegree && c->max_degree <= kWaveHeavyMaxDegree) {
-            const std::int64_t heavy_bound = c->medium_count + c->high_count;
-            const auto heavy_blocks = static_cast<unsigned>(std::max<std::int64_t>(
-                1, std::min<std::int64_t>(grid_for(heavy_bound * kWave), 4 * kRoundBlocks)));
-            hipLaunchKernelGGL((update_cache_kernel<O, W, kWave, C>), dim3(heavy_blocks), dim3(kBlock), 0,
-                               ctx->stream, &st->heavy, heavy, off, g->neighbors, w, labels, targets, flags, k, cache,
-                               &st->delta, &st->edges, rebuild_above);
-        } else if (c->max_degree > kHeavyDegree) {
-            const std::int64_t heavy_bound = c->medium_count + c->high_count;
+        // levels whose degrees stay at most kWaveHeavyMaxDegree update their heavy
+        // movers a wave each, in the same launch
+        const bool wave_heavy = c->max_degree > kHeavyDegree && c->max_degree <= kWaveHeavyMaxDegree;
+        const std::int64_t heavy_bound = c->medium_count + c->high_count;
+        const unsigned heavy_blocks =
+            wave_heavy ? static_cast<unsigned>(std::max<std::int64_t>(
+                             1, std::min<std::int64_t>(grid_for(heavy_bound * kWave), 4 * kRoundBlocks)))
+                       : 0;
+        hipLaunchKernelGGL((update_cache_kernel<O, W, C>), dim3(light_blocks + heavy_blocks), dim3(kBlock), 0,
+                           ctx->stream, st, light, wave_heavy ? heavy : nullptr, light_blocks, off, g->neighbors, w,
+                           labels, targets, flags, k, cache, &st->delta, rebuild_above);
+        if (!wave_heavy && c->max_degree > kHeavyDegree) {
             Scratch<long long> heavy_start(ctx, heavy_bound + 1);
             Scratch<std::int32_t> tile_mover(ctx, tile_count(g->m) + 1);
             device_scan<long long, SumOp<long long>>(ctx, HeavyDegreeLoad<O>{st, heavy, off},
