@@ -301,14 +301,16 @@ __global__ void __launch_bounds__(kBlock) lp_proposals_kernel(std::int64_t n, co
     gains[v] = gain;
 }
 
+// incremental_priority_filter (:48-67) over one class list with G lanes per row,
+// by the workgroups numbered `block` of the list's range. Called by every thread
+// of the workgroup.
 template <typename O, typename W, int G>
-__global__ void __launch_bounds__(kBlock) priority_filter_kernel(std::int64_t count, const std::int32_t* list,
-                                                                 const O* offsets, const std::int32_t* neighbors,
-                                                                 const W* weights, const std::int32_t* labels,
-                                                                 const std::int32_t* targets, const long long* gains,
-                                                                 std::int32_t* flags) {
-    __shared__ long long block_sum[kBlock / kWave];
-    const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+__device__ __forceinline__ void priority_filter_rows(std::int64_t count, const std::int32_t* list, long long block,
+                                                     const O* offsets, const std::int32_t* neighbors,
+                                                     const W* weights, const std::int32_t* labels,
+                                                     const std::int32_t* targets, const long long* gains,
+                                                     std::int32_t* flags, long long* block_sum) {
+    const std::int64_t group = (block * blockDim.x + threadIdx.x) / G;
     const int sub = threadIdx.x & (G - 1);
     const bool active = group < count;
     const std::int32_t v = active ? (list ? list[group] : static_cast<std::int32_t>(group)) : 0;
@@ -333,6 +335,28 @@ __global__ void __launch_bounds__(kBlock) priority_filter_kernel(std::int64_t co
             flags[v] = target >= 0 && gains[v] + total >= 0;
         }
     }
+}
+
+// The three degree classes in one launch: workgroups [0, low_blocks) take the low
+// rows (8 lanes each), the next medium_blocks the medium rows (a wave each), the
+// rest the high rows (a workgroup each).
+template <typename O, typename W>
+__global__ void __launch_bounds__(kBlock) priority_filter_kernel(
+    std::int64_t low_count, const std::int32_t* low, unsigned low_blocks, std::int64_t medium_count,
+    const std::int32_t* medium, unsigned medium_blocks, std::int64_t high_count, const std::int32_t* high,
+    const O* offsets, const std::int32_t* neighbors, const W* weights, const std::int32_t* labels,
+    const std::int32_t* targets, const long long* gains, std::int32_t* flags) {
+    __shared__ long long block_sum[kBlock / kWave];
+    const long long b = blockIdx.x;
+    if (b < low_blocks)
+        priority_filter_rows<O, W, 8>(low_count, low, b, offsets, neighbors, weights, labels, targets, gains, flags,
+                                      block_sum);
+    else if (b < low_blocks + medium_blocks)
+        priority_filter_rows<O, W, 64>(medium_count, medium, b - low_blocks, offsets, neighbors, weights, labels,
+                                       targets, gains, flags, block_sum);
+    else
+        priority_filter_rows<O, W, kBlock>(high_count, high, b - low_blocks - medium_blocks, offsets, neighbors,
+                                           weights, labels, targets, gains, flags, block_sum);
 }
 
 // Hub rows (degree > kHubChunk): each chunk adds its share of the correction to
@@ -1395,16 +1419,12 @@ struct Ops {
         const auto* off = static_cast<const O*>(g->offsets);
         const auto* w = static_cast<const W*>(g->edge_weights);
         profile_begin(ctx, GPART_PROFILE_REFINE_FILTER);
-        if (c->low_count)
-            hipLaunchKernelGGL((priority_filter_kernel<O, W, 8>), dim3(grid_for(c->low_count * 8)), dim3(kBlock), 0,
-                               ctx->stream, c->low_count, c->low, off, g->neighbors, w, labels, targets, gains, flags);
-        if (c->medium_count)
-            hipLaunchKernelGGL((priority_filter_kernel<O, W, 64>), dim3(grid_for(c->medium_count * 64)), dim3(kBlock),
-                               0, ctx->stream, c->medium_count, c->medium, off, g->neighbors, w, labels, targets, gains,
-                               flags);
-        if (c->high_count)
-            hipLaunchKernelGGL((priority_filter_kernel<O, W, kBlock>), dim3(static_cast<unsigned>(c->high_count)),
-                               dim3(kBlock), 0, ctx->stream, c->high_count, c->high, off, g->neighbors, w, labels,
+        const unsigned low_blocks = grid_for(c->low_count * 8), medium_blocks = grid_for(c->medium_count * 64);
+        const auto high_blocks = static_cast<unsigned>(c->high_count);
+        if (low_blocks + medium_blocks + high_blocks > 0)
+            hipLaunchKernelGGL((priority_filter_kernel<O, W>), dim3(low_blocks + medium_blocks + high_blocks),
+                               dim3(kBlock), 0, ctx->stream, c->low_count, c->low, low_blocks, c->medium_count,
+                               c->medium, medium_blocks, c->high_count, c->high, off, g->neighbors, w, labels,
                                targets, gains, flags);
         if (c->max_degree > kHubChunk) {
             HubChunks chunks;
