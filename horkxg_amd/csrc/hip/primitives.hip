@@ -141,9 +141,95 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const std::
 // Positions are distinct, so the order is total and equal digits keep their
 // input order, as in the multi-pass sort. The m - n padding entries carry the
 // largest digits and positions and end up past the n real ones.
+//
+// For m >= 2 * threads each thread owns R = m / threads consecutive entries: the
+// network's stages with strides below R compare entries of one thread and run in
+// registers (one LDS round trip per merge size instead of one per stage); the
+// longer strides go through LDS by pairs.
 constexpr int kLdsSortMax = 4 * kSortTile;
 constexpr int kLdsSortThreads = 1024;
 static_assert(kLdsSortMax <= 65536, "positions are 16-bit");
+
+// Compare-exchange of entries i < j of the network's stage for merge size `size`:
+// ascending where bit `size` of i is clear.
+__device__ __forceinline__ void bitonic_exchange(std::uint64_t& di, std::uint16_t& pi, std::uint64_t& dj,
+                                                 std::uint16_t& pj, bool ascending) {
+    const bool greater = di > dj || (di == dj && pi > pj);
+    if (greater == ascending) {
+        const std::uint64_t d = di;
+        di = dj;
+        dj = d;
+        const std::uint16_t q = pi;
+        pi = pj;
+        pj = q;
+    }
+}
+
+// The stages of merge size `size` with strides below R, on one thread's R entries
+// starting at index base (sizes below R run every stride, from size / 2 down).
+template <int R>
+__device__ __forceinline__ void bitonic_registers(std::uint64_t (&d)[R], std::uint16_t (&q)[R], int base,
+                                                  int size) {
+#pragma unroll
+    for (int stride = R / 2; stride > 0; stride >>= 1) {
+        if (stride >= size) continue;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int j = i ^ stride;
+            if (j > i) bitonic_exchange(d[i], q[i], d[j], q[j], ((base + i) & size) == 0);
+        }
+    }
+}
+
+template <typename V, bool HasValues, int R>
+__device__ void bitonic_blocked(std::uint64_t* digit, std::uint16_t* position, const std::uint64_t* keys_in, int n,
+                                int m, int shift, std::uint64_t mask) {
+    const int base = threadIdx.x * R;
+    std::uint64_t d[R];
+    std::uint16_t q[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int at = base + i;
+        d[i] = at < n ? (keys_in[at] >> shift) & mask : ~0ULL;
+        q[i] = static_cast<std::uint16_t>(at);
+    }
+    // merge sizes up to R inside the thread
+    for (int size = 2; size <= R; size <<= 1) bitonic_registers<R>(d, q, base, size);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        digit[base + i] = d[i];
+        position[base + i] = q[i];
+    }
+    __syncthreads();
+    for (int size = 2 * R; size <= m; size <<= 1) {
+        for (int stride = size >> 1; stride >= R; stride >>= 1) {
+            for (int p = threadIdx.x; p < (m >> 1); p += kLdsSortThreads) {
+                const int i = ((p & ~(stride - 1)) << 1) | (p & (stride - 1));
+                const int j = i + stride;
+                std::uint64_t di = digit[i], dj = digit[j];
+                std::uint16_t pi = position[i], pj = position[j];
+                bitonic_exchange(di, pi, dj, pj, (i & size) == 0);
+                digit[i] = di;
+                digit[j] = dj;
+                position[i] = pi;
+                position[j] = pj;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            d[i] = digit[base + i];
+            q[i] = position[base + i];
+        }
+        bitonic_registers<R>(d, q, base, size);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            digit[base + i] = d[i];
+            position[base + i] = q[i];
+        }
+        __syncthreads();
+    }
+}
 
 template <typename V, bool HasValues>
 __global__ void __launch_bounds__(kLdsSortThreads) lds_sort_kernel(const std::uint64_t* keys_in,
@@ -152,27 +238,33 @@ __global__ void __launch_bounds__(kLdsSortThreads) lds_sort_kernel(const std::ui
                                                                    std::uint64_t mask) {
     __shared__ std::uint64_t digit[kLdsSortMax];
     __shared__ std::uint16_t position[kLdsSortMax];
-    for (int i = threadIdx.x; i < m; i += kLdsSortThreads) {
-        digit[i] = i < n ? (keys_in[i] >> shift) & mask : ~0ULL;
-        position[i] = static_cast<std::uint16_t>(i);
-    }
-    __syncthreads();
-    for (int size = 2; size <= m; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int q = threadIdx.x; q < (m >> 1); q += kLdsSortThreads) {
-                const int i = ((q & ~(stride - 1)) << 1) | (q & (stride - 1));
-                const int j = i + stride;
-                const std::uint64_t di = digit[i], dj = digit[j];
-                const std::uint16_t pi = position[i], pj = position[j];
-                const bool greater = di > dj || (di == dj && pi > pj);
-                if (greater == ((i & size) == 0)) {  // ascending runs where bit `size` of i is clear
-                    digit[i] = dj;
-                    digit[j] = di;
-                    position[i] = pj;
-                    position[j] = pi;
+    if (m >= 8 * kLdsSortThreads) {
+        bitonic_blocked<V, HasValues, 8>(digit, position, keys_in, n, m, shift, mask);
+    } else if (m >= 4 * kLdsSortThreads) {
+        bitonic_blocked<V, HasValues, 4>(digit, position, keys_in, n, m, shift, mask);
+    } else if (m >= 2 * kLdsSortThreads) {
+        bitonic_blocked<V, HasValues, 2>(digit, position, keys_in, n, m, shift, mask);
+    } else {
+        for (int i = threadIdx.x; i < m; i += kLdsSortThreads) {
+            digit[i] = i < n ? (keys_in[i] >> shift) & mask : ~0ULL;
+            position[i] = static_cast<std::uint16_t>(i);
+        }
+        __syncthreads();
+        for (int size = 2; size <= m; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int p = threadIdx.x; p < (m >> 1); p += kLdsSortThreads) {
+                    const int i = ((p & ~(stride - 1)) << 1) | (p & (stride - 1));
+                    const int j = i + stride;
+                    std::uint64_t di = digit[i], dj = digit[j];
+                    std::uint16_t pi = position[i], pj = position[j];
+                    bitonic_exchange(di, pi, dj, pj, (i & size) == 0);
+                    digit[i] = di;
+                    digit[j] = dj;
+                    position[i] = pi;
+                    position[j] = pj;
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
     for (int r = threadIdx.x; r < n; r += kLdsSortThreads) {
