@@ -24,6 +24,9 @@ constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kSortItems = 8;  // per lane
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kDigits = 256;
+// Sorts of at most this many tiles scan the digit histograms inside each scatter
+// workgroup (at most 256 x 64 loads) instead of two scan launches per pass.
+constexpr std::int64_t kInlineHistogramTiles = 64;
 
 __device__ __forceinline__ std::int64_t tile_item(int wave, int it, int lane) {
     return static_cast<std::int64_t>(wave) * (kWave * kSortItems) + it * kWave + lane;
@@ -55,11 +58,15 @@ __global__ void __launch_bounds__(kSortThreads) radix_histogram_kernel(const std
     }
 }
 
+// offsets: the digit-major exclusive scan of the tiles' histograms, or, with
+// raw_histogram, the histograms themselves, which each workgroup then scans
+// for its own tile (grids of at most kInlineHistogramTiles tiles).
 template <typename V, bool HasValues>
 __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const std::uint64_t* keys_in, std::uint64_t* keys_out,
                                                                      const V* values_in, V* values_out, std::int64_t n,
                                                                      int shift, std::uint64_t mask,
-                                                                     const std::uint32_t* offsets, std::int64_t tiles) {
+                                                                     const std::uint32_t* offsets, std::int64_t tiles,
+                                                                     int raw_histogram) {
     __shared__ std::uint32_t wave_counts[kSortWaves][kDigits];
     __shared__ std::uint32_t digit_start[kDigits + 1];
     __shared__ std::uint32_t global_base[kDigits];
@@ -104,6 +111,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const std::
     __syncthreads();
     // per digit: wave offsets within the tile, digit totals
     std::uint32_t digit_total = 0;
+    std::uint32_t all_tiles = 0, tiles_before = 0;  // raw_histogram: the digit over all tiles, and before this one
     if (threadIdx.x < kDigits) {
         const int d = threadIdx.x;
 #pragma unroll
@@ -112,7 +120,22 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(const std::
             wave_counts[w][d] = digit_total;
             digit_total += c;
         }
-        global_base[d] = offsets[static_cast<std::int64_t>(d) * tiles + blockIdx.x];
+        if (raw_histogram) {
+            const std::uint32_t* row = offsets + static_cast<std::int64_t>(d) * tiles;
+            for (std::int64_t b = 0; b < tiles; ++b) {
+                const std::uint32_t c = row[b];
+                all_tiles += c;
+                tiles_before += b < static_cast<std::int64_t>(blockIdx.x) ? c : 0;
+            }
+        } else {
+            global_base[d] = offsets[static_cast<std::int64_t>(d) * tiles + blockIdx.x];
+        }
+    }
+    if (raw_histogram) {  // the same branch in every thread
+        std::uint32_t everything;
+        const std::uint32_t digit_first = block_exclusive_scan(all_tiles, SumOp<std::uint32_t>{}, &everything,
+                                                               scan_waves);
+        if (threadIdx.x < kDigits) global_base[threadIdx.x] = digit_first + tiles_before;
     }
     std::uint32_t total;
     const std::uint32_t start = block_exclusive_scan(digit_total, SumOp<std::uint32_t>{}, &total, scan_waves);
@@ -303,11 +326,13 @@ bool radix_sort_impl(gpart_ctx* ctx, std::uint64_t* keys, std::uint64_t* keys_al
         hipLaunchKernelGGL(radix_histogram_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads), 0,
                            ctx->stream, src_k, n, shift, mask, hist.get(), tiles);
         GPART_LAUNCH_CHECK();
-        device_scan<std::uint32_t, SumOp<std::uint32_t>>(ctx, ArrayLoad<std::uint32_t>{hist.get()},
-                                                         ExclusiveStore<std::uint32_t>{hist.get()},
-                                                         tiles * kDigits, SumOp<std::uint32_t>{});
+        const bool raw = tiles <= kInlineHistogramTiles;
+        if (!raw)
+            device_scan<std::uint32_t, SumOp<std::uint32_t>>(ctx, ArrayLoad<std::uint32_t>{hist.get()},
+                                                             ExclusiveStore<std::uint32_t>{hist.get()},
+                                                             tiles * kDigits, SumOp<std::uint32_t>{});
         hipLaunchKernelGGL((radix_scatter_kernel<V, HasValues>), dim3(static_cast<unsigned>(tiles)), dim3(kSortThreads),
-                           0, ctx->stream, src_k, dst_k, src_v, dst_v, n, shift, mask, hist.get(), tiles);
+                           0, ctx->stream, src_k, dst_k, src_v, dst_v, n, shift, mask, hist.get(), tiles, raw ? 1 : 0);
         GPART_LAUNCH_CHECK();
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
