@@ -860,9 +860,31 @@ __global__ void hub_table_size_kernel(std::int64_t count, const std::int32_t* li
 
 // ---------------------------------------------------------------- admission
 
-__global__ void valid_flags_kernel(std::int64_t n, const std::int32_t* targets, std::int32_t* flags) {
-    const std::int64_t v = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v < n) flags[v] = targets[v] != kSclpInvalid;
+// The vertices with a valid proposal (or favourite), compacted in id order
+// straight from the proposal array.
+struct ValidLoad {
+    const std::int32_t* targets;
+    __device__ __forceinline__ std::int64_t operator()(std::int64_t v) const {
+        return targets[v] != kSclpInvalid ? 1 : 0;
+    }
+};
+struct IdStore {
+    std::int32_t* out;
+    std::int64_t n;
+    std::int64_t* count;
+    __device__ __forceinline__ void operator()(std::int64_t v, std::int64_t prefix, std::int64_t item) const {
+        if (item) out[prefix] = static_cast<std::int32_t>(v);
+        if (v == n - 1) *count = prefix + item;
+    }
+};
+void compact_valid(gpart_ctx* ctx, std::int64_t n, const std::int32_t* targets, std::int32_t* out,
+                   std::int64_t* d_count) {
+    if (n <= 0) {
+        fill_bytes(ctx, d_count, 0, sizeof(std::int64_t));
+        return;
+    }
+    device_scan<std::int64_t, SumOp<std::int64_t>>(ctx, ValidLoad{targets}, IdStore{out, n, d_count}, n,
+                                                   SumOp<std::int64_t>{});
 }
 
 // Key components of the admission order (target asc, gain desc, tie desc, id asc).
@@ -889,10 +911,12 @@ __global__ void admission_key_kernel(std::int64_t p, const std::int32_t* order, 
     keys[i] = key;
 }
 
+// zero (optional): words that the pass clears for the kernels after it.
 __global__ void gather_order_kernel(std::int64_t p, const std::int32_t* order, const std::int32_t* targets,
                                     const std::uint64_t* vertex_weights, std::int32_t* ordered_targets,
-                                    std::uint64_t* ordered_weights) {
+                                    std::uint64_t* ordered_weights, unsigned long long* zero, int zero_words) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (zero && i < zero_words) zero[i] = 0;
     if (i >= p) return;
     const std::int32_t v = order[i];
     ordered_targets[i] = targets[v];
@@ -1504,12 +1528,9 @@ int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const 
     return guarded([&] {
         auto* cw = reinterpret_cast<unsigned long long*>(cluster_weights);
         const auto* gn = reinterpret_cast<const unsigned long long*>(gains);
-        Scratch<std::int32_t> flags(ctx, n), order(ctx, n), order_alt(ctx, n);
+        Scratch<std::int32_t> order(ctx, n), order_alt(ctx, n);
         Scratch<std::int64_t> d_count(ctx, 1);
-        hipLaunchKernelGGL(valid_flags_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, proposals,
-                           flags.get());
-        GPART_LAUNCH_CHECK();
-        compact_indices(ctx, flags.get(), n, order.get(), d_count.get());
+        compact_valid(ctx, n, proposals, order.get(), d_count.get());
         const std::int64_t p = read_scalar(ctx, d_count.get());
         *valid = p;
         *accepted = 0;
@@ -1518,16 +1539,16 @@ int gpart_sclp_admit(gpart_ctx* ctx, int64_t n, const int32_t* proposals, const 
                                                   gain_bound);
             Scratch<std::int32_t> ordered_targets(ctx, p), accept(ctx, p);
             Scratch<std::uint64_t> ordered_weights(ctx, p), prefix(ctx, p);
+            Scratch<unsigned long long> acc(ctx, 3);  // accepted, predicted gain, non-empty clusters
+            // the gather also clears the commit's two counters
             hipLaunchKernelGGL(gather_order_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
-                               proposals, vertex_weights, ordered_targets.get(), ordered_weights.get());
+                               proposals, vertex_weights, ordered_targets.get(), ordered_weights.get(), acc.get(), 2);
             GPART_LAUNCH_CHECK();
             device_scan<SegValue, SegOp>(ctx, SegWeightLoad{ordered_targets.get(), ordered_weights.get()},
                                          SegInclusiveStore{prefix.get()}, p, SegOp{});
             hipLaunchKernelGGL(admit_decide_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p,
                                ordered_targets.get(), prefix.get(), cw, capacity, accept.get());
             GPART_LAUNCH_CHECK();
-            Scratch<unsigned long long> acc(ctx, 3);  // accepted, predicted gain, non-empty clusters
-            fill_bytes(ctx, acc.get(), 0, 24);
             profile_begin(ctx, GPART_PROFILE_SCLP_COMMIT);
             hipLaunchKernelGGL(admit_apply_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
                                ordered_targets.get(), ordered_weights.get(), accept.get(),
@@ -1598,14 +1619,12 @@ int gpart_sclp_two_hop(gpart_ctx* ctx, int64_t n, const int32_t* proposals, cons
     return guarded([&] {
         auto* cw = reinterpret_cast<unsigned long long*>(cluster_weights);
         const auto* gn = reinterpret_cast<const unsigned long long*>(gains);
-        Scratch<std::int32_t> favorites(ctx, n), flags(ctx, n), order(ctx, n), order_alt(ctx, n);
+        Scratch<std::int32_t> favorites(ctx, n), order(ctx, n), order_alt(ctx, n);
         Scratch<std::int64_t> d_count(ctx, 1);
         hipLaunchKernelGGL(favorite_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, vertex_weights, cw,
                            clusters, proposals, favorites.get());
-        hipLaunchKernelGGL(valid_flags_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, favorites.get(),
-                           flags.get());
         GPART_LAUNCH_CHECK();
-        compact_indices(ctx, flags.get(), n, order.get(), d_count.get());
+        compact_valid(ctx, n, favorites.get(), order.get(), d_count.get());
         const std::int64_t p = read_scalar(ctx, d_count.get());
         *candidates = p;
         *pairable = 0;
@@ -1617,7 +1636,8 @@ int gpart_sclp_two_hop(gpart_ctx* ctx, int64_t n, const int32_t* proposals, cons
             Scratch<std::uint64_t> ordered_weights(ctx, p), ranks(ctx, p);
             Scratch<std::int64_t> pair_flags(ctx, p), positions(ctx, p + 1);
             hipLaunchKernelGGL(gather_order_kernel, dim3(grid_for(p)), dim3(kBlock), 0, ctx->stream, p, sorted,
-                               favorites.get(), vertex_weights, ordered_targets.get(), ordered_weights.get());
+                               favorites.get(), vertex_weights, ordered_targets.get(), ordered_weights.get(),
+                               nullptr, 0);
             GPART_LAUNCH_CHECK();
             device_scan<SegValue, SegOp>(ctx, SegOneLoad{ordered_targets.get()}, SegInclusiveStore{ranks.get()}, p,
                                          SegOp{});
