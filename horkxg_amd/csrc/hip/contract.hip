@@ -5,8 +5,9 @@
 //    duplicate keys are summed exactly);
 // 2. LSD radix sort of the records by (source, target): 8-byte keys only in the
 //    packed layout (weight in the low bits), key + value otherwise;
-// 3. one scan fuses segment counting and segmented weight sums; the store
-//    functor emits each (source, target, sum) at its segment end;
+// 3. one scan fuses segment counting and segmented weight sums and emits each
+//    (source, target, sum) at its segment end, staged per tile for coalesced
+//    stores;
 // 4. row offsets are written once per row from row boundaries.
 #include "internal.hpp"
 #include "scan.hpp"
@@ -152,44 +153,192 @@ struct RunOp {
     __device__ __forceinline__ RunValue identity() const { return RunValue{0, 0, 0, 0}; }
 };
 
-template <typename V>
-struct RunLoad {
-    const std::uint64_t* keys;
-    const V* values;
-    int shift;
-    std::uint64_t weight_mask;
-    bool packed;
-    __device__ __forceinline__ RunValue operator()(std::int64_t i) const {
-        const std::uint64_t k = keys[i];
-        const bool head = i == 0 || (keys[i - 1] >> shift) != (k >> shift);
-        const std::uint64_t w = packed ? (k & weight_mask) : static_cast<std::uint64_t>(values[i]);
-        return RunValue{head ? 1ULL : 0ULL, w, head ? 1U : 0U, 0};
-    }
-};
+// The reduce pass runs over the sorted keys in tiles of kScanTile entries. A tile
+// is loaded lane after lane (coalesced) into LDS, one padding slot per
+// kScanItems entries so that the blocked reads below spread over the banks, and
+// each lane then takes kScanItems consecutive entries.
+constexpr int kRunStage = kScanTile + kScanTile / kScanItems;
+__device__ __forceinline__ int run_slot(int local) { return local + local / kScanItems; }
 
-template <typename W>
-struct RunStore {
-    const std::uint64_t* keys;
-    std::int64_t n;
-    int shift;
-    std::uint64_t vertex_mask;
-    int vertex_bits;
-    std::uint32_t* sources;  // unique sources (for offsets)
-    std::int32_t* neighbors;
-    W* weights;
-    unsigned long long* total;
-    __device__ __forceinline__ void operator()(std::int64_t i, const RunValue& prefix, const RunValue& item) const {
-        const RunValue incl = RunOp{}(prefix, item);
-        if (i == n - 1) *total = incl.heads;
-        const std::uint64_t k = keys[i] >> shift;
-        const bool last = i == n - 1 || (keys[i + 1] >> shift) != k;
-        if (!last) return;
-        const std::int64_t j = static_cast<std::int64_t>(incl.heads) - 1;
-        sources[j] = static_cast<std::uint32_t>(k >> vertex_bits);
-        neighbors[j] = static_cast<std::int32_t>(k & vertex_mask);
-        weights[j] = static_cast<W>(incl.value);
+template <typename V, bool Packed>
+__device__ __forceinline__ void stage_run_tile(const std::uint64_t* keys, const V* values, std::int64_t tile_start,
+                                               int count, std::uint64_t* s_keys, V* s_values) {
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int local = i * kScanThreads + static_cast<int>(threadIdx.x);
+        if (local < count) {
+            s_keys[run_slot(local)] = keys[tile_start + local];
+            if constexpr (!Packed) s_values[run_slot(local)] = values[tile_start + local];
+        }
     }
-};
+    __syncthreads();
+}
+
+// This lane's kScanItems entries as (head, weight) items: an entry heads a
+// segment when its (source, target) pair differs from the entry before it.
+// Returns the lanes' combined item; pairs[] receives the entries' pairs.
+template <typename V, bool Packed>
+__device__ __forceinline__ RunValue run_items(const std::uint64_t* keys, std::int64_t tile_start, int count,
+                                              const std::uint64_t* s_keys, const V* s_values, int shift,
+                                              std::uint64_t weight_mask, RunValue (&items)[kScanItems],
+                                              std::uint64_t (&pairs)[kScanItems]) {
+    const int first = static_cast<int>(threadIdx.x) * kScanItems;
+    bool has_prev = first > 0 || tile_start > 0;
+    std::uint64_t prev = 0;
+    if (first > 0) {
+        if (first - 1 < count) prev = s_keys[run_slot(first - 1)] >> shift;
+    } else if (tile_start > 0) {
+        prev = keys[tile_start - 1] >> shift;
+    }
+    RunValue acc = RunOp{}.identity();
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int local = first + i;
+        if (local < count) {
+            const std::uint64_t k = s_keys[run_slot(local)];
+            const std::uint64_t pair = k >> shift;
+            const bool head = !has_prev || pair != prev;
+            const std::uint64_t w = Packed ? (k & weight_mask) : static_cast<std::uint64_t>(s_values[run_slot(local)]);
+            items[i] = RunValue{head ? 1ULL : 0ULL, w, head ? 1U : 0U, 0};
+            pairs[i] = pair;
+            prev = pair;
+            has_prev = true;
+        } else {
+            items[i] = RunOp{}.identity();
+            pairs[i] = 0;
+        }
+        acc = RunOp{}(acc, items[i]);
+    }
+    return acc;
+}
+
+// Pass 1: one (segment count, open-segment sum, any head) total per tile.
+template <typename V, bool Packed>
+__global__ void __launch_bounds__(kScanThreads)
+    run_reduce_kernel(const std::uint64_t* keys, const V* values, std::int64_t n, int shift,
+                      std::uint64_t weight_mask, RunValue* partials) {
+    __shared__ std::uint64_t s_keys[kRunStage];
+    __shared__ V s_values[Packed ? 1 : kRunStage];
+    __shared__ RunValue shared_waves[kScanThreads / kWave + 1];
+    const std::int64_t tile_start = static_cast<std::int64_t>(blockIdx.x) * kScanTile;
+    const int count = static_cast<int>(std::min<std::int64_t>(kScanTile, n - tile_start));
+    stage_run_tile<V, Packed>(keys, values, tile_start, count, s_keys, s_values);
+    RunValue items[kScanItems];
+    std::uint64_t pairs[kScanItems];
+    const RunValue acc =
+        run_items<V, Packed>(keys, tile_start, count, s_keys, s_values, shift, weight_mask, items, pairs);
+    RunValue total;
+    (void)block_exclusive_scan(acc, RunOp{}, &total, shared_waves);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Pass 3: the segmented scan of each tile with its carried-in prefix. Every
+// segment end j = (segments up to it) - 1 gets (source, target, weight sum); the
+// tile's ends form one contiguous range of j, staged in LDS and written out lane
+// after lane. partial_prefix holds either the exclusive prefixes of the tile
+// totals or (tile_totals) the totals themselves, reduced below the tile here.
+template <typename V, typename W, bool Packed>
+__global__ void __launch_bounds__(kScanThreads)
+    run_emit_kernel(const std::uint64_t* keys, const V* values, std::int64_t n, int shift, std::uint64_t weight_mask,
+                    std::uint64_t vertex_mask, int vertex_bits, const RunValue* partial_prefix, int tile_totals,
+                    std::uint32_t* sources, std::int32_t* neighbors, W* weights, unsigned long long* segments) {
+    __shared__ std::uint64_t s_keys[kRunStage];
+    __shared__ V s_values[Packed ? 1 : kRunStage];
+    __shared__ std::uint64_t s_ends[kScanTile + 1];  // source << 32 | target
+    __shared__ W s_sums[kScanTile + 1];
+    __shared__ RunValue shared_waves[kScanThreads / kWave + 1];
+    __shared__ int s_first_head, s_last_end;
+    const std::int64_t tile_start = static_cast<std::int64_t>(blockIdx.x) * kScanTile;
+    const int count = static_cast<int>(std::min<std::int64_t>(kScanTile, n - tile_start));
+    stage_run_tile<V, Packed>(keys, values, tile_start, count, s_keys, s_values);
+    RunValue items[kScanItems];
+    std::uint64_t pairs[kScanItems];
+    const RunValue acc =
+        run_items<V, Packed>(keys, tile_start, count, s_keys, s_values, shift, weight_mask, items, pairs);
+    const int first = static_cast<int>(threadIdx.x) * kScanItems;
+    unsigned last_mask = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int local = first + i;
+        if (local >= count) continue;
+        bool last = tile_start + local == n - 1;
+        if (!last) {
+            const std::uint64_t next = local + 1 < count ? s_keys[run_slot(local + 1)] >> shift
+                                                         : keys[tile_start + local + 1] >> shift;
+            last = next != pairs[i];
+        }
+        if (last) last_mask |= 1U << i;
+        if (local == count - 1) s_last_end = last ? 1 : 0;
+    }
+    if (threadIdx.x == 0) s_first_head = static_cast<int>(items[0].head);
+    RunValue carry = RunOp{}.identity();
+    if (partial_prefix && tile_totals) {
+        const int below = static_cast<int>(blockIdx.x);
+        const int per = (below + kScanThreads - 1) / kScanThreads;
+        RunValue part = RunOp{}.identity();
+        for (int i = 0; i < per; ++i) {
+            const int idx = static_cast<int>(threadIdx.x) * per + i;
+            if (idx < below) part = RunOp{}(part, partial_prefix[idx]);
+        }
+        (void)block_exclusive_scan(part, RunOp{}, &carry, shared_waves);
+    } else if (partial_prefix) {
+        carry = partial_prefix[blockIdx.x];
+    }
+    RunValue tile_total;
+    RunValue prefix = RunOp{}(carry, block_exclusive_scan(acc, RunOp{}, &tile_total, shared_waves));
+    // local slot 0 is segment end j = carry.heads - 1 (the segment open at the tile start)
+    const std::int64_t base_j = static_cast<std::int64_t>(carry.heads) - 1;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int local = first + i;
+        if (local >= count) continue;
+        const RunValue incl = RunOp{}(prefix, items[i]);
+        if (tile_start + local == n - 1) *segments = incl.heads;
+        if (last_mask & (1U << i)) {
+            const int slot = static_cast<int>(static_cast<std::int64_t>(incl.heads) - 1 - base_j);
+            s_ends[slot] = ((pairs[i] >> vertex_bits) << 32) | (pairs[i] & vertex_mask);
+            s_sums[slot] = static_cast<W>(incl.value);
+        }
+        prefix = incl;
+    }
+    __syncthreads();
+    const int lo = s_first_head ? 1 : 0;
+    const int ends = static_cast<int>(tile_total.heads) + (1 - lo) - (s_last_end ? 0 : 1);
+    for (int x = static_cast<int>(threadIdx.x); x < ends; x += kScanThreads) {
+        const std::int64_t j = base_j + lo + x;
+        const std::uint64_t e = s_ends[lo + x];
+        sources[j] = static_cast<std::uint32_t>(e >> 32);
+        neighbors[j] = static_cast<std::int32_t>(static_cast<std::uint32_t>(e));
+        weights[j] = s_sums[lo + x];
+    }
+}
+
+// Segment count and (source, target, summed weight) of every segment of the n
+// sorted keys: reduce, scan of the tile totals (inline up to kScanInlineTiles),
+// emit.
+template <typename V, typename W, bool Packed>
+void run_segments(gpart_ctx* ctx, const Layout& layout, const std::uint64_t* keys, const V* values, std::int64_t n,
+                  std::uint32_t* sources, std::int32_t* neighbors, W* weights, unsigned long long* segments) {
+    const std::int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    const int shift = layout.weight_bits;
+    Scratch<RunValue> partials;
+    int inline_totals = 0;
+    if (tiles > 1) {
+        partials.reset(ctx, tiles);
+        hipLaunchKernelGGL((run_reduce_kernel<V, Packed>), dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0,
+                           ctx->stream, keys, values, n, shift, layout.weight_mask, partials.get());
+        GPART_LAUNCH_CHECK();
+        inline_totals = tiles <= kScanInlineTiles ? 1 : 0;
+        if (!inline_totals)
+            device_scan<RunValue, RunOp>(ctx, ArrayLoad<RunValue>{partials.get()},
+                                         ExclusiveStore<RunValue>{partials.get()}, tiles, RunOp{});
+    }
+    hipLaunchKernelGGL((run_emit_kernel<V, W, Packed>), dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0,
+                       ctx->stream, keys, values, n, shift, layout.weight_mask, layout.vertex_mask,
+                       layout.vertex_bits, static_cast<const RunValue*>(partials.get()), inline_totals, sources,
+                       neighbors, weights, segments);
+    GPART_LAUNCH_CHECK();
+}
 
 template <typename O>
 __global__ void row_offsets_kernel(std::int64_t m, const std::uint32_t* sources, std::int64_t n_coarse, O* offsets) {
@@ -271,11 +420,12 @@ void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degre
     Scratch<unsigned long long> segments(ctx, 1);
     fill_bytes(ctx, segments.get(), 0, 8);
     if (cross > 0) {
-        device_scan<RunValue, RunOp>(
-            ctx, RunLoad<V>{sorted_keys, sorted_values, shift, layout.weight_mask, layout.weight_bits != 0},
-            RunStore<W>{sorted_keys, cross, shift, layout.vertex_mask, layout.vertex_bits, sources.get(),
-                        tmp_nbr.get(), tmp_w.get(), segments.get()},
-            cross, RunOp{});
+        if (layout.weight_bits)
+            run_segments<V, W, true>(ctx, layout, sorted_keys, sorted_values, cross, sources.get(), tmp_nbr.get(),
+                                     tmp_w.get(), segments.get());
+        else
+            run_segments<V, W, false>(ctx, layout, sorted_keys, sorted_values, cross, sources.get(), tmp_nbr.get(),
+                                      tmp_w.get(), segments.get());
     }
     const auto coarse_m = static_cast<std::int64_t>(read_scalar(ctx, segments.get()));
     std::int32_t* nbr = nullptr;
