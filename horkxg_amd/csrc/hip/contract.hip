@@ -1,7 +1,7 @@
 // contract.hip -- CSR contraction for gfx950 (contract_typed, src/coarsen.cu:1674-1814).
 //
-// 1. encode: every cross entry (map[v] != map[u]) becomes a record, appended with
-//    one atomic per wave (record order is irrelevant: the sort defines it and
+// 1. encode: every cross entry (map[v] != map[u]) becomes a record in its row's
+//    range (record order inside the range is irrelevant: the sort defines it and
 //    duplicate keys are summed exactly);
 // 2. LSD radix sort of the records by (source, target): 8-byte keys only in the
 //    packed layout (weight in the low bits), key + value otherwise;
@@ -18,6 +18,10 @@ namespace gpart_hip {
 namespace {
 
 constexpr int kBlock = 256;
+// Row entries a lane loads per step in the count and encode passes: groups of 8
+// lanes serve rows of at most 8 entries, wider groups take 4 per lane.
+template <int G>
+constexpr int entries_per_lane() { return G >= kWave ? 4 : 1; }
 
 struct Layout {
     int vertex_bits;
@@ -55,8 +59,17 @@ __global__ void __launch_bounds__(kBlock) cross_count_kernel(std::int64_t count,
     unsigned long long c = 0;
     if (active) {
         const std::int32_t source = map[v];
-        for (std::int64_t e = static_cast<std::int64_t>(offsets[v]) + sub; e < offsets[v + 1]; e += G)
-            c += map[neighbors[e]] != source;
+        const std::int64_t end = static_cast<std::int64_t>(offsets[v + 1]);
+        // independent neighbour loads, then their map gathers
+        constexpr int kEntriesPerLane = entries_per_lane<G>();
+        for (std::int64_t e0 = static_cast<std::int64_t>(offsets[v]) + sub; e0 < end; e0 += G * kEntriesPerLane) {
+            std::int32_t nb[kEntriesPerLane];
+#pragma unroll
+            for (int u = 0; u < kEntriesPerLane; ++u) nb[u] = e0 + u * G < end ? neighbors[e0 + u * G] : 0;
+#pragma unroll
+            for (int u = 0; u < kEntriesPerLane; ++u)
+                if (e0 + u * G < end) c += map[nb[u]] != source;
+        }
     }
     if constexpr (G <= kWave) {
         c = group_sum_u64<G>(c);
@@ -73,13 +86,17 @@ __global__ void __launch_bounds__(kBlock) cross_count_kernel(std::int64_t count,
     }
 }
 
-// Pass 2: write each vertex's cross records at base[v] + its rank among them.
+// Pass 2: write each vertex's cross records into [base[v], base[v] + cross[v]).
+// Their order inside a row is free (the sort that follows orders them, and
+// records with equal keys are summed exactly): groups of at most a wave place
+// them by ballot rank; a workgroup's waves (G == kBlock) reserve their slots
+// with one LDS atomic per wave and step, without workgroup barriers.
 template <typename O, typename W, typename V, int G>
 __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
                                                         const std::int32_t* neighbors, const W* weights,
                                                         const std::int32_t* map, const std::int64_t* base,
                                                         Layout layout, std::uint64_t* keys, V* values) {
-    __shared__ unsigned int wave_counts[kBlock / kWave];
+    __shared__ unsigned long long s_fill;
     const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
     const int sub = threadIdx.x & (G - 1);
     const bool active = group < count;
@@ -89,53 +106,74 @@ __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, cons
     const std::uint64_t source = active ? static_cast<std::uint32_t>(map[v]) : 0;
     std::int64_t next = active ? base[v] : 0;
     const int lane = lane_id();
-    const int wave = threadIdx.x / kWave;
-    // group-uniform trip count (a workgroup-wide maximum for G == kBlock)
-    const std::int64_t trips = end - begin > 0 ? (end - begin + G - 1) / G : 0;
+    if constexpr (G > kWave) {
+        if (threadIdx.x == 0) s_fill = 0;
+        __syncthreads();
+    }
+    constexpr int kEntriesPerLane = entries_per_lane<G>();
+    constexpr int kStep = G * kEntriesPerLane;
+    // group-uniform trip count
+    const std::int64_t trips = end - begin > 0 ? (end - begin + kStep - 1) / kStep : 0;
     for (std::int64_t it = 0; it < trips; ++it) {
-        const std::int64_t e = begin + sub + it * G;
-        bool emit = false;
-        std::uint64_t key = 0;
-        W w = 0;
-        if (e < end) {
-            const std::uint64_t target = static_cast<std::uint32_t>(map[neighbors[e]]);
-            if (target != source) {
-                emit = true;
-                w = weights[e];
-                key = layout.weight_bits
-                          ? (((source << layout.vertex_bits) | target) << layout.weight_bits) |
-                                static_cast<std::uint64_t>(w)
-                          : (source << layout.vertex_bits) | target;
+        const std::int64_t e0 = begin + sub + it * kStep;
+        std::int32_t nb[kEntriesPerLane];
+        W w[kEntriesPerLane];
+#pragma unroll
+        for (int u = 0; u < kEntriesPerLane; ++u) {
+            const std::int64_t e = e0 + u * G;
+            nb[u] = e < end ? neighbors[e] : 0;
+            w[u] = e < end ? weights[e] : W(0);
+        }
+        bool emit[kEntriesPerLane];
+        std::uint64_t key[kEntriesPerLane];
+        std::uint64_t mask[kEntriesPerLane];
+#pragma unroll
+        for (int u = 0; u < kEntriesPerLane; ++u) {
+            emit[u] = false;
+            key[u] = 0;
+            if (e0 + u * G < end) {
+                const std::uint64_t target = static_cast<std::uint32_t>(map[nb[u]]);
+                if (target != source) {
+                    emit[u] = true;
+                    key[u] = layout.weight_bits ? (((source << layout.vertex_bits) | target) << layout.weight_bits) |
+                                                      static_cast<std::uint64_t>(w[u])
+                                                : (source << layout.vertex_bits) | target;
+                }
             }
         }
-        const std::uint64_t mask = __ballot(emit);
-        std::int64_t rank;
+#pragma unroll
+        for (int u = 0; u < kEntriesPerLane; ++u) mask[u] = __ballot(emit[u]);
         if constexpr (G <= kWave) {
             const int first = lane & ~(G - 1);
             const std::uint64_t group_bits = G == kWave ? ~0ULL : (((1ULL << G) - 1) << first);
-            const std::uint64_t gmask = mask & group_bits;
-            rank = __popcll(gmask & lanemask_lt());
-            if (emit) {
-                keys[next + rank] = key;
-                if (layout.weight_bits == 0) values[next + rank] = static_cast<V>(w);
+#pragma unroll
+            for (int u = 0; u < kEntriesPerLane; ++u) {
+                const std::uint64_t gmask = mask[u] & group_bits;
+                if (emit[u]) {
+                    const std::int64_t at = next + __popcll(gmask & lanemask_lt());
+                    keys[at] = key[u];
+                    if (layout.weight_bits == 0) values[at] = static_cast<V>(w[u]);
+                }
+                next += __popcll(gmask);
             }
-            next += __popcll(gmask);
         } else {
-            if (lane == 0) wave_counts[wave] = static_cast<unsigned int>(__popcll(mask));
-            __syncthreads();
-            std::int64_t before = 0, total = 0;
-            for (int w2 = 0; w2 < kBlock / kWave; ++w2) {
-                const unsigned int c = wave_counts[w2];
-                before += w2 < wave ? c : 0;
-                total += c;
+            unsigned long long wave_total = 0;
+#pragma unroll
+            for (int u = 0; u < kEntriesPerLane; ++u) wave_total += __popcll(mask[u]);
+            unsigned long long start = 0;
+            if (lane == 0 && wave_total) start = atomicAdd(&s_fill, wave_total);
+            const unsigned lo = __shfl(static_cast<unsigned>(start), 0, kWave);
+            const unsigned hi = __shfl(static_cast<unsigned>(start >> 32), 0, kWave);
+            std::int64_t at_wave = next + static_cast<std::int64_t>((static_cast<unsigned long long>(hi) << 32) | lo);
+#pragma unroll
+            for (int u = 0; u < kEntriesPerLane; ++u) {
+                if (emit[u]) {
+                    const std::int64_t at = at_wave + __popcll(mask[u] & lanemask_lt());
+                    keys[at] = key[u];
+                    if (layout.weight_bits == 0) values[at] = static_cast<V>(w[u]);
+                }
+                at_wave += __popcll(mask[u]);
             }
-            rank = before + __popcll(mask & lanemask_lt());
-            if (emit) {
-                keys[next + rank] = key;
-                if (layout.weight_bits == 0) values[next + rank] = static_cast<V>(w);
-            }
-            next += total;
-            __syncthreads();
         }
     }
 }

@@ -49,6 +49,8 @@ GRAPHS = {
     "grid64": lambda hx: hx.generate_grid2d(64, 64),
     "grid256": lambda hx: hx.generate_grid2d(256, 256),
     "rmat12": lambda hx: hx.generate_rmat(12, 16),
+    # 3.6M cross entries after one aggregation: more than 1024 run-scan tiles
+    "rmat17": lambda hx: hx.generate_rmat(17, 16),
     "hub": lambda hx: hub_graph(),
     "bighub": lambda hx: hub_graph(60000, (45000, 20000, 9000, 5000, 1500, 700), seed=3),
     # rows above 524288 entries: the per-hub global tables outside singleton rounds;
@@ -169,7 +171,7 @@ def test_sclp_partition_overflow_splits(hx, oracle, bound):
         assert np.array_equal(p, op) and np.array_equal(g, ogn) and np.array_equal(t, ot), rnd
 
 
-@pytest.mark.parametrize("name", ["grid64", "rmat12", "hub"])
+@pytest.mark.parametrize("name", ["grid64", "rmat12", "hub", "rmat17"])
 @pytest.mark.parametrize("width", ["standard", "big"])
 def test_contract_matches_oracle(hx, oracle, name, width):
     say(f"case contract graph={name} width={width}")
