@@ -41,6 +41,7 @@ from ._native import (  # noqa: F401
     set_refine_engine,
     set_refine_graphs,
     set_refine_balance_blocks,
+    set_contract_stash,
     refine_graph_stats,
     set_spectral_queue,
     spectral_queue_settings,

@@ -139,6 +139,7 @@ def hip_lib():
         H.gpart_spectral_set_queue.argtypes = [C.c_int32]
         H.gpart_refine_set_graphs.argtypes = [C.c_void_p, C.c_int]
         H.gpart_refine_set_balance_blocks.argtypes = [C.c_void_p, C.c_int64]
+        H.gpart_contract_set_stash.argtypes = [C.c_void_p, C.c_int64]
         H.gpart_refine_graph_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64)]
         H.gpart_spectral_queue_set_tuning.argtypes = [C.c_int32, C.c_int32]
@@ -334,6 +335,13 @@ def set_refine_graphs(enabled: bool = True) -> None:
 def set_refine_balance_blocks(blocks: int) -> None:
     """The calling thread's cap on the rebalance proposals grid (result-neutral)."""
     _check(hip_lib().gpart_refine_set_balance_blocks(_ctx(), int(blocks)))
+
+
+def set_contract_stash(min_vertices: int) -> None:
+    """The calling thread's contraction threshold: levels of at least min_vertices
+    fine vertices keep every entry's coarse neighbour between the count and encode
+    passes (result-neutral; default 2^24)."""
+    _check(hip_lib().gpart_contract_set_stash(_ctx(), int(min_vertices)))
 
 
 def refine_graph_stats() -> dict:

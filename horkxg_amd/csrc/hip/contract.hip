@@ -50,7 +50,8 @@ Layout choose_layout(std::int64_t vertices, std::uint64_t bound) {
 template <typename O, int G>
 __global__ void __launch_bounds__(kBlock) cross_count_kernel(std::int64_t count, const std::int32_t* list,
                                                              const O* offsets, const std::int32_t* neighbors,
-                                                             const std::int32_t* map, std::int64_t* cross) {
+                                                             const std::int32_t* map, std::int64_t* cross,
+                                                             std::int32_t* targets) {
     __shared__ unsigned long long waves[kBlock / kWave];
     const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
     const int sub = threadIdx.x & (G - 1);
@@ -67,8 +68,13 @@ __global__ void __launch_bounds__(kBlock) cross_count_kernel(std::int64_t count,
 #pragma unroll
             for (int u = 0; u < kEntriesPerLane; ++u) nb[u] = e0 + u * G < end ? neighbors[e0 + u * G] : 0;
 #pragma unroll
-            for (int u = 0; u < kEntriesPerLane; ++u)
-                if (e0 + u * G < end) c += map[nb[u]] != source;
+            for (int u = 0; u < kEntriesPerLane; ++u) {
+                if (e0 + u * G < end) {
+                    const std::int32_t target = map[nb[u]];
+                    if (targets) targets[e0 + u * G] = target;
+                    c += target != source;
+                }
+            }
         }
     }
     if constexpr (G <= kWave) {
@@ -94,8 +100,9 @@ __global__ void __launch_bounds__(kBlock) cross_count_kernel(std::int64_t count,
 template <typename O, typename W, typename V, int G>
 __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, const std::int32_t* list, const O* offsets,
                                                         const std::int32_t* neighbors, const W* weights,
-                                                        const std::int32_t* map, const std::int64_t* base,
-                                                        Layout layout, std::uint64_t* keys, V* values) {
+                                                        const std::int32_t* map, const std::int32_t* targets,
+                                                        const std::int64_t* base, Layout layout, std::uint64_t* keys,
+                                                        V* values) {
     __shared__ unsigned long long s_fill;
     const std::int64_t group = (static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
     const int sub = threadIdx.x & (G - 1);
@@ -121,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, cons
 #pragma unroll
         for (int u = 0; u < kEntriesPerLane; ++u) {
             const std::int64_t e = e0 + u * G;
-            nb[u] = e < end ? neighbors[e] : 0;
+            nb[u] = e < end ? (targets ? targets[e] : neighbors[e]) : 0;
             w[u] = e < end ? weights[e] : W(0);
         }
         bool emit[kEntriesPerLane];
@@ -132,7 +139,7 @@ __global__ void __launch_bounds__(kBlock) encode_kernel(std::int64_t count, cons
             emit[u] = false;
             key[u] = 0;
             if (e0 + u * G < end) {
-                const std::uint64_t target = static_cast<std::uint32_t>(map[nb[u]]);
+                const std::uint64_t target = static_cast<std::uint32_t>(targets ? nb[u] : map[nb[u]]);
                 if (target != source) {
                     emit[u] = true;
                     key[u] = layout.weight_bits ? (((source << layout.vertex_bits) | target) << layout.weight_bits) |
@@ -408,32 +415,40 @@ void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degre
     }
     const std::int64_t n_fine = fine->n;
     Scratch<std::int64_t> base(ctx, n_fine + 1);
+    // From 2^24 fine vertices on (the default of ctx->contract_stash_vertices), map
+    // (64 MB and up) no longer stays in the caches and its random gathers dominate
+    // both passes: the count pass then stores every entry's coarse neighbour and
+    // the encode pass reads it sequentially.
+    Scratch<std::int32_t> targets;
+    if (m > 0 && n_fine >= ctx->contract_stash_vertices) targets.reset(ctx, m);
     fill_bytes(ctx, base.get() + n_fine, 0, 8);
     if (classes->low_count)
         hipLaunchKernelGGL((cross_count_kernel<O, 8>), dim3(grid_for(classes->low_count * 8)), dim3(kBlock), 0,
-                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, map, base.get());
+                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, map, base.get(),
+                           targets.get());
     if (classes->medium_count)
         hipLaunchKernelGGL((cross_count_kernel<O, 64>), dim3(grid_for(classes->medium_count * 64)), dim3(kBlock), 0,
-                           ctx->stream, classes->medium_count, classes->medium, off, fine->neighbors, map, base.get());
+                           ctx->stream, classes->medium_count, classes->medium, off, fine->neighbors, map, base.get(),
+                           targets.get());
     if (classes->high_count)
         hipLaunchKernelGGL((cross_count_kernel<O, kBlock>), dim3(static_cast<unsigned>(classes->high_count)),
                            dim3(kBlock), 0, ctx->stream, classes->high_count, classes->high, off, fine->neighbors, map,
-                           base.get());
+                           base.get(), targets.get());
     GPART_LAUNCH_CHECK();
     // exclusive scan over n + 1 entries: base[n] becomes the cross-entry total
     exclusive_scan<std::int64_t>(ctx, base.get(), base.get(), n_fine + 1);
     if (classes->low_count)
         hipLaunchKernelGGL((encode_kernel<O, W, V, 8>), dim3(grid_for(classes->low_count * 8)), dim3(kBlock), 0,
-                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, ew, map, base.get(),
-                           layout, keys.get(), values.get());
+                           ctx->stream, classes->low_count, classes->low, off, fine->neighbors, ew, map, targets.get(),
+                           base.get(), layout, keys.get(), values.get());
     if (classes->medium_count)
         hipLaunchKernelGGL((encode_kernel<O, W, V, 64>), dim3(grid_for(classes->medium_count * 64)), dim3(kBlock), 0,
                            ctx->stream, classes->medium_count, classes->medium, off, fine->neighbors, ew, map,
-                           base.get(), layout, keys.get(), values.get());
+                           targets.get(), base.get(), layout, keys.get(), values.get());
     if (classes->high_count)
         hipLaunchKernelGGL((encode_kernel<O, W, V, kBlock>), dim3(static_cast<unsigned>(classes->high_count)),
                            dim3(kBlock), 0, ctx->stream, classes->high_count, classes->high, off, fine->neighbors, ew,
-                           map, base.get(), layout, keys.get(), values.get());
+                           map, targets.get(), base.get(), layout, keys.get(), values.get());
     GPART_LAUNCH_CHECK();
     const auto cross = read_scalar(ctx, base.get() + n_fine);
     const int shift = layout.weight_bits;
@@ -501,6 +516,13 @@ extern "C" void gpart_record_layout(int64_t vertices, uint64_t bound, int* verte
     *vertex_bits = l.weight_bits ? l.vertex_bits : 32;
     *weight_bits = l.weight_bits;
     *weight_mask = l.weight_mask;
+}
+
+extern "C" int gpart_contract_set_stash(gpart_ctx* ctx, int64_t min_vertices) {
+    return guarded([&] {
+        if (min_vertices < 0) throw Error(GPART_ERR_INVALID_ARGUMENT, "negative vertex count");
+        ctx->contract_stash_vertices = min_vertices;
+    });
 }
 
 extern "C" int gpart_contract(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degree_classes* classes,

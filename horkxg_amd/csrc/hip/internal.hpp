@@ -76,6 +76,9 @@ struct gpart_ctx {
     // Workgroups of the rebalance proposals launch at most (each flushes its
     // bucket table into the global counters once).
     std::int64_t balance_blocks = gpart_tuned::kBalanceBlocks;
+    // contraction: levels of at least this many fine vertices stash every entry's
+    // coarse neighbour in the count pass (gpart_contract_set_stash)
+    std::int64_t contract_stash_vertices = std::int64_t{1} << 24;
     // The rebalance's bucket-counter replicas, kept between rounds: the bucket
     // scans zero every counter they read, so a completed round leaves them zero.
     void* rebalance_counters = nullptr;

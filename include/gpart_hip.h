@@ -309,6 +309,12 @@ int gpart_contract(gpart_ctx* ctx, const gpart_csr_view* fine,
                    const gpart_degree_classes* classes, const int32_t* map,
                    int64_t n_coarse, uint64_t coarse_row_bound, void** coarse_offsets,
                    int32_t** coarse_neighbors, void** coarse_edge_weights, int64_t* coarse_m);
+/* Contraction tuning of a context (no counterpart in the reference): levels of at
+ * least min_vertices fine vertices store every entry's coarse neighbour in the
+ * count pass and read it back in the encode pass instead of gathering map twice
+ * (default 2^24, where map outgrows the caches; 0 always, INT64_MAX never).
+ * Results are identical either way. */
+int gpart_contract_set_stash(gpart_ctx* ctx, int64_t min_vertices);
 
 /* The contraction record layout for n vertices and a row-weight bound: vertex bits,
  * weight bits (0 = unpacked 8-byte key + separate value) and weight mask

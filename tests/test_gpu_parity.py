@@ -181,11 +181,18 @@ def test_contract_matches_oracle(hx, oracle, name, width):
     mp, cw, coarse, cap, _ = oracle.aggregate(og, 4, 0, 0)
     expect = oracle.contract(og, mp, coarse, cw).arrays()
     dg = hx.device.DeviceGraph(indptr, indices, width=width)
-    for bound in (64, 2**64 - 1):  # packed and unpacked layouts
-        off, nbr, w = hx.device.contract(dg, mp, coarse, bound)
-        assert np.array_equal(off, expect[0])
-        assert np.array_equal(nbr, expect[1])
-        assert np.array_equal(w, expect[2])
+    # the coarse neighbours gathered twice (default at these sizes) and stashed
+    # by the count pass for the encode pass (threshold 0)
+    for stash in (1 << 24, 0):
+        hx.set_contract_stash(stash)
+        try:
+            for bound in (64, 2**64 - 1):  # packed and unpacked layouts
+                off, nbr, w = hx.device.contract(dg, mp, coarse, bound)
+                assert np.array_equal(off, expect[0]), (stash, bound)
+                assert np.array_equal(nbr, expect[1]), (stash, bound)
+                assert np.array_equal(w, expect[2]), (stash, bound)
+        finally:
+            hx.set_contract_stash(1 << 24)
 
 
 @pytest.mark.parametrize("scale", ["small", "saturating"])
