@@ -397,11 +397,12 @@ __global__ void row_offsets_kernel(std::int64_t m, const std::uint32_t* sources,
 
 inline unsigned grid_for(std::int64_t threads) { return static_cast<unsigned>((threads + kBlock - 1) / kBlock); }
 
-template <typename O, typename W>
+// V: the record value type of the unpacked layout, 4 bytes whenever the row bound
+// (which bounds every entry's weight and every summed coarse weight) fits them.
+template <typename O, typename W, typename V>
 void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degree_classes* classes,
                    const std::int32_t* map, std::int64_t n_coarse, std::uint64_t bound, void** out_offsets,
                    std::int32_t** out_neighbors, void** out_weights, std::int64_t* out_m) {
-    using V = W;  // unpacked value type follows the graph's weight type
     const Layout layout = choose_layout(n_coarse, bound);
     const std::int64_t m = fine->m;
     const auto* off = static_cast<const O*>(fine->offsets);
@@ -496,7 +497,7 @@ void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degre
     const double n = static_cast<double>(fine->n), md = static_cast<double>(m), mc = static_cast<double>(coarse_m);
     const double bytes = (n + 1) * sizeof(O) + md * (4 + sizeof(W)) + 4 * md + 4 * n + 32 * md + 16 * mc +
                          mc * (4 + sizeof(W)) + (n_coarse + 1.0) * sizeof(O) +
-                         (layout.weight_bits ? 0.0 : 2.0 * sizeof(W) * md);
+                         (layout.weight_bits ? 0.0 : 2.0 * sizeof(V) * md);
     profile_end(ctx, GPART_PROFILE_CONTRACT_SORT, bytes);
     *out_offsets = offsets;
     *out_neighbors = nbr;
@@ -532,12 +533,16 @@ extern "C" int gpart_contract(gpart_ctx* ctx, const gpart_csr_view* fine, const 
         if (n_coarse <= 0 || n_coarse > 0x7fffffff)
             throw Error(GPART_ERR_INVALID_ARGUMENT, "invalid coarse vertex count");
         if (fine->width == GPART_WIDTH_STANDARD)
-            contract_impl<std::int32_t, std::uint32_t>(ctx, fine, classes, map, n_coarse, coarse_row_bound,
-                                                       coarse_offsets, coarse_neighbors, coarse_edge_weights,
-                                                       coarse_m);
+            contract_impl<std::int32_t, std::uint32_t, std::uint32_t>(ctx, fine, classes, map, n_coarse,
+                                                                      coarse_row_bound, coarse_offsets,
+                                                                      coarse_neighbors, coarse_edge_weights, coarse_m);
+        else if (coarse_row_bound <= 0xffffffffULL)
+            contract_impl<std::int64_t, std::uint64_t, std::uint32_t>(ctx, fine, classes, map, n_coarse,
+                                                                      coarse_row_bound, coarse_offsets,
+                                                                      coarse_neighbors, coarse_edge_weights, coarse_m);
         else
-            contract_impl<std::int64_t, std::uint64_t>(ctx, fine, classes, map, n_coarse, coarse_row_bound,
-                                                       coarse_offsets, coarse_neighbors, coarse_edge_weights,
-                                                       coarse_m);
+            contract_impl<std::int64_t, std::uint64_t, std::uint64_t>(ctx, fine, classes, map, n_coarse,
+                                                                      coarse_row_bound, coarse_offsets,
+                                                                      coarse_neighbors, coarse_edge_weights, coarse_m);
     });
 }

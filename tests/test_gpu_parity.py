@@ -186,7 +186,9 @@ def test_contract_matches_oracle(hx, oracle, name, width):
     for stash in (1 << 24, 0):
         hx.set_contract_stash(stash)
         try:
-            for bound in (64, 2**64 - 1):  # packed and unpacked layouts
+            # packed; unpacked with 4-byte values once the bound outgrows the key
+            # (rmat17: 30 weight bits); unpacked with the weight type's values
+            for bound in (64, 2**32 - 1, 2**64 - 1):
                 off, nbr, w = hx.device.contract(dg, mp, coarse, bound)
                 assert np.array_equal(off, expect[0]), (stash, bound)
                 assert np.array_equal(nbr, expect[1]), (stash, bound)
