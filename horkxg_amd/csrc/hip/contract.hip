@@ -4,12 +4,15 @@
 //    range (record order inside the range is irrelevant: the sort defines it and
 //    duplicate keys are summed exactly);
 // 2. LSD radix sort of the records by (source, target): 8-byte keys only in the
-//    packed layout (weight in the low bits), key + value otherwise;
+//    packed layout (weight in the low bits: the row bound fits them, or else the
+//    largest entry weight does), key + value otherwise (4-byte values whenever
+//    the row bound fits them);
 // 3. one scan fuses segment counting and segmented weight sums and emits each
 //    (source, target, sum) at its segment end, staged per tile for coalesced
 //    stores;
 // 4. row offsets are written once per row from row boundaries.
 #include "internal.hpp"
+#include "reduce.hpp"
 #include "scan.hpp"
 
 #include <algorithm>
@@ -403,11 +406,23 @@ template <typename O, typename W, typename V>
 void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degree_classes* classes,
                    const std::int32_t* map, std::int64_t n_coarse, std::uint64_t bound, void** out_offsets,
                    std::int32_t** out_neighbors, void** out_weights, std::int64_t* out_m) {
-    const Layout layout = choose_layout(n_coarse, bound);
+    Layout layout = choose_layout(n_coarse, bound);
     const std::int64_t m = fine->m;
     const auto* off = static_cast<const O*>(fine->offsets);
     const auto* ew = static_cast<const W*>(fine->edge_weights);
     profile_begin(ctx, GPART_PROFILE_CONTRACT_SORT);
+    if (!layout.weight_bits && m > 0) {
+        // A packed key holds one fine entry's weight, never a sum (sums are taken
+        // in the run scan): when the row bound rules the packed layout out, it
+        // still fits if the largest entry weight does (8- instead of 12- or
+        // 16-byte records through the sort; the result is the same).
+        const Layout packed = choose_layout(n_coarse, 0);
+        if (packed.weight_bits) {
+            Scratch<unsigned long long> largest(ctx, 1);
+            device_reduce<unsigned long long>(ctx, WidenLoad<W>{ew}, m, MaxU64{}, largest.get());
+            if (read_scalar(ctx, largest.get()) <= packed.weight_mask) layout = packed;
+        }
+    }
     Scratch<std::uint64_t> keys(ctx, std::max<std::int64_t>(m, 1)), keys_alt(ctx, std::max<std::int64_t>(m, 1));
     Scratch<V> values, values_alt;
     if (!layout.weight_bits) {

@@ -186,8 +186,9 @@ def test_contract_matches_oracle(hx, oracle, name, width):
     for stash in (1 << 24, 0):
         hx.set_contract_stash(stash)
         try:
-            # packed; unpacked with 4-byte values once the bound outgrows the key
-            # (rmat17: 30 weight bits); unpacked with the weight type's values
+            # bounds that fit the packed key and bounds that do not (rmat17: 30
+            # weight bits); unit entry weights then still pack (the unpacked
+            # layouts: test_contract_unpacked_heavy_weights)
             for bound in (64, 2**32 - 1, 2**64 - 1):
                 off, nbr, w = hx.device.contract(dg, mp, coarse, bound)
                 assert np.array_equal(off, expect[0]), (stash, bound)
@@ -195,6 +196,29 @@ def test_contract_matches_oracle(hx, oracle, name, width):
                 assert np.array_equal(w, expect[2]), (stash, bound)
         finally:
             hx.set_contract_stash(1 << 24)
+
+
+@pytest.mark.parametrize("bound", [2**32 - 1, 2**64 - 1])
+def test_contract_unpacked_heavy_weights(hx, oracle, bound):
+    """Entry weights above what a packed key holds (rmat17's coarse level leaves
+    30 weight bits) keep the unpacked layout: 4-byte values under a 32-bit row
+    bound, 8-byte values otherwise; more than 1024 run-scan tiles."""
+    say(f"case contract heavy weights bound={bound}")
+    indptr, indices = GRAPHS["rmat17"](hx)
+    n = len(indptr) - 1
+    src = np.repeat(np.arange(n, dtype=np.uint64), np.diff(indptr))
+    lo = np.minimum(src, indices.astype(np.uint64))
+    hi = np.maximum(src, indices.astype(np.uint64))
+    w = (np.uint64(1) << np.uint64(30)) + (lo * np.uint64(2654435761) + hi) % (np.uint64(1) << np.uint64(30))
+    og = oracle.Graph(indptr, indices.astype(np.int32), edge_weights=w)
+    mp, cw, coarse, cap, _ = oracle.aggregate(og, 4, 0, 0)
+    expect = oracle.contract(og, mp, coarse, cw).arrays()
+    assert int(w.max()) >= 1 << 30 and coarse >= 1 << 16  # no packed layout fits
+    dg = hx.device.DeviceGraph(indptr, indices, edge_weights=w, width="big")
+    off, nbr, wt = hx.device.contract(dg, mp, coarse, bound)
+    assert np.array_equal(off, expect[0])
+    assert np.array_equal(nbr, expect[1])
+    assert np.array_equal(wt, expect[2])
 
 
 @pytest.mark.parametrize("scale", ["small", "saturating"])
