@@ -483,40 +483,34 @@ void contract_impl(gpart_ctx* ctx, const gpart_csr_view* fine, const gpart_degre
         }
     }
     // reduce: one scan counts (source, target) segments and sums their weights
+    // The coarse entries are emitted straight into the output arrays, sized to the
+    // cross-entry count (their upper bound), so no copy to exact sizes follows.
     Scratch<std::uint32_t> sources(ctx, std::max<std::int64_t>(cross, 1));
-    Scratch<std::int32_t> tmp_nbr(ctx, std::max<std::int64_t>(cross, 1));
-    Scratch<W> tmp_w(ctx, std::max<std::int64_t>(cross, 1));
+    Scratch<std::int32_t> out_nbr(ctx, std::max<std::int64_t>(cross, 1));
+    Scratch<W> out_w(ctx, std::max<std::int64_t>(cross, 1));
     Scratch<unsigned long long> segments(ctx, 1);
     fill_bytes(ctx, segments.get(), 0, 8);
     if (cross > 0) {
         if (layout.weight_bits)
-            run_segments<V, W, true>(ctx, layout, sorted_keys, sorted_values, cross, sources.get(), tmp_nbr.get(),
-                                     tmp_w.get(), segments.get());
+            run_segments<V, W, true>(ctx, layout, sorted_keys, sorted_values, cross, sources.get(), out_nbr.get(),
+                                     out_w.get(), segments.get());
         else
-            run_segments<V, W, false>(ctx, layout, sorted_keys, sorted_values, cross, sources.get(), tmp_nbr.get(),
-                                      tmp_w.get(), segments.get());
+            run_segments<V, W, false>(ctx, layout, sorted_keys, sorted_values, cross, sources.get(), out_nbr.get(),
+                                      out_w.get(), segments.get());
     }
     const auto coarse_m = static_cast<std::int64_t>(read_scalar(ctx, segments.get()));
-    std::int32_t* nbr = nullptr;
-    W* wts = nullptr;
-    O* offsets = static_cast<O*>(ctx_alloc(ctx, sizeof(O) * static_cast<std::size_t>(n_coarse + 1)));
-    nbr = static_cast<std::int32_t*>(ctx_alloc(ctx, sizeof(std::int32_t) * std::max<std::int64_t>(coarse_m, 1)));
-    wts = static_cast<W*>(ctx_alloc(ctx, sizeof(W) * std::max<std::int64_t>(coarse_m, 1)));
-    if (coarse_m > 0) {
-        copy_device(ctx, nbr, tmp_nbr.get(), sizeof(std::int32_t) * coarse_m);
-        copy_device(ctx, wts, tmp_w.get(), sizeof(W) * coarse_m);
-    }
+    Scratch<O> offsets(ctx, n_coarse + 1);
     hipLaunchKernelGGL(row_offsets_kernel<O>, dim3(grid_for(coarse_m + 1)), dim3(kBlock), 0, ctx->stream, coarse_m,
-                       sources.get(), n_coarse, offsets);
+                       sources.get(), n_coarse, offsets.get());
     GPART_LAUNCH_CHECK();
     const double n = static_cast<double>(fine->n), md = static_cast<double>(m), mc = static_cast<double>(coarse_m);
     const double bytes = (n + 1) * sizeof(O) + md * (4 + sizeof(W)) + 4 * md + 4 * n + 32 * md + 16 * mc +
                          mc * (4 + sizeof(W)) + (n_coarse + 1.0) * sizeof(O) +
                          (layout.weight_bits ? 0.0 : 2.0 * sizeof(V) * md);
     profile_end(ctx, GPART_PROFILE_CONTRACT_SORT, bytes);
-    *out_offsets = offsets;
-    *out_neighbors = nbr;
-    *out_weights = wts;
+    *out_offsets = offsets.take();
+    *out_neighbors = out_nbr.take();
+    *out_weights = out_w.take();
     *out_m = coarse_m;
 }
 

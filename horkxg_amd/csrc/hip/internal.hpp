@@ -159,6 +159,13 @@ public:
     }
     T* get() const { return ptr_; }
     std::int64_t size() const { return count_; }
+    // Hands the buffer to the caller, who frees it with ctx_free.
+    T* take() {
+        T* p = ptr_;
+        ptr_ = nullptr;
+        count_ = 0;
+        return p;
+    }
 
 private:
     gpart_ctx* ctx_ = nullptr;
