@@ -706,7 +706,8 @@ struct OverloadedLoad {
     const std::int32_t* labels;
     const unsigned long long* part_weights;
     std::uint64_t capacity;
-    __device__ __forceinline__ long long operator()(std::int64_t v) const {
+    // 32-bit counts: vertex ids, and so candidate counts, stay below 2^31
+    __device__ __forceinline__ int operator()(std::int64_t v) const {
         return part_weights[labels[v]] > capacity ? 1 : 0;
     }
 };
@@ -715,7 +716,7 @@ struct CandidateStore {
     std::int32_t* flags;
     std::int64_t n;
     RoundState* st;
-    __device__ __forceinline__ void operator()(std::int64_t v, long long prefix, long long item) const {
+    __device__ __forceinline__ void operator()(std::int64_t v, int prefix, int item) const {
         flags[v] = 0;
         if (item) candidates[prefix] = static_cast<std::int32_t>(v);
         if (v == n - 1) st->candidates = prefix + item;
@@ -1644,9 +1645,8 @@ struct Ops {
         const unsigned long long* column_prefix = inline_columns ? nullptr : prefix.get();
 
         run.phase(-1, [&] {
-            device_scan<long long, SumOp<long long>>(ctx, OverloadedLoad{labels, part_weights, capacity},
-                                                     CandidateStore{cand.get(), flags, n, st_ptr}, n,
-                                                     SumOp<long long>{});
+            device_scan<int, SumOp<int>>(ctx, OverloadedLoad{labels, part_weights, capacity},
+                                         CandidateStore{cand.get(), flags, n, st_ptr}, n, SumOp<int>{});
         });
         const int G = group_for_k(k);
         const auto blocks = static_cast<unsigned>(std::min<std::int64_t>(grid_for(n * G), ctx->balance_blocks));
